@@ -1,0 +1,264 @@
+"""Pins the oracle (oracle/oracle.c) before anything is checked against it.
+
+Each check is one that a plausible mistake in the restatement would fail: the reference's own known answers
+(exact products, the subnormal-pivot factors, the notebook's signed zeros, determinants), its invariant tests
+(residual bounds, exact power-of-two scaling, zero columns), a textbook library routine that shares the pivot rule
+(LAPACK getrf through SciPy), a higher-precision product, and equality between the recursive restatement and the
+flat one-column-at-a-time statement of the same arithmetic.
+"""
+import math
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle
+from support import FOLD_SIZES, KC, MC, integer_operands, random, norm, distance, backward_error
+
+EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps}
+
+
+def bits_equal(x, y):
+    """Equal bit for bit, so signed zeros count, with every NaN equal to every other: which operand's NaN an
+    operation passes on, and its sign, is the hardware's choice (x86's default NaN is negative)."""
+    x, y = np.array(x, copy=True), np.array(y, copy=True)
+    if x.shape != y.shape or x.dtype != y.dtype: return False
+    nx, ny = np.isnan(x), np.isnan(y)
+    if not (nx == ny).all(): return False
+    x[nx] = 0; y[ny] = 0
+    return x.tobytes() == y.tobytes()
+
+
+@pytest.mark.parametrize('size', FOLD_SIZES)
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_products_match_folds_in_order(size, dtype):
+    "tests/linalg.rs:160-177: small whole numbers make every sum exact, so the product equals the exact one."
+    m, k, n = size
+    a, b = integer_operands(m, k, n)
+    want = (a.astype(np.int64) @ b.astype(np.int64)).astype(dtype)
+    c = np.ones((m, n), dtype)
+    oracle.gemm(a.astype(dtype), b.astype(dtype), c=c)
+    assert bits_equal(c, want)
+    assert bits_equal(oracle.gemm(a.astype(dtype), b.astype(dtype), naive=True), want)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('ops', ['NN', 'NT', 'TN', 'TT'])
+@pytest.mark.parametrize('ab', [(1.0, 0.0), (-1.0, 1.0), (2.0, 0.5), (0.75, 1.0)])
+def test_blocked_kernels_match_the_in_order_fold(dtype, ops, ab):
+    """The blocked GotoBLAS driver with AVX2 tiles gives the same bits as the defining fold (one fused multiply-add
+    per step of p, in order): KC blocks reload c exactly, so blocking never changes a result."""
+    rng = np.random.default_rng(5)
+    alpha, beta = ab
+    for m, k, n in [(MC + 7, KC * 2 + 3, 19), (13, 300, 2053 + 16), (64, 64, 64), (6, 9, 17)]:
+        a = rng.standard_normal((m, k) if ops[0] == 'N' else (k, m)).astype(dtype)
+        b = rng.standard_normal((k, n) if ops[1] == 'N' else (n, k)).astype(dtype)
+        c0 = rng.standard_normal((m, n)).astype(dtype)
+        got = oracle.gemm(a, b, alpha, beta, c0.copy(), ops[0], ops[1])
+        want = oracle.gemm(a, b, alpha, beta, c0.copy(), ops[0], ops[1], naive=True)
+        assert bits_equal(got, want)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_products_within_the_rounding_bound(dtype):
+    "Each result lies within k·eps·(|A||B|) of the product in extended precision (the error bound of any fold)."
+    rng = np.random.default_rng(0)
+    for m, k, n in [(200, 333, 150), (1, 500, 300), (300, 500, 1), (40, 5, 40), (7, 3, 1), (1, 6, 9)]:
+        a, b = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+        got = oracle.matmul(a.astype(dtype), b.astype(dtype)).astype(np.longdouble)
+        exact = a.astype(dtype).astype(np.longdouble) @ b.astype(dtype).astype(np.longdouble)
+        bound = k * EPS[dtype] * (np.abs(a.astype(dtype)).astype(np.longdouble) @ np.abs(b.astype(dtype)).astype(np.longdouble))
+        assert (np.abs(got - exact) <= bound * 1.0001 + np.finfo(dtype).tiny).all()
+
+
+def test_vector_products_read_vectors_as_rows_and_columns():
+    "rustymath's matmul with a vector acts as one row on the left and one column on the right (py/src/linalg.rs:241)."
+    a, b = integer_operands(9, 20, 7)
+    v = np.arange(20.0) - 9
+    assert bits_equal(oracle.matmul(a, v), a @ v)
+    w = np.arange(9.0) - 4
+    assert bits_equal(oracle.matmul(w, a), w @ a)
+    assert oracle.matmul(v, v) == v @ v
+
+
+def test_notebook_known_answers():
+    """nbs/linalg.ipynb cells 4, 6, 25, 27 and README.md:79-82, including the signed zeros that the order of the
+    substitutions fixes: U's solve divides by a reciprocal for several right-hand sides and by a quotient for one."""
+    a = np.array([[1., 2], [2, 5]])
+    f, perm, _ = oracle.lu(a)
+    assert perm.tolist() == [1, 0]
+    assert bits_equal(oracle.lower(f), np.array([[1., 0], [0.5, 1]]))
+    assert bits_equal(oracle.upper(f), np.array([[2., 5], [0, -0.5]]))
+    x = oracle.lu_solve(f, perm, np.array([[1., 2], [2, 1]]))
+    assert bits_equal(x, np.array([[1., 8], [-0., -3]]))
+    assert bits_equal(oracle.lu_inverse(f, perm), np.array([[5., -2], [-2, 1]]))
+    assert oracle.det(f, perm) == 1.0 and oracle.slogdet(f, perm) == (1.0, 0.0)
+    rhs = np.array([[1., 2], [2, 1]])
+    want = [[[1.0, 8.0], [-0.0, -3.0]], [[0.5, 4.0], [-0.0, -1.5]], [[0.33333333, 2.6666667], [-0.0, -1.0]]]
+    vectors = [[1.0, -0.0], [4.0, -1.5], [1.0, -0.33333333]]
+    for s in range(3):
+        m = (s + 1) * a
+        x = oracle.solve(m, rhs)
+        assert np.allclose(x, want[s], rtol=1e-7, atol=0) and (np.signbit(x) == np.signbit(want[s])).all()
+        v = oracle.solve(m, np.array([[1., 2], [2, 1], [1, 1]])[s])
+        assert np.allclose(v, vectors[s], rtol=1e-7, atol=0) and (np.signbit(v) == np.signbit(vectors[s])).all()
+    x = oracle.solve(np.array([[4., 1], [1, 3]]), np.array([1., 2]))
+    assert np.allclose(x, [0.090909091, 0.63636364], rtol=1e-8)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_subnormal_pivot(dtype):
+    "tests/linalg.rs:510-511: lu([p, 1; q, 1]).factors == [p, q/p, 1, 1 - q/p], p, q below the normal range."
+    tiny = np.finfo(dtype).tiny
+    p, q = dtype(tiny / 1024), dtype(tiny / 1048576)
+    f, perm, _ = oracle.lu(np.array([[p, 1], [q, 1]], dtype))
+    want = np.array([p, q / p, 1, dtype(1) - q / p], dtype)
+    assert bits_equal(f.T.reshape(-1), want) or bits_equal(f.reshape(-1), want)
+    assert bits_equal(f.reshape(-1), want)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_determinants(dtype):
+    "tests/linalg.rs:553-558: a swapped diag(1, 2, 3, 4) has determinant -24; a zero column gives 0 and -inf."
+    swapped = np.array([[0., 2, 0, 0], [1, 0, 0, 0], [0, 0, 3, 0], [0, 0, 0, 4]], dtype)
+    assert oracle.det(*oracle.lu(swapped)[:2]) == -24
+    f, perm, _ = oracle.lu(np.array([[1., 0], [3, 0]], dtype))
+    assert oracle.det(f, perm) == 0 and oracle.slogdet(f, perm) == (0, -np.inf)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [0, 1, 2, 5, 31, 32, 33, 71])
+def test_decompositions(dtype, n):
+    "tests/linalg.rs:373-390: L U is A with its rows permuted, and solves and the inverse have small residuals."
+    bound = 4 * EPS[dtype] * max(n, 1)
+    a = random(n, n, n + 1, dtype)
+    aw = a.astype(np.float64)
+    f, perm, _ = oracle.lu(a)
+    l, u = oracle.lower(f).astype(np.float64), oracle.upper(f).astype(np.float64)
+    assert distance(l @ u, aw[perm], max(norm(aw), 1)) < bound
+    for k in [1, 3]:
+        b = random(n, k, 99, dtype)
+        x = oracle.lu_solve(f, perm, b).astype(np.float64)
+        assert distance(aw @ x, b, max(norm(aw) * norm(x), 1)) < bound
+    inv = oracle.lu_inverse(f, perm).astype(np.float64)
+    assert distance(aw @ inv, np.eye(n), max(norm(aw) * norm(inv), 1)) < bound
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_hard_matrices_scale_exactly(dtype):
+    """tests/linalg.rs:466-483: scaling by a power of two near underflow or overflow gives the same permutation,
+    U scaled exactly, L unchanged, the same solves of a scaled right-hand side and an exactly scaled inverse."""
+    n = 40
+    fi = np.finfo(dtype)
+    tiny = dtype(fi.tiny / fi.eps)
+    quartic = dtype(2.0 ** (int(math.log2(float(fi.max)) / 4) + 6))
+    a, b = random(n, n, 17, dtype), random(n, 3, 19, dtype)
+    f, perm, _ = oracle.lu(a)
+    for s in [tiny, dtype(1) / tiny, quartic]:
+        g, gperm, _ = oracle.lu(a * s)
+        assert (gperm == perm).all()
+        scaled = f.copy()
+        iu = np.triu_indices(n)            # column-major storage: row j of `f` is column j, so U is the lower part
+        scaled[iu[1], iu[0]] = f[iu[1], iu[0]] * s
+        assert bits_equal(g, scaled)
+        assert bits_equal(oracle.lu_solve(g, gperm, b * s), oracle.lu_solve(f, perm, b))
+        assert bits_equal(oracle.lu_inverse(g, gperm) * s, oracle.lu_inverse(f, perm))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_zero_columns(dtype):
+    "tests/linalg.rs:500-509: a zero column leaves finite factors with a zero pivot and a small residual."
+    n = 71
+    for zero in [0, n // 2, n - 1]:
+        a = random(n, n, 29, dtype)
+        a[:, zero] = 0
+        f, perm, _ = oracle.lu(a)
+        assert np.isfinite(f).all() and (np.diag(f) == 0).any()
+        l, u = oracle.lower(f).astype(np.float64), oracle.upper(f).astype(np.float64)
+        assert distance(l @ u, a.astype(np.float64)[perm], norm(a)) < 4 * EPS[dtype] * n
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [530, 1030])
+def test_blocked_sizes(dtype, n):
+    "tests/linalg.rs:569-590: LU, solve and inverse at sizes past the wider blocks, checked with random vectors."
+    bound = 4 * EPS[dtype] * n
+    a = random(n, n, n, dtype); aw = a.astype(np.float64)
+    x = random(n, 1, 3, dtype).astype(np.float64)[:, 0]
+    f, perm, _ = oracle.lu(a)
+    l, u = oracle.lower(f).astype(np.float64), oracle.upper(f).astype(np.float64)
+    assert distance(l @ (u @ x), (aw @ x)[perm], norm(aw) * norm(x)) < bound
+    b = random(n, 3, 5, dtype)
+    sol = oracle.lu_solve(f, perm, b).astype(np.float64)
+    assert distance(aw @ sol, b, norm(aw) * norm(sol)) < bound
+    inv = oracle.lu_inverse(f, perm).astype(np.float64)
+    assert distance(aw @ (inv @ x), x, norm(aw) * norm(inv) * norm(x)) < bound
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_recursion_equals_the_flat_in_order_factorization(dtype):
+    """lu_panel's recursion (halving to 16-column leaves, triangular solves and products between them) gives the
+    same bits as the flat right-looking statement: every item takes its updates in pivot order, one fused
+    multiply-add each, and a zero pivot skips only the rest of its own leaf. This is the arithmetic that the GPU
+    factorization reproduces, so it is checked on zero columns, NaN and infinities too."""
+    rng = np.random.default_rng(11)
+    cases = []
+    for n in [1, 2, 3, 16, 17, 33, 64, 100, 129, 257]:
+        cases.append(rng.standard_normal((n, n)))
+    z = rng.standard_normal((100, 100)); z[:, [3, 17, 50, 99]] = 0; cases.append(z)
+    z = rng.standard_normal((70, 70)); z[:, 5] = 0; z[7, 5] = np.nan; cases.append(z)
+    z = rng.standard_normal((70, 70)); z[3, 3] = np.inf; z[9, 40] = -np.inf; cases.append(z)
+    z = np.zeros((40, 40)); z[np.arange(40), (np.arange(40) * 7) % 40] = 1; cases.append(z)
+    for a in cases:
+        a = a.astype(dtype)
+        f1, p1, v1 = oracle.lu(a)
+        f2, p2, v2 = oracle.lu(a, flat=True)
+        assert bits_equal(f1, f2) and (p1 == p2).all() and (v1 == v2).all()
+
+
+def test_pivots_match_lapack_getrf():
+    """Partial pivoting with the first largest magnitude is LAPACK getrf's rule (idamax): SciPy's lu_factor picks
+    the same rows wherever no near-tie exists, and random normal matrices of these sizes have none."""
+    rng = np.random.default_rng(2)
+    for n in [10, 64, 200, 333]:
+        a = rng.standard_normal((n, n))
+        _, _, piv = oracle.lu(a)
+        _, lapack = scipy.linalg.lu_factor(a)
+        assert (piv == lapack).all()
+
+
+def test_lu_stack_fixture():
+    "tests/test_linalg.py:14-34: 64 matrices base + k/8 I, solved and inverted against NumPy's LAPACK."
+    base = np.array([[0., 2, 1], [3, 1, -1], [1, 0, 4]])
+    b = np.arange(12.).reshape(3, 4)[:, ::-1]
+    for dt in [np.float32, np.float64]:
+        for k in range(64):
+            a = (base + k / 8 * np.eye(3)).astype(dt)
+            f, perm, _ = oracle.lu(a)
+            assert np.allclose(oracle.lower(f) @ oracle.upper(f), a[perm], rtol=0.002, atol=0.002)
+            wide = a.astype(np.float64)
+            assert np.allclose(oracle.lu_solve(f, perm, b.astype(dt)), np.linalg.solve(wide, b), rtol=0.003, atol=0.003)
+            assert np.allclose(oracle.lu_inverse(f, perm), np.linalg.inv(wide), rtol=0.003, atol=0.003)
+            assert np.allclose(oracle.det(f, perm), np.linalg.det(wide), rtol=0.003, atol=0.003)
+
+
+def test_regular_rule():
+    "factor.rs:221-226: a negligible pivot (here an exactly singular matrix) gives no LU solution."
+    assert oracle.solve(np.array([[1., 2], [2, 4]]), np.array([1., 2])) is None
+    assert oracle.solve(np.zeros((3, 3)), np.ones(3)) is None
+    x = oracle.solve(np.array([[2., 0], [0, 4]]), np.array([1., 1]))
+    assert x.tolist() == [0.5, 0.25]
+
+
+def test_reference_generator_matches_known_bits():
+    """SplitMix64's published first outputs for seed 0 (Vigna's reference implementation): 0xe220a8397b1dcdaf,
+    0x6e789e6aa1b965f4, 0x06c45d188009454f, which pin `support.splitmix` and so the reference tests' inputs."""
+    from support import splitmix
+    got = [int(v) for v in splitmix(np.arange(3, dtype=np.uint64))]
+    assert got == [0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f]
+
+
+def test_backward_error_of_a_plain_solve():
+    "The north-star acceptance bound, 10·n·eps64 on the normwise backward error, at a config-1-like size."
+    a = np.random.default_rng(1).standard_normal((300, 300)); b = np.random.default_rng(1).standard_normal(300)
+    x = oracle.solve(a, b)
+    assert backward_error(a, x, b) <= 10 * 300 * EPS[np.float64]

@@ -1,0 +1,88 @@
+/* hip_sys.h — the C ABI of rustymath's MI355X (gfx950) dense linear-algebra path.
+ *
+ * Conventions (those of the reference's vendor seam, core/src/linalg/accelerate.rs):
+ *   - matrices are row-major with a leading dimension (CBLAS row-major, accelerate.rs:12), 64-bit sizes
+ *     (ILP64, accelerate.rs:9); op codes 111 / 112 / 113 = Normal / Transpose / Adjoint (accelerate.rs:36-37),
+ *     Adjoint being Transpose for real items;
+ *   - LU factors are rustymath's: column-major n×n ("row j holds column j", src/linalg/stacked.rs:25),
+ *     L strictly below the diagonal with a unit diagonal implied, U on and above; perm[i] is the source
+ *     row of row i (core/src/linalg/factor.rs:132-134);
+ *   - every entry point returns 0, or -i where argument i is invalid (LAPACK style), or > 0 for a HIP
+ *     runtime failure (100000 = a device-side bounded wait ran out); hs_last_error() gives the text.
+ *     A zero pivot is not an error (factor.rs:606); NaN propagates;
+ *   - the caller owns every buffer it passes. Host-pointer entry points copy in and out through a
+ *     device-memory pool the library keeps across calls (as LuWork keeps buffers, factor.rs:196-201).
+ *     hs_dev_* entry points take device pointers and a hipStream_t (NULL = the calling thread's own
+ *     stream) and leave results in device memory;
+ *   - thread safety: each host thread gets its own stream; calls from several threads may overlap.
+ *
+ * The arithmetic of every entry point is the reference's: a product takes its terms in order of p, one
+ * fused multiply-add each; the factorization and solves follow lu_factor and lu_solve's orders of
+ * operations (factor.rs:580-748).
+ */
+#ifndef HIP_SYS_H
+#define HIP_SYS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HS_NO_TRANS 111
+#define HS_TRANS 112
+#define HS_CONJ_TRANS 113
+
+/* Library set-up and tear-down; hs_finalize frees the device-memory pool. */
+int hs_init(void);
+int hs_finalize(void);
+/* The calling thread's last error text. */
+const char* hs_last_error(void);
+
+/* c = alpha op(a) op(b) + beta c, row-major, m×k times k×n.
+ * Replaces `Tiled::gemm` (core/src/linalg.rs:772, :799-807) and the CBLAS call of its vendor seam
+ * `accelerate::product` (core/src/linalg/accelerate.rs:208-223); routes as the reference does:
+ * m or n = 0: nothing; k = 0 or alpha = 0: c = beta c (`scale`, linalg.rs:1000); m or n = 1: `by_vector`
+ * (linalg.rs:1020); otherwise the tiled product (linalg.rs:1353). */
+int hs_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+             const double* b, int64_t ldb, double beta, double* c, int64_t ldc);
+int hs_sgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
+             const float* b, int64_t ldb, float beta, float* c, int64_t ldc);
+
+/* The LU factorization with partial pivoting of the row-major n×n matrix a: factors (n×n, column-major),
+ * perm (n) and, where `pivots` is not NULL, the row swapped with row c at step c (n).
+ * Replaces `Factor::matrix_lu_into` / `lu_factor` (core/src/linalg/factor.rs:85-87, :617-630). */
+int hs_dlu_factor(const double* a, int64_t n, double* factors, int64_t* perm, int64_t* pivots);
+int hs_slu_factor(const float* a, int64_t n, float* factors, int64_t* perm, int64_t* pivots);
+
+/* x (n×k, row-major) with A x = b, from factors and perm as hs_*lu_factor gives them; an out-of-range
+ * row position is argument -2. Replaces `Factor::matrix_lu_solve` / `lu_solve` (factor.rs:88, :633-641). */
+int hs_dlu_solve(const double* factors, const int64_t* perm, int64_t n, const double* b, int64_t k, double* x);
+int hs_slu_solve(const float* factors, const int64_t* perm, int64_t n, const float* b, int64_t k, float* x);
+
+/* `LuWork::solve` for a square a (factor.rs:207-226): x = a⁻¹ b through LU where every pivot exceeds
+ * eps·n·max|a| (*regular = 1); *regular = 0 and x untouched where the reference would take least squares,
+ * which this library leaves to the caller. */
+int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x, int* regular);
+int hs_ssolve(const float* a, int64_t n, const float* b, int64_t k, float* x, int* regular);
+
+/* Device-resident forms (no counterpart in the reference, whose API takes host slices): pointers are
+ * device memory, `stream` a hipStream_t or NULL. */
+int hs_dev_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                 const double* b, int64_t ldb, double beta, double* c, int64_t ldc, void* stream);
+int hs_dev_sgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
+                 const float* b, int64_t ldb, float beta, float* c, int64_t ldc, void* stream);
+/* Factors the row-major a (ld lda) in place into column-major factors (same buffer, same ld) and writes
+ * perm (device, n int64). Synchronises `stream` before returning. */
+int hs_dev_dgetrf(int64_t n, double* a, int64_t lda, int64_t* perm, void* stream);
+int hs_dev_sgetrf(int64_t n, float* a, int64_t lda, int64_t* perm, void* stream);
+/* Solves in place on b (n×k row-major, ld ldb) with device factors: A x = b (trans = 0) in the reference's
+ * substitution orders, or Aᵀ x = b (trans = 1, for adjoints). */
+int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_t ldf, const int64_t* perm, double* b,
+                  int64_t ldb, void* stream);
+int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t ldf, const int64_t* perm, float* b,
+                  int64_t ldb, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,62 @@
+"""ctypes binding of libhip_sys.so, the C ABI declared in include/hip_sys.h.
+
+The library is built in-tree by `python -m rustymath_amd.build`. Loading fails loudly where it is missing: there
+is no CPU path behind this package.
+"""
+import ctypes, os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATH = os.path.join(HERE, 'libhip_sys.so')
+
+NO_TRANS, TRANS, CONJ_TRANS = 111, 112, 113
+
+_i32, _i64, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+_d, _f = ctypes.c_double, ctypes.c_float
+
+# name: (restype, argtypes), mirroring include/hip_sys.h
+SIGNATURES = {
+    'hs_init': (_i32, []),
+    'hs_finalize': (_i32, []),
+    'hs_last_error': (ctypes.c_char_p, []),
+    'hs_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
+    'hs_sgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64]),
+    'hs_dlu_factor': (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    'hs_slu_factor': (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    'hs_dlu_solve': (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
+    'hs_slu_solve': (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
+    'hs_dsolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),
+    'hs_ssolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),
+    'hs_dev_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
+    'hs_dev_sgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]),
+    'hs_dev_dgetrf': (_i32, [_i64, _vp, _i64, _vp, _vp]),
+    'hs_dev_sgetrf': (_i32, [_i64, _vp, _i64, _vp, _vp]),
+    'hs_dev_dgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    'hs_dev_sgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+}
+
+_lib = None
+
+
+class HipSysError(RuntimeError):
+    "A failure status from the HIP library (a HIP runtime error or a device-side wait that ran out)."
+
+
+def load():
+    "The loaded library, with argument types set; raises where it has not been built."
+    global _lib
+    if _lib is None:
+        if not os.path.exists(PATH):
+            raise RuntimeError(f'{PATH} is not built: run `python -m rustymath_amd.build` (hipcc, gfx950)')
+        lib = ctypes.CDLL(PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype, f.argtypes = res, args
+        _lib = lib
+    return _lib
+
+
+def check(status, what):
+    "Turns a status into Python's errors: ValueError for a bad argument, HipSysError for a device failure."
+    if status == 0: return
+    if status < 0: raise ValueError(f'{what}: argument {-status} is invalid')
+    raise HipSysError(f'{what}: {load().hs_last_error().decode(errors="replace")} (status {status})')

@@ -1,0 +1,354 @@
+// The C ABI of hip-sys (declared in include/hip_sys.h): plain pointers and sizes, no torch types.
+// Host-pointer entry points are the drop-in for rustymath's `Tiled`/`Factor` seam; `hs_dev_*` entry
+// points take device pointers and a stream, for callers that keep data resident in HBM.
+#include "hs_kernels.h"
+#include "../../include/hip_sys.h"
+#include <map>
+#include <mutex>
+#include <vector>
+#include <cstring>
+#include <cmath>
+
+namespace hs {
+
+namespace {
+thread_local std::string t_error;
+thread_local hipStream_t t_stream = nullptr;
+std::mutex g_pool_mu;
+std::multimap<size_t, void*> g_free;
+std::map<void*, size_t> g_size;
+
+size_t size_class(size_t bytes) {
+    if (bytes <= (1u << 20)) return (bytes + 255) / 256 * 256;
+    size_t c = 1u << 20;
+    while (c < bytes) c += c / 4 > (64u << 20) ? (64u << 20) : c / 4;  // ~25% steps, at most 64 MiB apart
+    return c;
+}
+}  // namespace
+
+void set_error(const std::string& what) { t_error = what; }
+const char* last_error() { return t_error.c_str(); }
+
+int hip_status(hipError_t e, const char* where) {
+    t_error = std::string(where) + ": " + hipGetErrorString(e);
+    return (int)e > 0 ? (int)e : 1;
+}
+
+hipStream_t thread_stream() {
+    if (!t_stream) {
+        if (hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking) != hipSuccess) t_stream = nullptr;
+    }
+    return t_stream;
+}
+
+void* pool_get(size_t bytes) {
+    size_t c = size_class(bytes ? bytes : 1);
+    {
+        std::lock_guard<std::mutex> lk(g_pool_mu);
+        auto it = g_free.find(c);
+        if (it != g_free.end()) { void* p = it->second; g_free.erase(it); return p; }
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, c) != hipSuccess) {
+        // Give back cached buffers and try once more.
+        pool_release_all();
+        if (hipMalloc(&p, c) != hipSuccess) return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    g_size[p] = c;
+    return p;
+}
+
+void pool_put(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto it = g_size.find(p);
+    if (it == g_size.end()) return;
+    g_free.insert({it->second, p});
+}
+
+void pool_release_all() {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    for (auto& kv : g_free) { hipFree(kv.second); g_size.erase(kv.second); }
+    g_free.clear();
+}
+
+int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 256;
+    return cus;
+}
+
+namespace {
+
+// `greatest(a, modulus)` (core/src/linalg/factor.rs:488) over n×n items: NaN ignored, 0 for none.
+template <typename T>
+__global__ void maxabs_kernel(int64_t count, const T* a, unsigned long long* out) {
+    T m = T(0);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        T v = fabs(a[i]);
+        m = v > m ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) { T o = __shfl_xor(m, off); m = o > m ? o : m; }
+    // Magnitudes are non-negative, so their bit patterns order as their values.
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__builtin_bit_cast(typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type, m));
+}
+
+// `regular` (factor.rs:221-226): whether every |u_ii| exceeds eps·n·largest; NaN is not negligible.
+template <typename T>
+__global__ void regular_kernel(int64_t n, const T* f, int64_t ld, const unsigned long long* largest_bits, int* out) {
+    typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type U;
+    T largest = __builtin_bit_cast(T, (U)*largest_bits);
+    T negligible = (sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07f)) * T(n) * largest;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (fabs(f[i * ld + i]) <= negligible) atomicExch(out, 0);
+}
+
+__global__ void fill_int(int* p, int v) { *p = v; }
+
+template <typename T>
+int dev_getrf_perm(int64_t n, T* a, int64_t lda, int64_t* d_perm, int64_t* h_perm_out, int64_t* h_piv_out, hipStream_t s) {
+    int64_t* d_piv = (int64_t*)pool_get(sizeof(int64_t) * (size_t)(n ? n : 1));
+    if (!d_piv) return hip_status(hipErrorOutOfMemory, "pool_get");
+    int rc = getrf<T>(n, a, lda, d_piv, s);
+    if (rc) { pool_put(d_piv); return rc; }
+    std::vector<int64_t> piv(n), perm(n);
+    HS_HIP(hipMemcpyAsync(piv.data(), d_piv, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+    HS_HIP(hipStreamSynchronize(s));
+    pool_put(d_piv);
+    for (int64_t i = 0; i < n; ++i) perm[i] = i;
+    for (int64_t c = 0; c < n; ++c) std::swap(perm[c], perm[piv[c]]);
+    if (d_perm) HS_HIP(hipMemcpyAsync(d_perm, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+    if (h_perm_out) std::memcpy(h_perm_out, perm.data(), sizeof(int64_t) * n);
+    if (h_piv_out) std::memcpy(h_piv_out, piv.data(), sizeof(int64_t) * n);
+    HS_HIP(hipStreamSynchronize(s));
+    return OK;
+}
+
+int op_of(int code, bool& t) {
+    if (code == HS_NO_TRANS) { t = false; return 0; }
+    if (code == HS_TRANS || code == HS_CONJ_TRANS) { t = true; return 0; }
+    return 1;
+}
+
+template <typename T>
+int check_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, int64_t lda, int64_t ldb, int64_t ldc, bool& ta, bool& tb) {
+    if (op_of(opa, ta)) return -1;
+    if (op_of(opb, tb)) return -2;
+    if (m < 0) return -3;
+    if (n < 0) return -4;
+    if (k < 0) return -5;
+    if (m > 0 && k > 0 && lda < (ta ? m : k)) return -8;
+    if (k > 0 && n > 0 && ldb < (tb ? k : n)) return -10;
+    if (m > 0 && n > 0 && ldc < n) return -13;
+    return 0;
+}
+
+template <typename T>
+int host_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
+              T beta, T* C, int64_t ldc) {
+    bool ta, tb;
+    int e = check_gemm<T>(opa, opb, m, n, k, lda, ldb, ldc, ta, tb);
+    if (e) return e;
+    if (m == 0 || n == 0) return OK;
+    hipStream_t s = thread_stream();
+    const int64_t ar = ta ? k : m, ac = ta ? m : k, br = tb ? n : k, bc = tb ? k : n;
+    T *dA = nullptr, *dB = nullptr, *dC = (T*)pool_get(sizeof(T) * (size_t)(m * n));
+    if (!dC) return hip_status(hipErrorOutOfMemory, "pool_get");
+    int rc = OK;
+    if (k > 0 && alpha != T(0)) {
+        dA = (T*)pool_get(sizeof(T) * (size_t)(ar * ac ? ar * ac : 1));
+        dB = (T*)pool_get(sizeof(T) * (size_t)(br * bc ? br * bc : 1));
+        if (!dA || !dB) { rc = hip_status(hipErrorOutOfMemory, "pool_get"); goto done; }
+        if (hipMemcpy2DAsync(dA, ac * sizeof(T), A, lda * sizeof(T), ac * sizeof(T), ar, hipMemcpyHostToDevice, s) != hipSuccess ||
+            hipMemcpy2DAsync(dB, bc * sizeof(T), B, ldb * sizeof(T), bc * sizeof(T), br, hipMemcpyHostToDevice, s) != hipSuccess) {
+            rc = hip_status(hipErrorUnknown, "hipMemcpy2DAsync"); goto done;
+        }
+    }
+    if (beta != T(0)) {
+        if (hipMemcpy2DAsync(dC, n * sizeof(T), C, ldc * sizeof(T), n * sizeof(T), m, hipMemcpyHostToDevice, s) != hipSuccess) {
+            rc = hip_status(hipErrorUnknown, "hipMemcpy2DAsync"); goto done;
+        }
+    }
+    rc = gemm<T>(ta, tb, m, n, k, alpha, dA, ac, dB, bc, beta, dC, n, s);
+    if (rc) goto done;
+    if (hipMemcpy2DAsync(C, ldc * sizeof(T), dC, n * sizeof(T), n * sizeof(T), m, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        rc = hip_status(hipErrorUnknown, "copy back");
+done:
+    hipStreamSynchronize(s);
+    pool_put(dA); pool_put(dB); pool_put(dC);
+    return rc;
+}
+
+template <typename T>
+int host_lu_factor(const T* a, int64_t n, T* factors, int64_t* perm, int64_t* pivots) {
+    if (n < 0) return -2;
+    if (n == 0) return OK;
+    hipStream_t s = thread_stream();
+    T* d = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    if (!d) return hip_status(hipErrorOutOfMemory, "pool_get");
+    int rc = OK;
+    if (hipMemcpyAsync(d, a, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) rc = dev_getrf_perm<T>(n, d, n, nullptr, perm, pivots, s);
+    if (!rc && hipMemcpyAsync(factors, d, sizeof(T) * n * n, hipMemcpyDeviceToHost, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "D2H");
+    hipStreamSynchronize(s);
+    pool_put(d);
+    return rc;
+}
+
+template <typename T>
+int host_lu_solve(const T* factors, const int64_t* perm, int64_t n, const T* b, int64_t k, T* x) {
+    if (n < 0) return -3;
+    if (k < 0) return -5;
+    for (int64_t i = 0; i < n; ++i) if (perm[i] < 0 || perm[i] >= n) return -2;
+    if (n == 0 || k == 0) return OK;
+    hipStream_t s = thread_stream();
+    T* df = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    T* db = (T*)pool_get(sizeof(T) * (size_t)(n * k));
+    int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)n);
+    int rc = OK;
+    if (!df || !db || !dp) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && (hipMemcpyAsync(df, factors, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(db, b, sizeof(T) * n * k, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(dp, perm, sizeof(int64_t) * n, hipMemcpyHostToDevice, s) != hipSuccess))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) rc = getrs<T>(false, n, k, df, n, dp, db, k, s);
+    if (!rc && hipMemcpyAsync(x, db, sizeof(T) * n * k, hipMemcpyDeviceToHost, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "D2H");
+    hipStreamSynchronize(s);
+    pool_put(df); pool_put(db); pool_put(dp);
+    return rc;
+}
+
+template <typename T>
+int host_solve(const T* a, int64_t n, const T* b, int64_t k, T* x, int* regular) {
+    if (n < 0) return -2;
+    if (k < 0) return -4;
+    if (regular) *regular = 0;
+    if (n == 0) return OK;
+    hipStream_t s = thread_stream();
+    T* d = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    T* db = (T*)pool_get(sizeof(T) * (size_t)(n * (k ? k : 1)));
+    int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)n);
+    unsigned long long* dm = (unsigned long long*)pool_get(64);
+    int rc = OK, reg = 1;
+    if (!d || !db || !dp || !dm) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && (hipMemcpyAsync(d, a, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+                (k && hipMemcpyAsync(db, b, sizeof(T) * n * k, hipMemcpyHostToDevice, s) != hipSuccess)))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) {
+        hipMemsetAsync(dm, 0, 64, s);
+        maxabs_kernel<T><<<256, 256, 0, s>>>(n * n, d, dm);
+        rc = dev_getrf_perm<T>(n, d, n, dp, nullptr, nullptr, s);
+    }
+    if (!rc) {
+        int* dreg = (int*)(dm + 1);
+        fill_int<<<1, 1, 0, s>>>(dreg, 1);
+        regular_kernel<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, d, n, dm, dreg);
+        if (hipMemcpyAsync(&reg, dreg, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+            rc = hip_status(hipErrorUnknown, "regular");
+    }
+    if (!rc && reg && k) rc = getrs<T>(false, n, k, d, n, dp, db, k, s);
+    if (!rc && reg && k && hipMemcpyAsync(x, db, sizeof(T) * n * k, hipMemcpyDeviceToHost, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "D2H");
+    hipStreamSynchronize(s);
+    pool_put(d); pool_put(db); pool_put(dp); pool_put(dm);
+    if (regular) *regular = rc ? 0 : reg;
+    return rc;
+}
+
+template <typename T>
+int dev_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
+             T beta, T* C, int64_t ldc, void* stream) {
+    bool ta, tb;
+    int e = check_gemm<T>(opa, opb, m, n, k, lda, ldb, ldc, ta, tb);
+    if (e) return e;
+    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    return gemm<T>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
+}
+
+}  // namespace
+}  // namespace hs
+
+using namespace hs;
+
+extern "C" {
+
+int hs_init(void) {
+    int n = 0;
+    HS_HIP(hipGetDeviceCount(&n));
+    if (n <= 0) { set_error("no HIP device"); return 1; }
+    return thread_stream() ? OK : hip_status(hipErrorUnknown, "hipStreamCreate");
+}
+
+int hs_finalize(void) {
+    hipDeviceSynchronize();
+    pool_release_all();
+    return OK;
+}
+
+const char* hs_last_error(void) { return last_error(); }
+
+int hs_dgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda, const double* b,
+             int64_t ldb, double beta, double* c, int64_t ldc) {
+    return host_gemm<double>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+int hs_sgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda, const float* b,
+             int64_t ldb, float beta, float* c, int64_t ldc) {
+    return host_gemm<float>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+int hs_dlu_factor(const double* a, int64_t n, double* factors, int64_t* perm, int64_t* pivots) {
+    return host_lu_factor<double>(a, n, factors, perm, pivots);
+}
+int hs_slu_factor(const float* a, int64_t n, float* factors, int64_t* perm, int64_t* pivots) {
+    return host_lu_factor<float>(a, n, factors, perm, pivots);
+}
+int hs_dlu_solve(const double* factors, const int64_t* perm, int64_t n, const double* b, int64_t k, double* x) {
+    return host_lu_solve<double>(factors, perm, n, b, k, x);
+}
+int hs_slu_solve(const float* factors, const int64_t* perm, int64_t n, const float* b, int64_t k, float* x) {
+    return host_lu_solve<float>(factors, perm, n, b, k, x);
+}
+int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x, int* regular) {
+    return host_solve<double>(a, n, b, k, x, regular);
+}
+int hs_ssolve(const float* a, int64_t n, const float* b, int64_t k, float* x, int* regular) {
+    return host_solve<float>(a, n, b, k, x, regular);
+}
+
+int hs_dev_dgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda, const double* b,
+                 int64_t ldb, double beta, double* c, int64_t ldc, void* stream) {
+    return dev_gemm<double>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int hs_dev_sgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda, const float* b,
+                 int64_t ldb, float beta, float* c, int64_t ldc, void* stream) {
+    return dev_gemm<float>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
+}
+int hs_dev_dgetrf(int64_t n, double* a, int64_t lda, int64_t* perm, void* stream) {
+    if (n < 0) return -1;
+    if (lda < n) return -3;
+    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    return dev_getrf_perm<double>(n, a, lda, perm, nullptr, nullptr, s);
+}
+int hs_dev_sgetrf(int64_t n, float* a, int64_t lda, int64_t* perm, void* stream) {
+    if (n < 0) return -1;
+    if (lda < n) return -3;
+    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    return dev_getrf_perm<float>(n, a, lda, perm, nullptr, nullptr, s);
+}
+int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_t ldf, const int64_t* perm, double* b, int64_t ldb,
+                  void* stream) {
+    if (n < 0) return -2;
+    if (k < 0) return -3;
+    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    return getrs<double>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
+}
+int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t ldf, const int64_t* perm, float* b, int64_t ldb,
+                  void* stream) {
+    if (n < 0) return -2;
+    if (k < 0) return -3;
+    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    return getrs<float>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,273 @@
+// Matrix products on gfx950: an MFMA GEMM with LDS-staged, double-buffered tiles (FP64 and FP32), the
+// `by_vector` products of a matrix and a vector, and the scaling of `gemm`'s degenerate cases.
+//
+// Reference: core/src/linalg.rs gemm :799-807, scale :1000-1008, by_vector :1020-1073, product
+// :1353-1426. rustymath's product gives each result as one fused multiply-add per step of p, in order,
+// from 0 (beta = 0) or from beta c, with alpha folded into a's packed copy. This kernel keeps one
+// accumulator per result for the whole of k and feeds the MFMA unit in order of p, so it computes the
+// same chain of roundings; a k tail pads with +0 · (-0) = -0, which adds nothing to any value.
+#include "hs_kernels.h"
+
+namespace hs {
+
+namespace {
+
+constexpr int BM = 128, BN = 128, BK = 16, THREADS = 256;
+
+template <typename T> struct Pad { static constexpr int v = 2; };
+template <> struct Pad<float> { static constexpr int v = 4; };
+
+// Maps a launch index to an output tile so that each XCD (dispatch is round-robin over the 8 XCDs)
+// works on one contiguous run of tiles, and runs of 8 tile-rows share B's column panels in its L2.
+__device__ __forceinline__ bool tile_of(int w, int tiles_m, int tiles_n, int& tm, int& tn) {
+    int total = tiles_m * tiles_n;
+    int per = (total + 7) / 8;
+    int id = (w % 8) * per + w / 8;
+    if (id >= total) return false;
+    constexpr int G = 8;
+    int group = id / (G * tiles_n);
+    int first = group * G;
+    int gm = tiles_m - first < G ? tiles_m - first : G;
+    int in = id - group * G * tiles_n;
+    tm = first + in % gm;
+    tn = in / gm;
+    return true;
+}
+
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, int64_t k, T alpha,
+                                                         const T* __restrict__ A, int64_t lda,
+                                                         const T* __restrict__ B, int64_t ldb, T beta,
+                                                         T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
+    constexpr int P = Pad<T>::v;
+    __shared__ T As[2][BK][BM + P];
+    __shared__ T Bs[2][BK][BN + P];
+    typedef typename Mfma<T>::acc Acc;
+
+    int tm, tn;
+    if (!tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn)) return;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const bool scale_a = alpha != T(1);
+
+    // Accumulators start at 0, or at beta c (c itself where beta is 1), as `native_gemm` scales c first.
+    Acc acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int64_t r = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + e, col = n0 + wn * 64 + j * 16 + (lane & 15);
+                T v = T(0);
+                if (beta != T(0) && r < m && col < n) {
+                    v = C[r * ldc + col];
+                    if (beta != T(1)) v = beta * v;
+                }
+                acc[i][j][e] = v;
+            }
+
+    // Global loads: each thread brings 8 items of A's tile and 8 of B's per step of BK.
+    T ra[8], rb[8];
+    auto load = [&](int64_t p0) {
+        // alpha scales only the items that exist, so that padding stays +0 whatever alpha's sign.
+        if (!TA) {  // A row-major m×k: thread -> row t/2, steps (t%2)*8 ..+8
+            int64_t r = m0 + (t >> 1), p = p0 + (t & 1) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                T v = (r < m && p + q < k) ? A[r * lda + p + q] : T(0);
+                ra[q] = (scale_a && r < m && p + q < k) ? alpha * v : v;
+            }
+        } else {  // A stored k×m: thread -> step t/16, rows (t%16)*8 ..+8
+            int64_t p = p0 + (t >> 4), r = m0 + (t & 15) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                T v = (p < k && r + q < m) ? A[p * lda + r + q] : T(0);
+                ra[q] = (scale_a && p < k && r + q < m) ? alpha * v : v;
+            }
+        }
+        if (!TB) {  // B row-major k×n: thread -> step t/16, columns (t%16)*8 ..+8
+            int64_t p = p0 + (t >> 4), c = n0 + (t & 15) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rb[q] = p < k ? ((c + q < n) ? B[p * ldb + c + q] : T(0)) : T(-0.0);
+        } else {  // B stored n×k: thread -> column t/2, steps (t%2)*8 ..+8
+            int64_t c = n0 + (t >> 1), p = p0 + (t & 1) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) rb[q] = p + q < k ? ((c < n) ? B[c * ldb + p + q] : T(0)) : T(-0.0);
+        }
+    };
+    auto stash = [&](int buf) {
+        if (!TA) {
+            int r = t >> 1, p = (t & 1) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) As[buf][p + q][r] = ra[q];
+        } else {
+            int p = t >> 4, r = (t & 15) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) As[buf][p][r + q] = ra[q];
+        }
+        if (!TB) {
+            int p = t >> 4, c = (t & 15) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[buf][p][c + q] = rb[q];
+        } else {
+            int c = t >> 1, p = (t & 1) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[buf][p + q][c] = rb[q];
+        }
+    };
+
+    const int64_t steps = (k + BK - 1) / BK;
+    load(0);
+    stash(0);
+    __syncthreads();
+    for (int64_t s = 0; s < steps; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < steps) load((s + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            T fa[4], fb[4];
+            const int kr = kk * 4 + (lane >> 4), li = lane & 15;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) fa[i] = As[buf][kr][wm * 64 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) fb[j] = Bs[buf][kr][wn * 64 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::step(fa[i], fb[j], acc[i][j]);
+        }
+        if (s + 1 < steps) stash(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                int64_t r = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + e, col = n0 + wn * 64 + j * 16 + (lane & 15);
+                if (r < m && col < n) C[r * ldc + col] = acc[i][j][e];
+            }
+}
+
+// `scale` (core/src/linalg.rs:1000-1008): c = beta c, zeros for beta = 0.
+template <typename T>
+__global__ void scale_kernel(int64_t m, int64_t n, T beta, T* C, int64_t ldc) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m * n) return;
+    int64_t i = idx / n, j = idx % n;
+    T* x = &C[i * ldc + j];
+    *x = beta == T(0) ? T(0) : beta * *x;
+}
+
+// `by_vector` (core/src/linalg.rs:1020-1073): one thread per result, each summing in the order the
+// reference's kernel for its case uses — `dots` (:903, quads of rows with eight running sums added
+// pairwise; rows past the last quad take `dot`, factor.rs:460), `narrow_dots` (:923, two sums) for
+// stored rows of at most 8 items, and for a transposed matrix `narrow_sums` (:941, four interleaved
+// sets) for at most 8 results or an in-order fold (`scaled_runs`, :883) otherwise. `mlen` is the
+// matrix slice's length from its start, which `narrow_sums` reads.
+template <typename T>
+__global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ mat, int64_t ld, int mtrans, int64_t mlen,
+                                 const T* __restrict__ vec, int64_t vstride, T alpha, T beta, T* C, int64_t step) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    T y;
+    if (!mtrans) {
+        const T* row = mat + i * ld;
+        if (k <= 8) {
+            T s0 = 0, s1 = 0;
+            for (int64_t l = 0; l < k; ++l) {
+                if (l % 2 == 0) s0 = __builtin_fma(row[l], vec[l * vstride], s0);
+                else s1 = __builtin_fma(row[l], vec[l * vstride], s1);
+            }
+            y = s0 + s1;
+        } else {
+            T s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            int64_t whole = k / 8 * 8;
+            for (int64_t c = 0; c < whole; c += 8)
+#pragma unroll
+                for (int l = 0; l < 8; ++l) s[l] = __builtin_fma(row[c + l], vec[(c + l) * vstride], s[l]);
+            T tsum;
+            if (i < rows / 4 * 4) tsum = ((s[0] + s[4]) + (s[2] + s[6])) + ((s[1] + s[5]) + (s[3] + s[7]));
+            else {
+                tsum = T(0);
+#pragma unroll
+                for (int l = 0; l < 8; ++l) tsum = tsum + s[l];
+            }
+            for (int64_t p = whole; p < k; ++p) tsum = __builtin_fma(row[p], vec[p * vstride], tsum);
+            y = tsum;
+        }
+    } else if (rows <= 8) {
+        int64_t whole;
+        if (mlen < 8) whole = 0;
+        else { whole = (mlen - 8) / (ld > 1 ? ld : 1) + 1; if (whole > k) whole = k; }
+        T acc[4] = {0, 0, 0, 0};
+        int64_t quads = whole / 4;
+        for (int64_t q = 0; q < quads; ++q)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) acc[s] = __builtin_fma(mat[(4 * q + s) * ld + i], vec[(4 * q + s) * vstride], acc[s]);
+        for (int64_t p = quads * 4; p < k; ++p) acc[0] = __builtin_fma(mat[p * ld + i], vec[p * vstride], acc[0]);
+        y = (acc[0] + acc[2]) + (acc[1] + acc[3]);
+    } else {
+        T a = 0;
+        for (int64_t p = 0; p < k; ++p) a = __builtin_fma(vec[p * vstride], mat[p * ld + i], a);
+        y = a;
+    }
+    if (alpha != T(1)) y = alpha * y;
+    T* z = &C[i * step];
+    *z = beta == T(0) ? y : __builtin_fma(beta, *z, y);
+}
+
+}  // namespace
+
+template <typename T>
+int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
+         T beta, T* c, int64_t ldc, hipStream_t s) {
+    if (m == 0 || n == 0) return OK;
+    if (k == 0 || alpha == T(0)) {
+        if (beta == T(1)) return OK;
+        int64_t total = m * n;
+        scale_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(m, n, beta, c, ldc);
+        HS_HIP(hipGetLastError());
+        return OK;
+    }
+    if (n == 1 || m == 1) {
+        // As by_vector: for n = 1 the matrix is a and the vector b's column; for m = 1 the matrix is bᵀ.
+        const T* mat; int64_t ld, rows, step, vstride, mlen; int mtrans; const T* vec;
+        if (n == 1) {
+            mat = a; ld = lda; mtrans = ta; rows = m; step = ldc;
+            vec = b; vstride = tb ? 1 : ldb;
+            mlen = ta ? (k - 1) * lda + m : (m - 1) * lda + k;
+        } else {
+            mat = b; ld = ldb; mtrans = !tb; rows = n; step = 1;
+            vec = a; vstride = ta ? lda : 1;
+            mlen = tb ? (n - 1) * ldb + k : (k - 1) * ldb + n;
+        }
+        by_vector_kernel<T><<<(unsigned)((rows + 255) / 256), 256, 0, s>>>(rows, k, mat, ld, mtrans, mlen, vec, vstride,
+                                                                          alpha, beta, c, step);
+        HS_HIP(hipGetLastError());
+        return OK;
+    }
+    int tiles_m = (int)((m + BM - 1) / BM), tiles_n = (int)((n + BN - 1) / BN);
+    int total = tiles_m * tiles_n;
+    unsigned grid = (unsigned)(((total + 7) / 8) * 8);
+#define HS_LAUNCH(TA_, TB_) \
+    gemm_mfma<T, TA_, TB_><<<grid, THREADS, 0, s>>>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tiles_m, tiles_n)
+    if (!ta && !tb) HS_LAUNCH(false, false);
+    else if (!ta && tb) HS_LAUNCH(false, true);
+    else if (ta && !tb) HS_LAUNCH(true, false);
+    else HS_LAUNCH(true, true);
+#undef HS_LAUNCH
+    HS_HIP(hipGetLastError());
+    return OK;
+}
+
+template int gemm<double>(bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*, int64_t,
+                          double, double*, int64_t, hipStream_t);
+template int gemm<float>(bool, bool, int64_t, int64_t, int64_t, float, const float*, int64_t, const float*, int64_t,
+                         float, float*, int64_t, hipStream_t);
+
+}  // namespace hs

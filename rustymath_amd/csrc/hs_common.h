@@ -1,0 +1,75 @@
+// Shared plumbing of the hip-sys library: status codes, the per-thread error text, the per-thread
+// stream, the device-memory pool, and the MFMA operand traits that the kernels share.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <type_traits>
+
+namespace hs {
+
+// Status codes of every entry point: 0 = ok, -i = argument i is invalid (LAPACK style),
+// > 0 = a HIP runtime failure (the hipError_t value), HS_ERR_LIMIT = a device-side bounded wait ran out.
+constexpr int OK = 0;
+constexpr int ERR_LIMIT = 100000;
+
+void set_error(const std::string& what);
+const char* last_error();
+
+// The calling thread's stream: one per host thread, so that calls from concurrent host threads
+// (the reference runs cells on Rayon workers) neither serialise on nor race through one queue.
+hipStream_t thread_stream();
+
+// Device memory from a size-keyed pool that keeps buffers for reuse across calls, as `LuWork` keeps
+// its buffers across factorizations (core/src/linalg/factor.rs:196-201).
+void* pool_get(size_t bytes);
+void pool_put(void* p);
+void pool_release_all();
+
+// Number of compute units of the current device (for grid sizing of persistent kernels).
+int device_cus();
+
+int hip_status(hipError_t e, const char* where);
+
+}  // namespace hs
+
+#define HS_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) return hs::hip_status(e_, #call);               \
+    } while (0)
+
+// A loop over compile-time indices C..E-1: f(std::integral_constant<int, i>) for each i, so that
+// register arrays indexed by i stay in registers.
+namespace hs {
+template <int C, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (C < E) {
+        f(std::integral_constant<int, C>{});
+        static_for<C + 1, E>(f);
+    }
+}
+}  // namespace hs
+
+// MFMA operand traits. Each 16x16x4 instruction takes one item of A (16x4) and one of B (4x16) per
+// lane: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; the 16x16 result gives lane l the four
+// items C[4 * (l / 16) + e][l % 16], e = 0..3.
+namespace hs {
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef d4 acc;
+    static __device__ __forceinline__ acc step(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef f4 acc;
+    static __device__ __forceinline__ acc step(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+}  // namespace hs

@@ -1,0 +1,42 @@
+// Internal launch API of the hip-sys kernels (C++; the C ABI in hip_sys.h wraps it).
+// Every matrix is row-major with a leading dimension; `ta`/`tb` read an operand as the transpose of
+// what it stores. Launches go to `s` and return a status (hs_common.h).
+#pragma once
+#include "hs_common.h"
+
+namespace hs {
+
+// Tiled::gemm (core/src/linalg.rs:772, :799-807): c = alpha op(a) op(b) + beta c, with the reference's
+// routing: nothing for m or n = 0, a scaling for k = 0 or alpha = 0, `by_vector` for m or n = 1
+// (:1020-1073) and the MFMA kernel otherwise. Each result starts from 0 or beta c and takes alpha a b
+// in order of p, one fused multiply-add per step (:1353-1426).
+template <typename T>
+int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda,
+         const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s);
+
+// BF16-input, FP32-accumulate product for the low-precision factorization: c (f32) = c - a b with a, b
+// rounded to bfloat16 once (MFMA 16x16x32 bf16).
+int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t lda, const float* b, int64_t ldb,
+                     float* c, int64_t ldc, hipStream_t s);
+
+// LU with partial pivoting of the row-major n×n matrix `a` (ld `lda`) in place: L strictly below the
+// diagonal (unit diagonal implicit), U on and above; pivots[c] = the row swapped with row c (absolute,
+// 0-based, LAPACK ipiv style). The arithmetic is rustymath's lu_factor (core/src/linalg/factor.rs:617),
+// item for item. `info` (device, may be null) gets 0 or 1 + the first zero pivot.
+template <typename T>
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s);
+
+// Solves with factors from getrf: x = U⁻¹ L⁻¹ P b in place on b (n×k, ld ldb, row-major), with the
+// reference's substitution orders (factor.rs:633-641, :668-748). `trans` solves Aᵀ x = b instead
+// (Uᵀ, then Lᵀ, then the inverse row permutation).
+template <typename T>
+int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int64_t* pivots, T* b, int64_t ldb,
+          hipStream_t s);
+
+// Converts between the device layouts: transpose (square or rectangular), f64 <-> f32 casts.
+template <typename T>
+int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s);
+int cast_d2s(int64_t rows, int64_t cols, const double* src, int64_t lds, float* dst, int64_t ldd, hipStream_t s);
+int cast_s2d(int64_t rows, int64_t cols, const float* src, int64_t lds, double* dst, int64_t ldd, hipStream_t s);
+
+}  // namespace hs

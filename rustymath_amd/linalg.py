@@ -1,0 +1,246 @@
+"""rustymath's dense linear-algebra hot path on MI355X, under the names of python/rustymath/linalg.py:
+`matmul`, `solve`, `lu`, the `Lu` factors with `solve`, `determinant` and `log_determinant`, and `lu_solve`.
+
+Arguments are NumPy-convertible. Functions take stacks of matrices in the last two axes; only a one-dimensional
+right-hand side is a vector. Frames pair from the front, as rustymath broadcasts (src/linalg/stacked.rs:1,
+tests/test_linalg.py:111-112), where NumPy pairs them from the back. float32 and float64 compute at their own
+width; float16 computes in float32 with results rounded to float16; booleans, uint8 and int16 compute as float32
+and wider integers as float64 (src/linalg/stacked.rs:78-85). Every product, factorization and solve runs on the
+GPU through libhip_sys.so; a square matrix with a negligible pivot, where rustymath falls back to least squares
+(core/src/linalg/factor.rs:208), raises `NotImplementedError` (least squares is outside this library).
+"""
+from typing import NamedTuple
+import ctypes
+import numpy as np
+
+from . import _lib
+
+__all__ = ['matmul', 'solve', 'lu', 'Lu', 'lu_solve', 'lu_det', 'lu_slogdet', 'lu_lower', 'lu_upper']
+
+
+def _ptr(x): return x.ctypes.data_as(ctypes.c_void_p)
+
+
+def _compute_dtype(*arrays):
+    "The dtypes (compute, store) of rustymath's ladder for these arguments (src/linalg/stacked.rs:78-85)."
+    common = np.result_type(*arrays)
+    if np.issubdtype(common, np.complexfloating):
+        raise TypeError('complex matrices are outside this library (rustymath computes them on the CPU)')
+    if common == np.float64: return np.float64, np.float64
+    if common == np.float32: return np.float32, np.float32
+    if common == np.float16: return np.float32, np.float16
+    if common in (np.bool_, np.uint8, np.int8, np.int16): return np.float32, np.float32
+    if np.issubdtype(common, np.integer): return np.float64, np.float64
+    raise TypeError(f'linalg does not take {common}')
+
+
+def _pair_frames(fa, fb):
+    "Pairs two frames from the front: the shorter gains trailing unit axes, then they broadcast."
+    la, lb = len(fa), len(fb)
+    fa2 = tuple(fa) + (1,) * (max(la, lb) - la)
+    fb2 = tuple(fb) + (1,) * (max(la, lb) - lb)
+    return fa2, fb2, np.broadcast_shapes(fa2, fb2)
+
+
+def _cells(x, frame, core):
+    "x with its frame broadcast to `frame`, as a C-contiguous array of cells [count, *core]."
+    x = np.broadcast_to(x, tuple(frame) + tuple(core))
+    return np.ascontiguousarray(x.reshape((-1,) + tuple(core)))
+
+
+def _sfx(dt): return 'd' if dt == np.float64 else 's'
+
+
+def matmul(x1, x2):
+    """The matrix product of x1 and x2 (py/src/linalg.rs:241-247, the `+.×` inner product). A vector on the left
+    acts as one row and on the right as one column, and the result leaves that axis out."""
+    a, b = np.asarray(x1), np.asarray(x2)
+    if np.issubdtype(np.result_type(a, b), np.integer) or np.result_type(a, b) == np.bool_:
+        raise TypeError('integer and Boolean products (int64 `+.×`) are outside this library')
+    compute, store = _compute_dtype(a, b)
+    av, bv = a.ndim == 1, b.ndim == 1
+    a2 = a[None, :] if av else a
+    b2 = b[:, None] if bv else b
+    if a2.ndim < 2 or b2.ndim < 2: raise ValueError('matmul needs arguments with at least one axis')
+    m, k = a2.shape[-2:]
+    kb, n = b2.shape[-2:]
+    if k != kb: raise ValueError(f'matmul contracts {k} items with {kb}')
+    fa, fb, frame = _pair_frames(a2.shape[:-2], b2.shape[:-2])
+    ac = _cells(a2.reshape(fa + (m, k)).astype(compute, copy=False), frame, (m, k))
+    bc = _cells(b2.reshape(fb + (k, n)).astype(compute, copy=False), frame, (k, n))
+    out = np.zeros((ac.shape[0], m, n), compute)
+    lib = _lib.load()
+    f = getattr(lib, 'hs_' + _sfx(compute) + 'gemm')
+    for i in range(ac.shape[0]):
+        _lib.check(f(_lib.NO_TRANS, _lib.NO_TRANS, m, n, k, 1.0, _ptr(ac[i]), max(k, 1), _ptr(bc[i]), max(n, 1), 0.0,
+                     _ptr(out[i]), max(n, 1)), 'matmul')
+    out = out.reshape(tuple(frame) + (m, n))
+    if av: out = out[..., 0, :]
+    if bv: out = out[..., 0]
+    out = out.astype(store, copy=False)
+    return out[()] if out.ndim == 0 else out
+
+
+class Lu(NamedTuple):
+    "Packed LU factors in rustymath's order, where row `j` holds column `j`, and row positions (python/rustymath/linalg.py:26-51)"
+    factors: np.ndarray
+    p: np.ndarray
+
+    @property
+    def l(self): return lu_lower(self.factors)
+
+    @property
+    def u(self): return lu_upper(self.factors)
+
+    def solve(self, b):
+        "Solve for `b` using these factors, without factoring again or applying a cutoff"
+        return lu_solve(self.factors, self.p, b)
+
+    def determinant(self):
+        "The determinant from these factors"
+        return lu_det(self.factors, self.p)
+
+    def log_determinant(self):
+        "The determinant's sign and the natural logarithm of its magnitude, from these factors"
+        return lu_slogdet(self.factors, self.p)
+
+
+def lu(a):
+    """The packed LU factors of each square matrix, with partial pivoting, and the positions of its rows
+    (src/linalg/stacked.rs:174-187). Row `i` of `l @ u` is row `p[i]` of the matrix."""
+    a = np.asarray(a)
+    compute, store = _compute_dtype(a)
+    if a.ndim < 2 or a.shape[-1] != a.shape[-2]: raise ValueError(f'lu needs square matrices, not {a.shape[-2:]}')
+    n = a.shape[-1]
+    frame = a.shape[:-2]
+    cells = np.ascontiguousarray(a.reshape((-1, n, n)).astype(compute, copy=False))
+    factors = np.empty_like(cells)
+    perm = np.empty((cells.shape[0], n), np.int64)
+    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_factor')
+    for i in range(cells.shape[0]):
+        _lib.check(f(_ptr(cells[i]), n, _ptr(factors[i]), _ptr(perm[i]), None), 'lu')
+    return Lu(factors.reshape(frame + (n, n)).astype(store, copy=False), perm.reshape(frame + (n,)))
+
+
+def _positions(p, n):
+    p = np.asarray(p)
+    if p.shape[-1:] != (n,): raise ValueError(f'needs {n} row positions for each matrix')
+    if np.issubdtype(p.dtype, np.floating):
+        if not (p == np.round(p)).all(): raise IndexError('row positions must be whole numbers')
+    q = p.astype(np.int64)
+    if ((q < 0) | (q >= n)).any(): raise IndexError(f'row positions must lie from 0 to {n - 1}')
+    return q
+
+
+def _rhs(b, n):
+    "(core dims, k, vector) of a right-hand side beside n-row matrices (src/linalg/stacked.rs:497-507)."
+    if b.ndim == 0: raise ValueError('a right-hand side needs at least one axis')
+    if b.ndim == 1:
+        if b.shape[0] != n: raise ValueError(f"the right-hand side has {b.shape[0]} rows, not {n}")
+        return 1, 1, True
+    if b.shape[-2] != n: raise ValueError(f"the right-hand side has {b.shape[-2]} rows, not {n}")
+    return 2, b.shape[-1], False
+
+
+def lu_solve(factors, p, b):
+    """`x` with `a @ x == b`, from the packed LU factors and row positions of each matrix `a`, as `lu` gives them,
+    without factoring again (src/linalg/stacked.rs:189-204)."""
+    factors, b = np.asarray(factors), np.asarray(b)
+    compute, store = _compute_dtype(factors, b)
+    n = factors.shape[-1]
+    perm = _positions(p, n)
+    core, k, vector = _rhs(b, n)
+    b2 = b[:, None] if vector else b
+    ff, fb, frame = _pair_frames(factors.shape[:-2], b2.shape[:-2])
+    fc = _cells(factors.reshape(ff + (n, n)).astype(compute, copy=False), frame, (n, n))
+    pc = _cells(perm.reshape(ff + (n,)), frame, (n,))
+    bc = _cells(b2.reshape(fb + (n, k)).astype(compute, copy=False), frame, (n, k))
+    out = np.zeros_like(bc)
+    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_solve')
+    for i in range(fc.shape[0]):
+        _lib.check(f(_ptr(fc[i]), _ptr(pc[i]), n, _ptr(bc[i]), k, _ptr(out[i])), 'lu_solve')
+    out = out.reshape(tuple(frame) + (n, k))
+    if vector: out = out[..., 0]
+    return out.astype(store, copy=False)
+
+
+def solve(a, b):
+    """The solution `x` of `a @ x == b` for each matrix `a` and right-hand side `b`, a vector or matrices
+    (src/linalg/stacked.rs:88-101, core/src/linalg/factor.rs:207-226)."""
+    a, b = np.asarray(a), np.asarray(b)
+    compute, store = _compute_dtype(a, b)
+    if a.ndim < 2: raise ValueError('solve needs matrices')
+    m, n = a.shape[-2:]
+    if m != n: raise NotImplementedError('non-square least squares is outside this library')
+    core, k, vector = _rhs(b, n)
+    b2 = b[:, None] if vector else b
+    fa, fb, frame = _pair_frames(a.shape[:-2], b2.shape[:-2])
+    ac = _cells(a.reshape(fa + (n, n)).astype(compute, copy=False), frame, (n, n))
+    bc = _cells(b2.reshape(fb + (n, k)).astype(compute, copy=False), frame, (n, k))
+    out = np.zeros_like(bc)
+    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'solve')
+    regular = ctypes.c_int(0)
+    for i in range(ac.shape[0]):
+        _lib.check(f(_ptr(ac[i]), n, _ptr(bc[i]), k, _ptr(out[i]), ctypes.byref(regular)), 'solve')
+        if not regular.value:
+            raise NotImplementedError('a negligible LU pivot: rustymath solves this matrix by least squares, '
+                                      'which is outside this library')
+    out = out.reshape(tuple(frame) + (n, k))
+    if vector: out = out[..., 0]
+    return out.astype(store, copy=False)
+
+
+def lu_lower(factors):
+    "The unit lower triangle of each matrix's packed LU factors (core/src/linalg/factor.rs:158-165)."
+    f = np.swapaxes(np.asarray(factors), -1, -2)
+    n = f.shape[-1]
+    return np.tril(f, -1) + np.eye(n, dtype=f.dtype)
+
+
+def lu_upper(factors):
+    "The upper triangle of each matrix's packed LU factors (core/src/linalg/factor.rs:166-172)."
+    return np.triu(np.swapaxes(np.asarray(factors), -1, -2))
+
+
+def _odd(perm):
+    "Whether a permutation is odd, by counting its cycles at their smallest positions (factor.rs:186-194)."
+    n, cycles = len(perm), 0
+    for s in range(n):
+        i = perm[s]
+        while i > s: i = perm[i]
+        cycles += i == s
+    return (n - cycles) % 2 == 1
+
+
+def lu_det(factors, p):
+    """The determinant from packed factors and row positions: U's diagonal multiplied in order from 1, negated for an
+    odd permutation (factor.rs:174). An O(n) fold over the factors the GPU computed."""
+    factors = np.asarray(factors); n = factors.shape[-1]
+    perm = _positions(p, n)
+    fc = factors.reshape(-1, n, n); pc = perm.reshape(-1, n)
+    out = np.empty(fc.shape[0], factors.dtype)
+    for i in range(fc.shape[0]):
+        d = factors.dtype.type(1)
+        for j in range(n): d = d * fc[i, j, j]
+        out[i] = -d if _odd(pc[i]) else d
+    out = out.reshape(factors.shape[:-2])
+    return out[()] if out.ndim == 0 else out
+
+
+def lu_slogdet(factors, p):
+    "The determinant's sign and the log of its magnitude from packed factors and row positions (factor.rs:176-182)."
+    factors = np.asarray(factors); n = factors.shape[-1]
+    t = factors.dtype.type
+    perm = _positions(p, n)
+    fc = factors.reshape(-1, n, n); pc = perm.reshape(-1, n)
+    signs = np.empty(fc.shape[0], factors.dtype); logs = np.empty(fc.shape[0], factors.dtype)
+    for i in range(fc.shape[0]):
+        s, l = (t(-1) if _odd(pc[i]) else t(1)), t(0)
+        for j in range(n):
+            d = fc[i, j, j]; m = abs(d)
+            if m == 0: s, l = t(0), t(-np.inf)
+            else: s, l = s * (d * (t(1) / m)), l + np.log(m)
+        signs[i], logs[i] = s, l
+    shape = factors.shape[:-2]
+    signs, logs = signs.reshape(shape), logs.reshape(shape)
+    return (signs[()], logs[()]) if not shape else (signs, logs)

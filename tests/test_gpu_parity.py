@@ -1,0 +1,184 @@
+"""GPU parity of the products, the LU factorization and the solves against the oracle, through the C ABI.
+
+The bar: the same bits as the oracle's restatement of rustymath wherever the arithmetic is defined item by item
+(exact products, pivots, factors, solves, including signed zeros), and the reference's own invariants (residual
+bounds, power-of-two scaling, zero columns) beside them. Sizes stay where the oracle finishes in well under a
+second.
+"""
+import math
+import numpy as np
+import pytest
+
+import oracle
+from support import FOLD_SIZES, integer_operands, random, norm, distance, backward_error
+from test_oracle import bits_equal
+
+pytestmark = pytest.mark.gpu
+EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps}
+
+
+@pytest.fixture(scope='module')
+def la():
+    from rustymath_amd import linalg
+    return linalg
+
+
+def hs_gemm(a, b, alpha=1.0, beta=0.0, c=None, opa='N', opb='N'):
+    import ctypes
+    from rustymath_amd import _lib
+    a = np.ascontiguousarray(a); b = np.ascontiguousarray(b, dtype=a.dtype)
+    m, k = (a.shape[1], a.shape[0]) if opa != 'N' else a.shape
+    n = b.shape[0] if opb != 'N' else b.shape[1]
+    c = np.zeros((m, n), a.dtype) if c is None else np.ascontiguousarray(c)
+    f = getattr(_lib.load(), 'hs_dgemm' if a.dtype == np.float64 else 'hs_sgemm')
+    p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    _lib.check(f(oracle.OPS[opa], oracle.OPS[opb], m, n, k, alpha, p(a), max(a.shape[1], 1), p(b), max(b.shape[1], 1),
+                 beta, p(c), max(n, 1)), 'gemm')
+    return c
+
+
+@pytest.mark.parametrize('size', FOLD_SIZES)
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_products_match_folds_in_order(la, size, dtype):
+    "tests/linalg.rs:160-177 on the GPU: exact small-integer products, whatever the summation order."
+    m, k, n = size
+    a, b = integer_operands(m, k, n)
+    want = (a.astype(np.int64) @ b.astype(np.int64)).astype(dtype)
+    assert bits_equal(la.matmul(a.astype(dtype), b.astype(dtype)), want)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('ops', ['NN', 'NT', 'TN', 'TT'])
+@pytest.mark.parametrize('ab', [(1.0, 0.0), (-1.0, 1.0), (2.0, 0.5)])
+def test_products_match_the_oracle_bit_for_bit(dtype, ops, ab):
+    """The MFMA product takes each result's terms in order of p, one rounding each, from 0 or beta c: the oracle's
+    (and rustymath's) fold, bit for bit, on random normal operands with ragged edges."""
+    rng = np.random.default_rng(7)
+    alpha, beta = ab
+    for m, k, n in [(131, 261, 37), (200, 1000, 129), (16, 16, 16), (3, 5, 2)]:
+        a = rng.standard_normal((m, k) if ops[0] == 'N' else (k, m)).astype(dtype)
+        b = rng.standard_normal((k, n) if ops[1] == 'N' else (n, k)).astype(dtype)
+        c0 = rng.standard_normal((m, n)).astype(dtype)
+        got = hs_gemm(a, b, alpha, beta, c0.copy(), ops[0], ops[1])
+        want = oracle.gemm(a, b, alpha, beta, c0.copy(), ops[0], ops[1])
+        assert bits_equal(got, want), f'{(m, k, n)}: max diff {np.abs(got - want).max()}'
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_vector_products_follow_by_vector(la, dtype):
+    "`by_vector` (core/src/linalg.rs:1020): matrix times vector in the reference's summation orders, bit for bit."
+    rng = np.random.default_rng(3)
+    for m, k in [(9, 5), (33, 100), (1000, 1003), (7, 8)]:
+        a = rng.standard_normal((m, k)).astype(dtype)
+        v = rng.standard_normal(k).astype(dtype)
+        w = rng.standard_normal(m).astype(dtype)
+        assert bits_equal(la.matmul(a, v), oracle.matmul(a, v))
+        assert bits_equal(la.matmul(w, a), oracle.matmul(w, a))
+
+
+def test_notebook_known_answers(la):
+    "nbs/linalg.ipynb cells 4, 6, 25, 27 and README.md:79-82 on the GPU, signed zeros included."
+    a = np.array([[1., 2], [2, 5]])
+    f = la.lu(a)
+    assert f.p.tolist() == [1, 0]
+    assert bits_equal(f.l, np.array([[1., 0], [0.5, 1]])) and bits_equal(f.u, np.array([[2., 5], [0, -0.5]]))
+    assert bits_equal(f.solve(np.array([[1., 2], [2, 1]])), np.array([[1., 8], [-0., -3]]))
+    assert f.determinant() == 1.0 and f.log_determinant() == (1.0, 0.0)
+    rhs = np.array([[1., 2], [2, 1]])
+    stack = np.stack([a, 2 * a, 3 * a])
+    x = la.solve(stack, rhs)
+    want = np.array([[[1.0, 8.0], [-0.0, -3.0]], [[0.5, 4.0], [-0.0, -1.5]], [[0.33333333, 2.6666667], [-0.0, -1.0]]])
+    assert np.allclose(x, want, rtol=1e-7, atol=0) and (np.signbit(x) == np.signbit(want)).all()
+    v = la.solve(stack, np.array([[1., 2], [2, 1], [1, 1]])[..., None])[..., 0]
+    wv = np.array([[1.0, -0.0], [4.0, -1.5], [1.0, -0.33333333]])
+    assert np.allclose(v, wv, rtol=1e-7, atol=0) and (np.signbit(v) == np.signbit(wv)).all()
+    assert np.allclose(la.solve(np.array([[4., 1], [1, 3]]), np.array([1., 2])), [0.090909091, 0.63636364], rtol=1e-8)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_subnormal_pivot(la, dtype):
+    "tests/linalg.rs:510-511: lu([p, 1; q, 1]).factors == [p, q/p, 1, 1 - q/p]."
+    tiny = np.finfo(dtype).tiny
+    p, q = dtype(tiny / 1024), dtype(tiny / 1048576)
+    f = la.lu(np.array([[p, 1], [q, 1]], dtype))
+    assert bits_equal(f.factors.reshape(-1), np.array([p, q / p, 1, dtype(1) - q / p], dtype))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [1, 2, 5, 16, 17, 31, 32, 33, 63, 64, 65, 71, 100, 129, 257, 530, 1030])
+def test_lu_matches_the_oracle_bit_for_bit(la, dtype, n):
+    "Pivots, factors and perm equal the oracle's (rustymath's lu_factor arithmetic) item for item."
+    a = random(n, n, n + 1, dtype)
+    f = la.lu(a)
+    want, perm, _ = oracle.lu(a)
+    assert (f.p == perm).all()
+    assert bits_equal(f.factors, want), f'max diff {np.abs(f.factors - want).max()}'
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_lu_edge_matrices_match_the_oracle(la, dtype):
+    "Zero columns (a zero pivot skips only the rest of its reference leaf), NaN, infinities and a permutation matrix."
+    rng = np.random.default_rng(11)
+    cases = []
+    z = rng.standard_normal((100, 100)); z[:, [3, 17, 50, 99]] = 0; cases.append(z)
+    z = rng.standard_normal((70, 70)); z[:, 5] = 0; z[7, 5] = np.nan; cases.append(z)
+    z = rng.standard_normal((70, 70)); z[3, 3] = np.inf; z[9, 40] = -np.inf; cases.append(z)
+    z = np.zeros((40, 40)); z[np.arange(40), (np.arange(40) * 7) % 40] = 1; cases.append(z)
+    cases.append(np.zeros((20, 20)))
+    for a in cases:
+        a = a.astype(dtype)
+        f = la.lu(a)
+        want, perm, _ = oracle.lu(a)
+        assert (f.p == perm).all() and bits_equal(f.factors, want)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_hard_matrices_scale_exactly(la, dtype):
+    "tests/linalg.rs:466-483 on the GPU: power-of-two scaling scales U exactly and leaves perm, L and solves alone."
+    n = 40
+    fi = np.finfo(dtype)
+    tiny = dtype(fi.tiny / fi.eps)
+    quartic = dtype(2.0 ** (int(math.log2(float(fi.max)) / 4) + 6))
+    a, b = random(n, n, 17, dtype), random(n, 3, 19, dtype)
+    f = la.lu(a)
+    for s in [tiny, dtype(1) / tiny, quartic]:
+        g = la.lu(a * s)
+        assert (g.p == f.p).all()
+        scaled = f.factors.copy()
+        iu = np.triu_indices(n)
+        scaled[iu[1], iu[0]] = f.factors[iu[1], iu[0]] * s
+        assert bits_equal(g.factors, scaled)
+        assert bits_equal(g.solve(b * s), f.solve(b))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [1, 5, 33, 71, 200, 530])
+@pytest.mark.parametrize('k', [1, 3, 16])
+def test_lu_solve_matches_the_oracle_bit_for_bit(la, dtype, n, k):
+    """lu_solve's substitution orders (increasing for L, decreasing with a quotient for one column of U, the
+    halving order for several) give the oracle's bits."""
+    a = random(n, n, n + 3, dtype)
+    b = random(n, k, 99, dtype)
+    f, perm, _ = oracle.lu(a)
+    got = la.lu_solve(f, perm, b)
+    want = oracle.lu_solve(f, perm, b)
+    assert bits_equal(got, want), f'max diff {np.abs(got - want).max()}'
+    got1 = la.lu_solve(f, perm, b[:, 0])
+    assert bits_equal(got1, oracle.lu_solve(f, perm, b[:, 0]))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [530, 1030])
+def test_blocked_sizes_have_small_residuals(la, dtype, n):
+    "tests/linalg.rs:569-590 on the GPU, plus the north-star bound 10·n·eps on the solve's backward error."
+    a = random(n, n, n, dtype); aw = a.astype(np.float64)
+    b = random(n, 3, 5, dtype)
+    x = la.solve(a, b).astype(np.float64)
+    assert distance(aw @ x, b, norm(aw) * norm(x)) < 4 * EPS[dtype] * n
+    assert backward_error(aw, x, b) <= 10 * n * EPS[dtype]
+
+
+def test_singular_matrix_is_not_solved_by_lu(la):
+    "factor.rs:221-226: a negligible pivot sends rustymath to least squares, which this library leaves out loudly."
+    with pytest.raises(NotImplementedError):
+        la.solve(np.array([[1., 2], [2, 4]]), np.array([1., 2]))

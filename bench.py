@@ -1,0 +1,148 @@
+"""Benchmark: FP64-accurate solve GFLOP/s on one MI355X, low-precision LU + FP64 iterative refinement against the plain
+FP64 LU, on the mixed-precision workload (n = 16384, A = U diag(s) Vᵀ with U, V products of 8 Householder reflectors
+and s log-spaced from 1 to 1e-6, condition number 1e6; b iid N(0, 1)).
+
+One step = one full solve from a resident A and b: cast A to FP32, LU on MFMA, solve, FP64 residuals and corrections
+until the backward error ||b - A x|| / (||A|| ||x||) is at most 10·n·eps64. `value` is (2/3 n³ + 2 n²) / step time
+(the HPL-MxP convention: the same FP64 flop count for every path, so speedups are time ratios). The plain FP64 path
+(copy A, LU, solve) is timed on the same matrix for the comparison. Prints one JSON line.
+"""
+import argparse, json, math, os, sys, time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'FP64-accurate solve GFLOP/s: low-prec LU+IR vs plain FP64 LU; batched solves/s'
+PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'hbm': 8.0e12}
+
+
+def householder_matrix(n, seed, cond=1e6, reflectors=8):
+    """A = U diag(s) Vᵀ with U and V each a product of `reflectors` Householder reflectors H = I - 2 v vᵀ / vᵀv (v iid
+    N(0, 1)) and s log-spaced from 1 to 1/cond, built in FP64 on the GPU by rank-1 updates, never forming U or V."""
+    import torch
+    g = torch.Generator(device='cuda'); g.manual_seed(seed)
+    s = torch.logspace(0, -math.log10(cond), n, dtype=torch.float64, device='cuda')
+    a = torch.diag(s)
+    for _ in range(reflectors):       # A <- A H (columns): A - 2 (A v) vᵀ / vᵀv
+        v = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+        a.addr_(a @ v, v, alpha=-2.0 / float(v @ v))
+    for _ in range(reflectors):       # A <- H A (rows): A - 2 v (vᵀ A) / vᵀv
+        v = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+        a.addr_(v, v @ a, alpha=-2.0 / float(v @ v))
+    b = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+    return a.contiguous(), b
+
+
+def cpu_baseline(n=6144, runs=3, threads=16):
+    """The oracle's LU solve (the C restatement of rustymath's blocked CPU kernels) on a bounded sample: an FP64 N(0, 1)
+    system of order n on `threads` host threads, best of `runs`, in GFLOP/s (2/3 n³ + 2 n²)."""
+    import numpy as np
+    import oracle
+    oracle.set_threads(threads)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((n, n)); b = rng.standard_normal(n)
+    best = math.inf
+    for _ in range(runs):
+        t0 = time.perf_counter(); oracle.solve(a, b); best = min(best, time.perf_counter() - t0)
+    # Fidelity point beside the reference's published single-thread figure (lu(256).solve at 24.5 GFLOP/s on a
+    # Ryzen 9 7950X3D, tests/speed.txt:149-150): the same restatement at n = 256 on one thread, best of 5.
+    oracle.set_threads(1)
+    a1 = rng.standard_normal((256, 256)); b1 = rng.standard_normal(256)
+    one = math.inf
+    for _ in range(5):
+        t0 = time.perf_counter(); oracle.solve(a1, b1); one = min(one, time.perf_counter() - t0)
+    oracle.set_threads(threads)
+    return {'value': round((2 / 3 * n ** 3 + 2 * n * n) / best / 1e9, 2), 'unit': 'GFLOP/s', 'cores': threads, 'kind': 'port',
+            'single_thread_n256_gflops': round((2 / 3 * 256 ** 3 + 2 * 256 * 256) / one / 1e9, 2),
+            'sample': f'oracle (C restatement of rustymath lu_factor/lu_solve, AVX2 6x8 tiles) FP64 solve n={n}, 1 RHS, '
+                      f'best of {runs}, {threads} threads'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--n', type=int, default=16384)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU: this build measures the single-GPU path only (asked for --gpus {args.gpus})',
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from rustymath_amd import device as dv, _lib
+    n = args.n
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    a, b = householder_matrix(n, seed=3)
+    torch.cuda.synchronize()
+    tol = 10 * n * torch.finfo(torch.float64).eps
+    flops = 2 / 3 * n ** 3 + 2 * n * n
+
+    # Plain FP64: copy A (the LU works in place), factor, solve.
+    work = torch.empty_like(a)
+    def fp64_step():
+        work.copy_(a)
+        f, perm = dv.getrf(work)
+        x = b.reshape(n, 1).clone()
+        dv.getrs(f, perm, x)
+        return x.reshape(n)
+    x64 = fp64_step(); torch.cuda.synchronize()
+    t0 = time.perf_counter(); x64 = fp64_step(); torch.cuda.synchronize(); t64 = time.perf_counter() - t0
+    r64 = ((b - a @ x64).norm() / (a.norm() * x64.norm())).item()
+
+    # Low precision + refinement: the timed steps.
+    for _ in range(args.warmup):
+        dv.dsgesv(a, b, tol=tol)
+    torch.cuda.synchronize()
+    lib = _lib.load()
+    import ctypes
+    prof = (ctypes.c_int64 * 8)()
+    times, iters, berr, x = [], 0, 0.0, None
+    for _ in range(args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, iters, berr = dv.dsgesv(a, b, tol=tol)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t = sum(times) / len(times)
+    fwd = ((x - x64).norm() / x64.norm()).item()
+
+    # The dominant kernel's roofline: the FP32 MFMA trailing product of the factorization, timed live with HIP events
+    # on its stream by the library over one more factorization (hs_profile_gemm).
+    roof = None
+    try:
+        gms, gl, gf = ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0)
+        lib.hs_profile_gemm(1, None, None, None)
+        dv.dsgesv(a, b, tol=tol); torch.cuda.synchronize()
+        lib.hs_profile_gemm(0, ctypes.byref(gms), ctypes.byref(gl), ctypes.byref(gf))
+        if gl.value:
+            per_launch = gf.value / gl.value
+            avg_s = gms.value / 1e3 / gl.value
+            achieved = per_launch / avg_s / 1e12
+            roof = {'bound': 'mfma', 'achieved': round(achieved, 2), 'peak': PEAK['fp32_mfma'] / 1e12, 'unit': 'TFLOP/s',
+                    'frac': round(achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': None,
+                    'kernel': 'gemm_mfma<float> (LU trailing update)', 'launches': gl.value,
+                    'flop_per_launch': round(per_launch), 'avg_launch_us': round(avg_s * 1e6, 2)}
+    except AttributeError:
+        pass
+
+    out = {
+        'metric': METRIC, 'value': round(flops / t / 1e9, 2), 'unit': 'GFLOP/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(t * 1e3, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32 LU on MFMA + f64 refinement', 'data': 'synthetic',
+        'config': {'workload': f'mixed precision solve, n={n}, A = U diag(s) V^T (8+8 Householder reflectors, '
+                               f's = logspace(0, -6)), cond 1e6, b N(0,1), seed 3', 'n': n, 'rhs': 1},
+        'fp64_lu_solve_gflops': round(flops / t64 / 1e9, 2), 'speedup_vs_fp64_lu': round(t64 / t, 3),
+        'refinement_solves': iters, 'backward_error': berr, 'backward_error_fp64_lu': r64, 'tolerance': tol,
+        'forward_error_vs_fp64_lu': fwd, 'roofline': roof,
+    }
+    if not args.no_cpu:
+        out['cpu_baseline'] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

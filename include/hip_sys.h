@@ -12,8 +12,8 @@
  *     A zero pivot is not an error (factor.rs:606); NaN propagates;
  *   - the caller owns every buffer it passes. Host-pointer entry points copy in and out through a
  *     device-memory pool the library keeps across calls (as LuWork keeps buffers, factor.rs:196-201).
- *     hs_dev_* entry points take device pointers and a hipStream_t (NULL = the calling thread's own
- *     stream) and leave results in device memory;
+ *     hs_dev_* entry points take device pointers and a hipStream_t (NULL = the null stream) and
+ *     leave results in device memory;
  *   - thread safety: each host thread gets its own stream; calls from several threads may overlap.
  *
  * The arithmetic of every entry point is the reference's: a product takes its terms in order of p, one
@@ -66,7 +66,7 @@ int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x,
 int hs_ssolve(const float* a, int64_t n, const float* b, int64_t k, float* x, int* regular);
 
 /* Device-resident forms (no counterpart in the reference, whose API takes host slices): pointers are
- * device memory, `stream` a hipStream_t or NULL. */
+ * device memory, `stream` a hipStream_t (NULL = the null stream). */
 int hs_dev_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
                  const double* b, int64_t ldb, double beta, double* c, int64_t ldc, void* stream);
 int hs_dev_sgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
@@ -81,6 +81,22 @@ int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_
                   int64_t ldb, void* stream);
 int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t ldf, const int64_t* perm, float* b,
                   int64_t ldb, void* stream);
+
+/* Mixed precision (no counterpart in the reference, which has no iterative refinement): solves A x = b for
+ * the n×n f64 matrix a (row-major, lda = n) and one right-hand side b (device, n f64) by factoring a copy of A
+ * in FP32 on MFMA and refining x in FP64 on the GPU until the normwise backward error
+ * ||b - A x||_2 / (||A||_F ||x||_2) is at most tol or max_iter corrections have run. Writes x (device, n),
+ * the number of corrections and the final backward error. A is read, never written. */
+int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol,
+                  int* iters, double* berr, void* stream);
+
+/* Profiling (no counterpart in the reference): on = 1 clears and starts the panel kernel's per-phase timers
+ * (100 MHz ticks summed over launches: candidate search, publish, exchange wait, pivot bookkeeping, update),
+ * on = 0 stops them; `out` (8 values, may be NULL) receives the sums. */
+int hs_profile_panel(int on, int64_t* out);
+/* Profiling of the MFMA product: on = 1 clears and starts bracketing every product launch with HIP events on its
+ * stream, on = 0 stops; gives the launches' summed time (ms), their count and their algorithmic flops (2 m n k). */
+int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops);
 
 #ifdef __cplusplus
 }

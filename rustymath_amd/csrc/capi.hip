@@ -263,7 +263,7 @@ int dev_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T
     bool ta, tb;
     int e = check_gemm<T>(opa, opb, m, n, k, lda, ldb, ldc, ta, tb);
     if (e) return e;
-    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    hipStream_t s = (hipStream_t)stream;
     return gemm<T>(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, s);
 }
 
@@ -327,28 +327,39 @@ int hs_dev_sgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha,
 int hs_dev_dgetrf(int64_t n, double* a, int64_t lda, int64_t* perm, void* stream) {
     if (n < 0) return -1;
     if (lda < n) return -3;
-    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    hipStream_t s = (hipStream_t)stream;
     return dev_getrf_perm<double>(n, a, lda, perm, nullptr, nullptr, s);
 }
 int hs_dev_sgetrf(int64_t n, float* a, int64_t lda, int64_t* perm, void* stream) {
     if (n < 0) return -1;
     if (lda < n) return -3;
-    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    hipStream_t s = (hipStream_t)stream;
     return dev_getrf_perm<float>(n, a, lda, perm, nullptr, nullptr, s);
 }
 int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_t ldf, const int64_t* perm, double* b, int64_t ldb,
                   void* stream) {
     if (n < 0) return -2;
     if (k < 0) return -3;
-    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    hipStream_t s = (hipStream_t)stream;
     return getrs<double>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
 }
 int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t ldf, const int64_t* perm, float* b, int64_t ldb,
                   void* stream) {
     if (n < 0) return -2;
     if (k < 0) return -3;
-    hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+    hipStream_t s = (hipStream_t)stream;
     return getrs<float>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
 }
+
+int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol, int* iters,
+                  double* berr, void* stream) {
+    if (n < 0) return -1;
+    if (lda != n) return -3;
+    if (max_iter < 1) return -6;
+    return gesv_ir(n, a, lda, b, x, max_iter, tol, iters, berr, (hipStream_t)stream);
+}
+
+int hs_profile_panel(int on, int64_t* out) { return panel_profile(on, (long long*)out); }
+int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops) { return gemm_profile(on, ms, launches, flops); }
 
 }  // extern "C"

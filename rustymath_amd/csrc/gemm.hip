@@ -7,8 +7,39 @@
 // accumulator per result for the whole of k and feeds the MFMA unit in order of p, so it computes the
 // same chain of roundings; a k tail pads with +0 · (-0) = -0, which adds nothing to any value.
 #include "hs_kernels.h"
+#include <mutex>
+#include <vector>
 
 namespace hs {
+
+namespace {
+// The live profile of the MFMA product: while on, each launch is bracketed by HIP events on its own stream.
+struct GemmProfile {
+    std::mutex mu;
+    bool on = false;
+    struct Rec { hipEvent_t a, b; double flops; };
+    std::vector<Rec> recs;
+} g_prof;
+}  // namespace
+
+int gemm_profile(int on, double* ms, int64_t* launches, double* flops) {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    double total = 0, fl = 0;
+    int64_t count = 0;
+    for (auto& r : g_prof.recs) {
+        float e = 0;
+        HS_HIP(hipEventSynchronize(r.b));
+        HS_HIP(hipEventElapsedTime(&e, r.a, r.b));
+        total += e; fl += r.flops; ++count;
+        hipEventDestroy(r.a); hipEventDestroy(r.b);
+    }
+    g_prof.recs.clear();
+    if (ms) *ms = total;
+    if (launches) *launches = count;
+    if (flops) *flops = fl;
+    g_prof.on = on != 0;
+    return OK;
+}
 
 namespace {
 
@@ -59,7 +90,7 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, in
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                int64_t r = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + e, col = n0 + wn * 64 + j * 16 + (lane & 15);
+                int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
                 T v = T(0);
                 if (beta != T(0) && r < m && col < n) {
                     v = C[r * ldc + col];
@@ -148,7 +179,7 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, in
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                int64_t r = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + e, col = n0 + wn * 64 + j * 16 + (lane & 15);
+                int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
                 if (r < m && col < n) C[r * ldc + col] = acc[i][j][e];
             }
 }
@@ -256,12 +287,28 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
     unsigned grid = (unsigned)(((total + 7) / 8) * 8);
 #define HS_LAUNCH(TA_, TB_) \
     gemm_mfma<T, TA_, TB_><<<grid, THREADS, 0, s>>>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tiles_m, tiles_n)
+    GemmProfile::Rec rec{nullptr, nullptr, 2.0 * (double)m * (double)n * (double)k};
+    bool profiled = false;
+    {
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        profiled = g_prof.on;
+    }
+    if (profiled) {
+        HS_HIP(hipEventCreate(&rec.a));
+        HS_HIP(hipEventCreate(&rec.b));
+        HS_HIP(hipEventRecord(rec.a, s));
+    }
     if (!ta && !tb) HS_LAUNCH(false, false);
     else if (!ta && tb) HS_LAUNCH(false, true);
     else if (ta && !tb) HS_LAUNCH(true, false);
     else HS_LAUNCH(true, true);
 #undef HS_LAUNCH
     HS_HIP(hipGetLastError());
+    if (profiled) {
+        HS_HIP(hipEventRecord(rec.b, s));
+        std::lock_guard<std::mutex> lk(g_prof.mu);
+        g_prof.recs.push_back(rec);
+    }
     return OK;
 }
 

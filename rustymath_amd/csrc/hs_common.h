@@ -53,8 +53,8 @@ __device__ __forceinline__ void static_for(F&& f) {
 }  // namespace hs
 
 // MFMA operand traits. Each 16x16x4 instruction takes one item of A (16x4) and one of B (4x16) per
-// lane: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; the 16x16 result gives lane l the four
-// items C[4 * (l / 16) + e][l % 16], e = 0..3.
+// lane: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]. The 16x16 result gives lane l four items in
+// column l % 16: rows 4 (l / 16) + e for FP32, but rows l / 16 + 4 e for FP64 (measured on gfx950).
 namespace hs {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -62,12 +62,14 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef d4 acc;
+    static __device__ __forceinline__ int row(int lane, int e) { return (lane >> 4) + 4 * e; }
     static __device__ __forceinline__ acc step(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
 };
 template <> struct Mfma<float> {
     typedef f4 acc;
+    static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
     static __device__ __forceinline__ acc step(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }

@@ -33,6 +33,17 @@ template <typename T>
 int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int64_t* pivots, T* b, int64_t ldb,
           hipStream_t s);
 
+// Mixed-precision solve: FP32 factors on MFMA, FP64 iterative refinement on the GPU (ir.hip).
+int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol,
+            int* iters, double* berr, hipStream_t s);
+
+// The MFMA product's live profile: on = 1 clears and starts, 0 stops; gives the summed event time (ms), the
+// number of launches and their algorithmic flops (2 m n k each) since the last call.
+int gemm_profile(int on, double* ms, int64_t* launches, double* flops);
+
+// The panel kernel's per-phase profile: on = 1 clears and enables, 0 disables; out[8] gets tick sums.
+int panel_profile(int on, long long* out);
+
 // Converts between the device layouts: transpose (square or rectangular), f64 <-> f32 casts.
 template <typename T>
 int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s);

@@ -22,9 +22,8 @@ namespace hs {
 namespace {
 
 constexpr int PW = 64;            // widest panel
-constexpr int PT = 256;           // threads of a panel workgroup
-constexpr int MAX_WG = 64;        // panel workgroups: one wave polls one flag per lane
-constexpr int PANEL_ROWS = 256;   // rows of a panel workgroup (64 per wave)
+constexpr int PT = 512;           // threads (rows) of a panel workgroup
+constexpr int MAX_WG = 32;        // panel workgroups: one per CU of one XCD, one flag per lane of a wave
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
 
 template <typename T> struct Bits;
@@ -55,7 +54,6 @@ template <typename T> struct Slot {
     T val;
     long long row;
     T cand[PW];
-    T rowc[PW];
 };
 
 // Prefers the larger magnitude, then the smaller row: `largest`'s first maximum (factor.rs:476).
@@ -74,184 +72,175 @@ __device__ __forceinline__ void wave_best(T& v, long long& r) {
     }
 }
 
-// Factors the panel of columns [c0, c0 + w) over rows [c0, c0 + rows) of the row-major `a`. Workgroup g
-// holds rows [4R g, 4R (g + 1)) in registers, wave q of it rows [4R g + R q, + R), lane l column l: so a
-// column's items sit in one lane of each wave, a row's in one wave, and every step of a column loads,
-// stores and updates whole rows at once. Every column takes one exchange between the workgroups: each
-// publishes its best candidate, waits for every other's, and all pick the same pivot. Writes
-// pivots[c0 + c] (absolute rows) and `moves`, the net row permutation of the panel's swaps (pairs dst,
-// src), for the row-swap kernel. `leaf_end[c]` is the end of the reference leaf of column c.
-template <typename T, int R>
+// Wall-clock ticks (100 MHz) for the optional per-phase profile of the panel kernel.
+__device__ __forceinline__ long long ticks() { return __builtin_amdgcn_s_memrealtime(); }
+
+// Factors the panel of columns [c0, c0 + w) over rows [c0, c0 + rows) of the row-major `a`.
+//
+// One thread holds one row in registers, shifted so that v[0] is always the current column: at column c,
+// v[j] is the row's item in column c + j. The rows never move between threads. A swap only relabels
+// positions (each thread knows its row's current position `pos`), and every finished item (the row's
+// multiplier at its column, or the pivot row's U items at its pivot step) goes to `scratch[column][row]`,
+// from which each row is written once, at its final position, when the panel is done.
+//
+// With one workgroup, a column's pivot takes workgroup barriers only. With several (launch indices 0, 8,
+// 16, ..., one XCD under round-robin dispatch, one workgroup per CU, so all are resident; the other launch
+// indices return at once), every column takes one exchange: each workgroup publishes its best candidate
+// (magnitude, position, the candidate row) with device-coherent (sc1) stores, raises its flag, waits for
+// every other flag, and all pick the same pivot. Candidates compare by magnitude, NaN never chosen, then by
+// position: `largest`'s first maximum (core/src/linalg/factor.rs:476).
+//
+// Writes pivots[c0 + c] (absolute rows) and `moves` (count, then pairs dst, src: the rows whose position
+// changed), for the row-swap kernel. `leaf_end[c]` is the end of the reference leaf of column c; a zero
+// pivot leaves its column undivided and skips the rest of that leaf (factor.rs:606). Where `prof` is not
+// null, workgroup 0 adds the ticks of each phase of the column loop into prof[0..7].
+template <typename T>
 __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int64_t rows, int w,
                                                    int64_t* __restrict__ pivots, long long* __restrict__ moves,
-                                                   const int* __restrict__ leaf_end, Slot<T>* slots,
-                                                   unsigned long long* flags, unsigned long long epoch, int nwg,
-                                                   int* status) {
-    const int tid = threadIdx.x, col = tid & 63, grp = tid >> 6, g = blockIdx.x;
-    constexpr int64_t PER = 4 * (int64_t)R;
-    const int64_t base = (int64_t)g * PER + (int64_t)grp * R;  // this wave's first row, relative to c0
+                                                   const int* __restrict__ leaf_end, T* __restrict__ scratch,
+                                                   char* state, unsigned long long epoch, int nwg, int* status,
+                                                   long long* prof) {
+    if (nwg > 1 && (blockIdx.x & 7)) return;
+    const int g = nwg > 1 ? (int)(blockIdx.x >> 3) : 0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int WAVES = PT / 64;
+    unsigned long long* flags = reinterpret_cast<unsigned long long*>(state);
+    Slot<T>* slots = reinterpret_cast<Slot<T>*>(state + MAX_WG * sizeof(unsigned long long));
+    const bool timed = prof != nullptr && g == 0 && threadIdx.x == 0;
+    long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tl = timed ? ticks() : 0;
+#define HS_PHASE(q) do { if (timed) { long long t_ = ticks(); tp[q] += t_ - tl; tl = t_; } } while (0)
 
-    T v[R];
+    const int64_t phys = (int64_t)g * PT + tid;  // this thread's row, relative to c0
+    const bool has = phys < rows;
+    int64_t pos = phys;
+    T v[PW];
+    {
+        const T* src = a + (c0 + phys) * lda + c0;
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-        const int64_t r = base + j;
-        v[j] = (r < rows && col < w) ? a[(c0 + r) * lda + c0 + col] : T(0);
+        for (int j = 0; j < PW; ++j) v[j] = (has && j < w) ? src[j] : T(0);
     }
 
-    __shared__ T s_val[4];
-    __shared__ long long s_row[4];
+    __shared__ T s_val[WAVES];
+    __shared__ long long s_key[WAVES];
     __shared__ T s_piv[PW];
-    __shared__ T s_rowc[PW];
-    __shared__ T s_l[4 * R];
     __shared__ long long s_p;
-    __shared__ int s_abort;
-    // The net permutation of the swaps so far (workgroup 0 only): content of block position q, and the
-    // positions outside the block that a swap has touched with their content.
-    __shared__ long long s_blk[PW];
-    __shared__ long long s_opos[PW];
-    __shared__ long long s_oval[PW];
-    __shared__ int s_nout;
-    if (tid < PW) s_blk[tid] = c0 + tid;
-    if (tid == 0) { s_nout = 0; s_abort = 0; }
-    __syncthreads();
+    __shared__ int s_gp, s_abort;
+    if (tid == 0) { s_abort = 0; if (g == 0) moves[0] = 0; }
+    const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
 
+#pragma clang loop unroll(disable)
     for (int c = 0; c < w; ++c) {
-        // Local candidate: the first row at or below c with the greatest magnitude, NaN never chosen.
-        if (col == c) {
-            T best = T(-1);
-            long long brow = 0x7fffffffffffffffll;
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int64_t r = base + j;
-                if (r < rows && r >= c) {
-                    T x = fabs(v[j]);
-                    if (x != x) x = T(-1);
-                    better(best, brow, x, (long long)r);
-                }
-            }
-            s_val[grp] = best;
-            s_row[grp] = brow;
+        // This workgroup's candidate.
+        T val = T(-1);
+        long long key = 0x7fffffffffffffffll;
+        if (has && pos >= c) {
+            T x = fabs(v[0]);
+            val = x != x ? T(-1) : x;
+            key = pos;
         }
+        wave_best(val, key);
+        if (lane == 0) { s_val[wave] = val; s_key[wave] = key; }
         __syncthreads();
+        HS_PHASE(0);
         T wb = s_val[0];
-        long long wr = s_row[0];
+        long long wk = s_key[0];
 #pragma unroll
-        for (int q = 1; q < 4; ++q) better(wb, wr, s_val[q], s_row[q]);
+        for (int q = 1; q < WAVES; ++q) better(wb, wk, s_val[q], s_key[q]);
+        if (has && pos == wk) {
+#pragma unroll
+            for (int j = 0; j < PW; ++j) s_piv[j] = v[j];
+        }
+        __syncthreads();
+        HS_PHASE(1);
 
-        // Publish: the candidate row and, from the wave holding it, the row now at position c.
-        Slot<T>* mine = slots + (c & 1) * MAX_WG + g;
-        if (col < w) {
-            const int64_t jw = wr - base, jc = (int64_t)c - base;
-            T xw = T(0), xc = T(0);
+        long long p = wk;
+        if (nwg > 1) {
+            Slot<T>* mine = slots + (c & 1) * MAX_WG + g;
+            if (tid < w - c) put(&mine->cand[tid], s_piv[tid]);
+            if (tid == 0) { put(&mine->val, wb); put64(&mine->row, wk); }
+            __syncthreads();  // every store of this workgroup has completed (s_waitcnt before the barrier)
+            const unsigned long long target = epoch * 1024ull + (unsigned long long)c + 1ull;
+            if (tid == 0) __hip_atomic_store(&flags[g], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (wave == 0) {
+                T gv = T(-1);
+                long long gk = 0x7fffffffffffffffll;
+                bool ok = true;
+                if (lane < nwg) {
+                    uint64_t spins = 0;
+                    while (__hip_atomic_load(&flags[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                        if (++spins > SPIN_LIMIT) { ok = false; break; }
+                    }
+                    if (ok) {
+                        const Slot<T>* sl = slots + (c & 1) * MAX_WG + lane;
+                        gv = get(&sl->val);
+                        gk = get64(&sl->row);
+                    }
+                }
+                const bool all = __all(ok);
+                // Reduce (magnitude, position) and remember which workgroup holds the winner.
+                long long who = lane;
 #pragma unroll
-            for (int j = 0; j < R; ++j) { if (j == jw) xw = v[j]; if (j == jc) xc = v[j]; }
-            if (jw >= 0 && jw < R) put(&mine->cand[col], xw);
-            if (jc >= 0 && jc < R) put(&mine->rowc[col], xc);
+                for (int off = 32; off > 0; off >>= 1) {
+                    T v2 = __shfl_xor(gv, off);
+                    long long k2 = __shfl_xor(gk, off), w2 = __shfl_xor(who, off);
+                    if (v2 > gv || (v2 == gv && k2 < gk)) { gv = v2; gk = k2; who = w2; }
+                }
+                if (lane == 0) {
+                    s_p = gk;
+                    s_gp = (int)who;
+                    if (!all) { s_abort = 1; atomicExch(status, ERR_LIMIT); }
+                }
+            }
+            __syncthreads();
+            HS_PHASE(2);
+            if (s_abort) return;
+            p = s_p;
+            if (s_gp != g && tid < w - c) s_piv[tid] = get(&slots[(c & 1) * MAX_WG + s_gp].cand[tid]);
+            __syncthreads();
+            HS_PHASE(3);
         }
-        if (tid == 0) { put(&mine->val, wb); put64(&mine->row, wr); }
-        __syncthreads();
-        const unsigned long long target = epoch * 1024ull + (unsigned long long)c + 1ull;
-        if (tid == 0) __hip_atomic_store(&flags[g], target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 
-        // Wait for every workgroup's candidate, then agree on the pivot.
-        if (grp == 0) {
-            const int lane = col;
-            T gv = T(-1);
-            long long gr = 0x7fffffffffffffffll;
-            bool ok = true;
-            if (lane < nwg) {
-                uint64_t spins = 0;
-                while (__hip_atomic_load(&flags[lane], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                    if (++spins > SPIN_LIMIT) { ok = false; break; }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-                if (ok) {
-                    const Slot<T>* sl = slots + (c & 1) * MAX_WG + lane;
-                    gv = get(&sl->val);
-                    gr = get64(&sl->row);
-                }
-            }
-            const bool all = __all(ok);
-            wave_best(gv, gr);
-            if (lane == 0) {
-                s_p = gr;
-                if (!all) { s_abort = 1; atomicExch(status, ERR_LIMIT); }
-            }
+        // The winner takes position c and its U items are final; the row at position c moves to p.
+        const T pivot = s_piv[0];
+        if (has && pos == p) {
+#pragma unroll
+            for (int j = 0; j < PW; ++j)
+                if (j < w - c) scratch[(int64_t)(c + j) * rows + phys] = v[j];
+            pos = c;
+        } else if (has && pos == c) {
+            pos = p;
         }
-        __syncthreads();
-        if (s_abort) return;
-        const long long p = s_p;
-        const int gp = (int)(p / PER), gc = (int)(c / PER);
-        if (tid < w) {
-            s_piv[tid] = get(&slots[(c & 1) * MAX_WG + gp].cand[tid]);
-            s_rowc[tid] = get(&slots[(c & 1) * MAX_WG + gc].rowc[tid]);
-        }
-        if (g == 0 && tid == 0) {
-            pivots[c0 + c] = c0 + p;
-            // Track the net permutation: swap the contents of positions c0 + c and c0 + p.
-            const long long pos = c0 + p;
-            const long long x = s_blk[c];
-            if (p < w) { s_blk[c] = s_blk[p]; s_blk[p] = x; }
-            else {
-                int k = 0;
-                while (k < s_nout && s_opos[k] != pos) ++k;
-                const long long y = k < s_nout ? s_oval[k] : pos;
-                if (k == s_nout) { s_opos[k] = pos; ++s_nout; }
-                s_oval[k] = x;
-                s_blk[c] = y;
-            }
-        }
-        __syncthreads();
+        if (g == 0 && tid == 0) pivots[c0 + c] = c0 + p;
 
-        // Swap rows c and p, then divide column c below the pivot and update the columns after it.
-        const T pivot = s_piv[c];
-        const bool zero = pivot == T(0);
-        const bool normal = fabs(pivot) >= (sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f));
-        const T rcp = zero ? T(0) : T(1) / pivot;
-        if (col < w) {
-            const T pc = s_piv[col], rc = s_rowc[col];
+        // Rows below the pivot: divide (a quotient for a subnormal pivot, nothing for a zero one), record the
+        // multiplier, update the rest of the row and shift it one column left.
+        if (has && pos > c) {
+            const bool zero = pivot == T(0);
+            T l = v[0];
+            if (!zero) l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
+            scratch[(int64_t)c * rows + phys] = l;
+            const int skip = zero ? leaf_end[c0 + c] - (int)c0 - c : 1;  // columns c + j, j < skip, not updated
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int64_t r = base + j;
-                if (r == c) v[j] = pc;
-                else if (r == p) v[j] = rc;
+            for (int j = 1; j < PW; ++j) {
+                const T nv = __builtin_fma(l, -s_piv[j], v[j]);
+                v[j - 1] = j >= skip ? nv : v[j];
             }
+            v[PW - 1] = T(0);
         }
-        if (col == c) {
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int64_t r = base + j;
-                T l = v[j];
-                if (r < rows && r > c && !zero) {
-                    l = normal ? l * rcp : l / pivot;
-                    v[j] = l;
-                }
-                s_l[grp * R + j] = l;
-            }
-        }
-        __syncthreads();
-        const int le = leaf_end[c0 + c] - (int)c0;
-        if (col > c && col < w && (!zero || col >= le)) {
-            const T u = -s_piv[col];
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const int64_t r = base + j;
-                if (r < rows && r > c) v[j] = __builtin_fma(s_l[grp * R + j], u, v[j]);
-            }
-        }
+        HS_PHASE(4);
     }
+    if (timed)
+        for (int q = 0; q < 8; ++q) atomicAdd((unsigned long long*)&prof[q], (unsigned long long)tp[q]);
+#undef HS_PHASE
 
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-        const int64_t r = base + j;
-        if (r < rows && col < w) a[(c0 + r) * lda + c0 + col] = v[j];
-    }
-    if (g == 0) {
-        __syncthreads();
-        // moves: [0] = count, then pairs (dst, src).
-        if (tid == 0) moves[0] = w + s_nout;
-        if (tid < w) { moves[1 + 2 * tid] = c0 + tid; moves[2 + 2 * tid] = s_blk[tid]; }
-        if (tid < s_nout) { moves[1 + 2 * (w + tid)] = s_opos[tid]; moves[2 + 2 * (w + tid)] = s_oval[tid]; }
+    if (has) {
+        T* dst = a + (c0 + pos) * lda + c0;
+        for (int j = 0; j < w; ++j) dst[j] = scratch[(int64_t)j * rows + phys];
+        if (pos != phys) {
+            const long long k = atomicAdd((unsigned long long*)moves, 1ull);
+            moves[1 + 2 * k] = c0 + pos;
+            moves[2 + 2 * k] = c0 + phys;
+        }
     }
 }
 
@@ -262,19 +251,24 @@ template <typename T>
 __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t lda, int64_t n, int64_t c0, int64_t c1,
                                                    const long long* __restrict__ moves) {
     __shared__ T tmp[2 * PW][64];
-    const int nm = (int)moves[0];
+    __shared__ long long mv[1 + 4 * PW];
+    for (int q = threadIdx.x; q < 1 + 4 * PW; q += blockDim.x) mv[q] = moves[q];
+    __syncthreads();
+    const int nm = (int)mv[0];
     const int tid = threadIdx.x, col_l = tid & 63, grp = tid >> 6;
     int64_t col = (int64_t)blockIdx.x * 64 + col_l;  // over the columns outside [c0, c1)
     if (col >= c0) col += c1 - c0;
     const bool live = col < n;
-    for (int k = grp; k < nm; k += 4) {
-        long long src = moves[2 + 2 * k];
-        if (live) tmp[k][col_l] = a[src * lda + col];
+#pragma unroll
+    for (int q = 0; q < 2 * PW / 4; ++q) {  // all sources in flight at once
+        const int k = grp + 4 * q;
+        if (k < nm && live) tmp[k][col_l] = a[mv[2 + 2 * k] * lda + col];
     }
     __syncthreads();
-    for (int k = grp; k < nm; k += 4) {
-        long long dst = moves[1 + 2 * k];
-        if (live) a[dst * lda + col] = tmp[k][col_l];
+#pragma unroll
+    for (int q = 0; q < 2 * PW / 4; ++q) {
+        const int k = grp + 4 * q;
+        if (k < nm && live) a[mv[1 + 2 * k] * lda + col] = tmp[k][col_l];
     }
 }
 
@@ -288,26 +282,29 @@ __global__ __launch_bounds__(TC) void trsm_block_kernel(T* __restrict__ a, int64
     __shared__ T X[PW][TC + 1];
     const int t = threadIdx.x;
     const int64_t col0 = c0 + w + (int64_t)blockIdx.x * TC;
-    for (int idx = t; idx < w * w; idx += TC) {
-        int r = idx / w, q = idx % w;
-        L[r][q] = a[(c0 + r) * lda + c0 + q];
+#pragma unroll
+    for (int q = 0; q < PW * PW / TC; ++q) {  // all loads in flight at once; zeros pad a narrow panel
+        const int idx = t + q * TC, r = idx / PW, c = idx % PW;
+        L[r][c] = (r < w && c < w) ? a[(c0 + r) * lda + c0 + c] : T(0);
     }
-    for (int r = 0; r < w; ++r) X[r][t] = col0 + t < n ? a[(c0 + r) * lda + col0 + t] : T(0);
+#pragma unroll
+    for (int r = 0; r < PW; ++r) X[r][t] = (r < w && col0 + t < n) ? a[(c0 + r) * lda + col0 + t] : T(0);
     __syncthreads();
     T x[PW];
 #pragma unroll
     for (int r = 0; r < PW; ++r) x[r] = X[r][t];
+    // Padding rows and columns are zero, so the full 64-row sweep leaves the real rows' bits unchanged.
 #pragma unroll
     for (int q = 0; q < PW; ++q) {
 #pragma unroll
-        for (int r = q + 1; r < PW; ++r)
-            if (r < w) x[r] = __builtin_fma(-L[r][q], x[q], x[r]);
-        __builtin_amdgcn_sched_barrier(0);  // keeps each step's LDS reads with its step
+        for (int r = q + 1; r < PW; ++r) x[r] = __builtin_fma(-L[r][q], x[q], x[r]);
     }
 #pragma unroll
     for (int r = 0; r < PW; ++r) X[r][t] = x[r];
     __syncthreads();
-    for (int r = 0; r < w; ++r) if (col0 + t < n) a[(c0 + r) * lda + col0 + t] = X[r][t];
+#pragma unroll
+    for (int r = 0; r < PW; ++r)
+        if (r < w && col0 + t < n) a[(c0 + r) * lda + col0 + t] = X[r][t];
 }
 
 template <typename T>
@@ -364,8 +361,9 @@ void leaves(int64_t c0, int64_t c1, std::vector<int64_t>& ends) {
 // Per-device state of the panel exchange: slots and flags, allocated once and never freed while the
 // library is loaded; the epoch makes each launch's flag targets larger than any value left behind.
 struct PanelState {
-    void* slots = nullptr;
-    unsigned long long* flags = nullptr;
+    char* state = nullptr;  // MAX_WG flags, then 2 x MAX_WG slots
+    long long* prof = nullptr;
+    bool prof_on = false;
     int* status = nullptr;
     long long* moves = nullptr;
     unsigned long long epoch = 0;
@@ -378,12 +376,14 @@ int panel_state(PanelState*& st) {
     HS_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_panel_mu);
     st = &g_panel[dev & 15];
-    if (!st->slots) {
-        HS_HIP(hipMalloc(&st->slots, 2 * MAX_WG * sizeof(Slot<double>)));
-        HS_HIP(hipMalloc(&st->flags, MAX_WG * sizeof(unsigned long long)));
+    if (!st->state) {
+        const size_t bytes = MAX_WG * sizeof(unsigned long long) + 2 * MAX_WG * sizeof(Slot<double>);
+        HS_HIP(hipMalloc(&st->state, bytes));
+        HS_HIP(hipMemset(st->state, 0, bytes));
         HS_HIP(hipMalloc(&st->status, sizeof(int)));
+        HS_HIP(hipMalloc(&st->prof, 8 * sizeof(long long)));
+        HS_HIP(hipMemset(st->prof, 0, 8 * sizeof(long long)));
         HS_HIP(hipMalloc(&st->moves, (1 + 4 * PW) * sizeof(long long)));
-        HS_HIP(hipMemset(st->flags, 0, MAX_WG * sizeof(unsigned long long)));
         HS_HIP(hipMemset(st->status, 0, sizeof(int)));
         HS_HIP(hipDeviceSynchronize());
     }
@@ -391,6 +391,18 @@ int panel_state(PanelState*& st) {
 }
 
 }  // namespace
+
+// The panel kernel's per-phase profile (ticks of 10 ns, summed over launches): on = 1 clears and enables,
+// on = 0 disables; out (8 values) receives the sums where not null.
+int panel_profile(int on, long long* out) {
+    PanelState* st;
+    int e = panel_state(st);
+    if (e) return e;
+    if (out) HS_HIP(hipMemcpy(out, st->prof, 8 * sizeof(long long), hipMemcpyDeviceToHost));
+    if (on) HS_HIP(hipMemset(st->prof, 0, 8 * sizeof(long long)));
+    st->prof_on = on != 0;
+    return OK;
+}
 
 template <typename T>
 int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
@@ -416,22 +428,24 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
         panels.push_back({start, cur});
     }
     int* d_leaf = (int*)pool_get(n * sizeof(int));
-    if (!d_leaf) return hip_status(hipErrorOutOfMemory, "pool_get");
+    T* scratch = (T*)pool_get(sizeof(T) * (size_t)n * PW);
+    if (!d_leaf || !scratch) { pool_put(d_leaf); pool_put(scratch); return hip_status(hipErrorOutOfMemory, "pool_get"); }
     HS_HIP(hipMemcpyAsync(d_leaf, leaf_end.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
 
     int rc = OK;
     for (auto [c0, c1] : panels) {
         const int w = (int)(c1 - c0);
         const int64_t rows = n - c0;
-        if (rows > (int64_t)MAX_WG * PANEL_ROWS) { set_error("n above 16384: the panel kernel holds at most 64 x 256 rows"); rc = -1; break; }
-        const int nwg = (int)((rows + PANEL_ROWS - 1) / PANEL_ROWS);
+        if (rows > (int64_t)MAX_WG * PT) { set_error("n above 16384: the panel kernel holds at most 32 x 512 rows"); rc = -1; break; }
+        const int nwg = (int)((rows + PT - 1) / PT);
         unsigned long long epoch;
         {
             std::lock_guard<std::mutex> lk(g_panel_mu);
             epoch = ++st->epoch;
         }
-        auto slots = reinterpret_cast<Slot<T>*>(st->slots);
-        panel_kernel<T, PANEL_ROWS / 4><<<nwg, PT, 0, s>>>(a, lda, c0, rows, w, pivots, st->moves, d_leaf, slots, st->flags, epoch, nwg, st->status);
+        const unsigned grid = nwg > 1 ? (unsigned)(8 * nwg) : 1u;
+        panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, st->moves, d_leaf, scratch, st->state, epoch, nwg,
+                                             st->status, st->prof_on ? st->prof : nullptr);
         HS_HIP(hipGetLastError());
         const int64_t outside = n - w;
         if (outside > 0) {
@@ -463,6 +477,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
         HS_HIP(hipStreamSynchronize(s));
     }
     pool_put(d_leaf);
+    pool_put(scratch);
     return rc;
 }
 

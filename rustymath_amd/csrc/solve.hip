@@ -23,9 +23,19 @@ constexpr int BR = 32;         // rows per block (NS of factor.rs:82; leaves hav
 constexpr int MAXK = 16;       // right-hand sides per launch
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
 
+
+enum Mode : int {
+    FWD_UNIT = 0,    // L (unit lower of F): increasing order, no division
+    BWD_SINGLE = 1,  // U, one column: decreasing order, then a quotient
+    BWD_LEAVES = 2,  // U, several columns: the halving order, then a reciprocal multiply
+    FWD_DIAG = 3,    // Uᵀ (lower, diagonal of U): increasing order, then a reciprocal multiply
+    BWD_UNIT = 4,    // Lᵀ (upper, unit): decreasing order, no division
+};
+
 template <typename T> struct Bits;
 template <> struct Bits<double> { typedef unsigned long long u; };
 template <> struct Bits<float> { typedef unsigned int u; };
+// Device-coherent (sc1) loads and stores of the rows that blocks publish to each other.
 template <typename T>
 __device__ __forceinline__ void put(T* p, T v) {
     typedef typename Bits<T>::u U;
@@ -38,34 +48,27 @@ __device__ __forceinline__ T get(const T* p) {
     return __builtin_bit_cast(T, u);
 }
 
-enum Mode : int {
-    FWD_UNIT = 0,    // L (unit lower of F): increasing order, no division
-    BWD_SINGLE = 1,  // U, one column: decreasing order, then a quotient
-    BWD_LEAVES = 2,  // U, several columns: the halving order, then a reciprocal multiply
-    FWD_DIAG = 3,    // Uᵀ (lower, diagonal of U): increasing order, then a reciprocal multiply
-    BWD_UNIT = 4,    // Lᵀ (upper, unit): decreasing order, no division
-};
-
-// entry(i, p) of the triangle the mode solves with, from column-major factors F (F(i, j) = f[j*ld + i]).
-template <typename T>
-__device__ __forceinline__ T entry(const T* f, int64_t ld, int mode, int64_t i, int64_t p) {
-    return (mode == FWD_DIAG || mode == BWD_UNIT) ? f[i * ld + p] : f[p * ld + i];
-}
-
 // Segments: seg[seg_off[b] .. seg_off[b+1]) are (first row, end row, direction) of the other blocks'
 // rows that block b's rows add, in order; direction 1 = increasing rows, 0 = decreasing.
+// The blocks are launch indices 0, 8, 16, ... (one XCD under round-robin dispatch; the others return at
+// once): a block publishes its solved rows to x and raises its flag with device-coherent (sc1) stores, and
+// the blocks after it read both with sc1 loads. Each producer block's rows of x and the 32x32 tile of
+// matrix items that the consumer needs come in through LDS with coalesced loads.
 template <typename T>
 __global__ __launch_bounds__(BR * MAXK) void trsv_blocks(int mode, int64_t n, int k, const T* __restrict__ f, int64_t ld,
                                                          T* x, int64_t ldx, const int* __restrict__ blk_r0,
                                                          const int* __restrict__ blk_r1, const int* __restrict__ blk_of_row,
                                                          const int* __restrict__ seg_off, const int* __restrict__ seg,
                                                          unsigned long long* flags, int* status) {
-    const int b = blockIdx.x;
+    if (blockIdx.x & 7) return;
+    const int b = blockIdx.x >> 3;
     const int r0 = blk_r0[b], r1 = blk_r1[b], rows = r1 - r0;
     const int tid = threadIdx.x, li = tid % BR, col = tid / BR;
     const bool live = li < rows && col < k;
     const int64_t i = r0 + li;
+    const bool tmode = mode == FWD_DIAG || mode == BWD_UNIT;  // entry(i, p) = f[i*ld + p]
     __shared__ T xs[BR][MAXK];
+    __shared__ T es[BR][BR + 1];  // es[row of this block][producer row]
     __shared__ int s_abort;
     if (tid == 0) s_abort = 0;
 
@@ -74,69 +77,80 @@ __global__ __launch_bounds__(BR * MAXK) void trsv_blocks(int mode, int64_t n, in
 
     for (int s = seg_off[b]; s < seg_off[b + 1]; ++s) {
         const int a0 = seg[3 * s], a1 = seg[3 * s + 1], inc = seg[3 * s + 2];
-        // Walk the producer blocks covering [a0, a1) in the segment's direction.
         int p = inc ? a0 : a1 - 1;
         while (inc ? p < a1 : p >= a0) {
             const int pb = blk_of_row[p];
             const int q0 = blk_r0[pb], q1 = blk_r1[pb];
-            const int lo = q0 > a0 ? q0 : a0, hi = q1 < a1 ? q1 : a1;
+            const int lo = q0 > a0 ? q0 : a0, hi = q1 < a1 ? q1 : a1, cnt = hi - lo;
             if (tid == 0) {
                 uint64_t spins = 0;
-                while (__hip_atomic_load(&flags[pb], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0ull) {
+                while (__hip_atomic_load(&flags[pb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull) {
                     if (++spins > SPIN_LIMIT) { s_abort = 1; atomicExch(status, ERR_LIMIT); break; }
-                    __builtin_amdgcn_s_sleep(1);
                 }
             }
             __syncthreads();
             if (s_abort) return;
-            for (int idx = tid; idx < (hi - lo) * k; idx += blockDim.x) {
-                int rr = idx / k, cc = idx % k;
+            // Stage the producer's solved rows and this block's matrix items against them.
+            for (int idx = tid; idx < cnt * k; idx += blockDim.x) {
+                const int rr = idx / k, cc = idx % k;
                 xs[rr][cc] = get(&x[(int64_t)(lo + rr) * ldx + cc]);
+            }
+            for (int idx = tid; idx < rows * cnt; idx += blockDim.x) {
+                int rr, qq;
+                if (tmode) { rr = idx / cnt; qq = idx % cnt; }   // row-contiguous items
+                else { qq = idx / rows; rr = idx % rows; }       // column-contiguous items
+                es[rr][qq] = tmode ? f[(int64_t)(r0 + rr) * ld + lo + qq] : f[(int64_t)(lo + qq) * ld + r0 + rr];
             }
             __syncthreads();
             if (live) {
                 if (inc)
-                    for (int q = lo; q < hi; ++q) acc = __builtin_fma(xs[q - lo][col], -entry(f, ld, mode, i, q), acc);
+                    for (int q = 0; q < cnt; ++q) acc = __builtin_fma(xs[q][col], -es[li][q], acc);
                 else
-                    for (int q = hi - 1; q >= lo; --q) acc = __builtin_fma(xs[q - lo][col], -entry(f, ld, mode, i, q), acc);
+                    for (int q = cnt - 1; q >= 0; --q) acc = __builtin_fma(xs[q][col], -es[li][q], acc);
             }
             __syncthreads();
             p = inc ? hi : lo - 1;
         }
     }
 
-    // Substitution within the block.
+    // Substitution within the block, with its own triangle staged in LDS.
+    for (int idx = tid; idx < rows * rows; idx += blockDim.x) {
+        int rr, qq;
+        if (tmode) { rr = idx / rows; qq = idx % rows; }
+        else { qq = idx / rows; rr = idx % rows; }
+        es[rr][qq] = tmode ? f[(int64_t)(r0 + rr) * ld + r0 + qq] : f[(int64_t)(r0 + qq) * ld + r0 + rr];
+    }
+    __syncthreads();
     const bool down = mode == FWD_UNIT || mode == FWD_DIAG;
+    const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
     if (mode == BWD_LEAVES) {
         // Row r1-1 first; each row adds the rows after it in increasing order, then divides.
         for (int t = rows - 1; t >= 0; --t) {
             if (live && li == t) {
-                for (int q = t + 1; q < rows; ++q) acc = __builtin_fma(xs[q][col], -entry(f, ld, mode, i, r0 + q), acc);
-                T d = f[i * ld + i];
-                acc = fabs(d) >= (sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f)) ? acc * (T(1) / d) : acc / d;
+                for (int q = t + 1; q < rows; ++q) acc = __builtin_fma(xs[q][col], -es[li][q], acc);
+                const T d = es[li][li];
+                acc = fabs(d) >= tiny ? acc * (T(1) / d) : acc / d;
                 xs[t][col] = acc;
             }
             __syncthreads();
         }
     } else {
-        for (int s = 0; s < rows; ++s) {
-            const int t = down ? s : rows - 1 - s;
+        for (int st = 0; st < rows; ++st) {
+            const int t = down ? st : rows - 1 - st;
             if (live && li == t) {
-                if (mode == BWD_SINGLE) acc = acc / f[i * ld + i];
-                else if (mode == FWD_DIAG) {
-                    T d = f[i * ld + i];
-                    acc = fabs(d) >= (sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f)) ? acc * (T(1) / d) : acc / d;
-                }
+                const T d = es[li][li];
+                if (mode == BWD_SINGLE) acc = acc / d;
+                else if (mode == FWD_DIAG) acc = fabs(d) >= tiny ? acc * (T(1) / d) : acc / d;
                 xs[t][col] = acc;
             }
             __syncthreads();
             const bool after = down ? li > t : li < t;
-            if (live && after) acc = __builtin_fma(xs[t][col], -entry(f, ld, mode, i, r0 + t), acc);
+            if (live && after) acc = __builtin_fma(xs[t][col], -es[li][t], acc);
         }
     }
     if (live) put(&x[i * ldx + col], acc);
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(&flags[b], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();  // every store of this block has completed (s_waitcnt before the barrier)
+    if (tid == 0) __hip_atomic_store(&flags[b], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
@@ -230,7 +244,7 @@ int trsv_pass(int mode, int64_t n, int k, const T* f, int64_t ld, T* x, int64_t 
     const int* d_of = d + 2 * nb;
     const int* d_soff = d_of + n;
     const int* d_seg = d_soff + nb + 1;
-    trsv_blocks<T><<<nb, BR * MAXK, 0, s>>>(mode, n, k, f, ld, x, ldx, d_r0, d_r1, d_of, d_soff, d_seg, flags, status);
+    trsv_blocks<T><<<8 * nb, BR * MAXK, 0, s>>>(mode, n, k, f, ld, x, ldx, d_r0, d_r1, d_of, d_soff, d_seg, flags, status);
     HS_HIP(hipGetLastError());
     int st = 0;
     HS_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));

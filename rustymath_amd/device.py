@@ -61,3 +61,19 @@ def getrs(factors, perm, b, trans=False):
     _lib.check(f(1 if trans else 0, n, k, factors.data_ptr(), factors.stride(0), perm.data_ptr(), b.data_ptr(),
                  b.stride(0), _stream()), 'getrs')
     return b
+
+
+def dsgesv(a, b, max_iter=30, tol=None):
+    """Solves A x = b for the f64 matrix `a` (n×n, compact) and vector `b` by an FP32 LU on MFMA and FP64 iterative
+    refinement on the GPU. Returns (x, corrections, backward error); `tol` defaults to 10·n·eps64."""
+    import ctypes
+    _mat(a, 'a')
+    n = a.shape[0]
+    if a.dtype != torch.float64 or b.dtype != torch.float64: raise TypeError('dsgesv takes float64')
+    b = b.contiguous().reshape(n)
+    x = torch.empty(n, dtype=torch.float64, device=a.device)
+    if tol is None: tol = 10 * n * torch.finfo(torch.float64).eps
+    it, be = ctypes.c_int(0), ctypes.c_double(0)
+    _lib.check(_lib.load().hs_dev_dsgesv(n, a.data_ptr(), a.stride(0), b.data_ptr(), x.data_ptr(), max_iter, tol,
+                                         ctypes.byref(it), ctypes.byref(be), _stream()), 'dsgesv')
+    return x, it.value, be.value

@@ -45,7 +45,7 @@ def _pair_frames(fa, fb):
 def _cells(x, frame, core):
     "x with its frame broadcast to `frame`, as a C-contiguous array of cells [count, *core]."
     x = np.broadcast_to(x, tuple(frame) + tuple(core))
-    return np.ascontiguousarray(x.reshape((-1,) + tuple(core)))
+    return np.ascontiguousarray(x.reshape((int(np.prod(frame, dtype=np.int64)),) + tuple(core)))
 
 
 def _sfx(dt): return 'd' if dt == np.float64 else 's'
@@ -113,7 +113,7 @@ def lu(a):
     if a.ndim < 2 or a.shape[-1] != a.shape[-2]: raise ValueError(f'lu needs square matrices, not {a.shape[-2:]}')
     n = a.shape[-1]
     frame = a.shape[:-2]
-    cells = np.ascontiguousarray(a.reshape((-1, n, n)).astype(compute, copy=False))
+    cells = np.ascontiguousarray(a.reshape((int(np.prod(frame, dtype=np.int64)), n, n)).astype(compute, copy=False))
     factors = np.empty_like(cells)
     perm = np.empty((cells.shape[0], n), np.int64)
     f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_factor')
@@ -217,7 +217,7 @@ def lu_det(factors, p):
     odd permutation (factor.rs:174). An O(n) fold over the factors the GPU computed."""
     factors = np.asarray(factors); n = factors.shape[-1]
     perm = _positions(p, n)
-    fc = factors.reshape(-1, n, n); pc = perm.reshape(-1, n)
+    fc = factors.reshape(int(np.prod(factors.shape[:-2], dtype=np.int64)), n, n); pc = perm.reshape(fc.shape[0], n)
     out = np.empty(fc.shape[0], factors.dtype)
     for i in range(fc.shape[0]):
         d = factors.dtype.type(1)
@@ -232,7 +232,7 @@ def lu_slogdet(factors, p):
     factors = np.asarray(factors); n = factors.shape[-1]
     t = factors.dtype.type
     perm = _positions(p, n)
-    fc = factors.reshape(-1, n, n); pc = perm.reshape(-1, n)
+    fc = factors.reshape(int(np.prod(factors.shape[:-2], dtype=np.int64)), n, n); pc = perm.reshape(fc.shape[0], n)
     signs = np.empty(fc.shape[0], factors.dtype); logs = np.empty(fc.shape[0], factors.dtype)
     for i in range(fc.shape[0]):
         s, l = (t(-1) if _odd(pc[i]) else t(1)), t(0)

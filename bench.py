@@ -33,6 +33,27 @@ def householder_matrix(n, seed, cond=1e6, reflectors=8):
     return a.contiguous(), b
 
 
+def batched_workload(count=10000, seed=3, nmin=8, nmax=512, kmax=8, dtype=None):
+    """The batched uneven-solve workload: `count` float64 systems, n uniform in [nmin, nmax], k uniform in [1, kmax],
+    A = G + sqrt(n) I with G iid N(0, 1), B iid N(0, 1). Orders and column counts come from numpy's generator (seed),
+    the entries from torch's CUDA generator (seed), straight into the flat device buffers of BatchedSolve."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, size=count).astype(np.int64)
+    ks = rng.integers(1, kmax + 1, size=count).astype(np.int64)
+    g = torch.Generator(device='cuda'); g.manual_seed(seed)
+    dtype = dtype or torch.float64
+    a = torch.randn(int((ns * ns).sum()), dtype=torch.float64, device='cuda', generator=g)
+    b = torch.randn(int((ns * ks).sum()), dtype=torch.float64, device='cuda', generator=g)
+    # The diagonals: offset of system i plus j (n_i + 1), for j < n_i.
+    offs = np.concatenate([[0], np.cumsum(ns * ns)[:-1]])
+    diag = np.concatenate([o + np.arange(n) * (n + 1) for o, n in zip(offs, ns)])
+    shift = np.repeat(np.sqrt(ns.astype(np.float64)), ns)
+    a[torch.from_numpy(diag).cuda()] += torch.from_numpy(shift).cuda()
+    return a.to(dtype), b.to(dtype), ns, ks
+
+
 def cpu_baseline(n=6144, runs=3, threads=16):
     """The oracle's LU solve (the C restatement of rustymath's blocked CPU kernels) on a bounded sample: an FP64 N(0, 1)
     system of order n on `threads` host threads, best of `runs`, in GFLOP/s (2/3 n³ + 2 n²)."""

@@ -90,6 +90,25 @@ int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t
 int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol,
                   int* iters, double* berr, void* stream);
 
+/* Batched uneven solves: the per-cell solves of a stack, which rustymath runs one `LuWork::solve` per Rayon
+ * worker (src/linalg/stacked.rs:88-101 over core/src/parallel.rs:118-192 and core/src/linalg/factor.rs:207-226),
+ * as one persistent launch over all cells. A plan covers `count` systems in device buffers: A_i (n[i]×n[i],
+ * row-major, n[i] <= 512) at a + a_off[i], B_i (n[i]×k[i], row-major, k[i] <= 8) at b + b_off[i], and perm_i
+ * (n[i] int64) at perm + p_off[i]; n, k and the offsets are host arrays. A run overwrites each A_i with its
+ * factors (row perm_i[j] holds position j, rows are not moved) and, where regular[i] = 1, B_i with X_i, bit for
+ * bit the reference's `solve` of that cell; regular[i] = 0 marks a cell the reference would solve by least
+ * squares, whose B_i is left as it was. panel_width (16 or 32) and wgs_per_cu (1 or 2) are tuning knobs that
+ * leave the results unchanged. A run with graph = 1 replays a hipGraph captured at the plan's first run.
+ * The plan keeps the pointers; runs that overlap on one plan race. */
+int hs_dev_dgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
+                              const int64_t* b_off, const int64_t* p_off, double* a, double* b, int64_t* perm,
+                              int* regular, int panel_width, int wgs_per_cu, int64_t* plan);
+int hs_dev_sgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
+                              const int64_t* b_off, const int64_t* p_off, float* a, float* b, int64_t* perm,
+                              int* regular, int panel_width, int wgs_per_cu, int64_t* plan);
+int hs_dev_gesv_batched_run(int64_t plan, int graph, void* stream);
+int hs_dev_gesv_batched_free(int64_t plan);
+
 /* Profiling (no counterpart in the reference): on = 1 clears and starts the panel kernel's per-phase timers
  * (100 MHz ticks summed over launches: candidate search, publish, exchange wait, pivot bookkeeping, update),
  * on = 0 stops them; `out` (8 values, may be NULL) receives the sums. */

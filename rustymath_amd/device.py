@@ -77,3 +77,105 @@ def dsgesv(a, b, max_iter=30, tol=None):
     _lib.check(_lib.load().hs_dev_dsgesv(n, a.data_ptr(), a.stride(0), b.data_ptr(), x.data_ptr(), max_iter, tol,
                                          ctypes.byref(it), ctypes.byref(be), _stream()), 'dsgesv')
     return x, it.value, be.value
+
+
+class BatchedSolve:
+    """Many independent systems A_i X_i = B_i (orders n_i <= 512, k_i <= 8 right-hand sides) solved by one persistent
+    launch, replayed from a hipGraph: the cells of a rustymath stack, each solved as `LuWork::solve` would
+    (core/src/linalg/factor.rs:207-226) and bit for bit its result.
+
+    `a` and `b` are flat device buffers holding the systems back to back: A_i row-major (n_i² items) and B_i row-major
+    (n_i k_i items), in the order of `ns`/`ks` (see `pack`). A run overwrites them: A_i with its factors, rows left in
+    place (row perm_i[j] holds position j), and B_i with X_i where `regular[i]`. `panel_width` (16 or 32) and
+    `wgs_per_cu` (1 or 2) change the speed, never the results; `tune` picks them."""
+
+    def __init__(self, a, b, ns, ks, panel_width=32, wgs_per_cu=1):
+        import ctypes
+        import numpy as np
+        if not (a.is_cuda and b.is_cuda): raise ValueError('batched buffers must be GPU tensors')
+        if a.dtype != b.dtype or a.dtype not in (torch.float64, torch.float32): raise TypeError('a and b: float64 or float32')
+        if not (a.is_contiguous() and b.is_contiguous()): raise ValueError('batched buffers must be contiguous')
+        ns = np.ascontiguousarray(ns, dtype=np.int64); ks = np.ascontiguousarray(ks, dtype=np.int64)
+        if ns.shape != ks.shape or ns.ndim != 1: raise ValueError('ns and ks: one order and one column count per system')
+        self.ns, self.ks = ns, ks
+        self.a_off = np.concatenate([[0], np.cumsum(ns * ns)[:-1]]).astype(np.int64) if len(ns) else ns.copy()
+        self.b_off = np.concatenate([[0], np.cumsum(ns * ks)[:-1]]).astype(np.int64) if len(ns) else ns.copy()
+        self.p_off = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64) if len(ns) else ns.copy()
+        if a.numel() < int((ns * ns).sum()) or b.numel() < int((ns * ks).sum()):
+            raise ValueError('the buffers are smaller than the systems they should hold')
+        self.a, self.b = a, b
+        self.perm = torch.empty(max(int(ns.sum()), 1), dtype=torch.int64, device=a.device)
+        self.regular = torch.zeros(max(len(ns), 1), dtype=torch.int32, device=a.device)
+        self.panel_width, self.wgs_per_cu = panel_width, wgs_per_cu
+        h = ctypes.c_int64(0)
+        p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        f = getattr(_lib.load(), f'hs_dev_{_sfx(a.dtype)}gesv_batched_plan')
+        _lib.check(f(len(ns), p(ns), p(ks), p(self.a_off), p(self.b_off), p(self.p_off), a.data_ptr(), b.data_ptr(),
+                     self.perm.data_ptr(), self.regular.data_ptr(), panel_width, wgs_per_cu, ctypes.byref(h)),
+                   'gesv_batched_plan')
+        self.handle = h.value
+
+    def run(self, graph=True):
+        "Solves every system on the current stream (asynchronous)."
+        if self.handle is None: raise ValueError('the plan is closed')
+        _lib.check(_lib.load().hs_dev_gesv_batched_run(self.handle, 1 if graph else 0, _stream()), 'gesv_batched_run')
+        return self
+
+    def close(self):
+        if self.handle is not None:
+            _lib.check(_lib.load().hs_dev_gesv_batched_free(self.handle), 'gesv_batched_free')
+            self.handle = None
+
+    def __del__(self):
+        try: self.close()
+        except Exception: pass
+
+    def x(self, i):
+        "X_i (n_i×k_i view of b)."
+        n, k, o = int(self.ns[i]), int(self.ks[i]), int(self.b_off[i])
+        return self.b[o:o + n * k].view(n, k)
+
+    def factors(self, i):
+        "(A_i's overwritten buffer n_i×n_i, perm_i): position j of the factors is row perm_i[j]."
+        n, o, po = int(self.ns[i]), int(self.a_off[i]), int(self.p_off[i])
+        return self.a[o:o + n * n].view(n, n), self.perm[po:po + n]
+
+
+def pack(mats, rhs, dtype=torch.float64, device='cuda'):
+    "Flat device buffers for BatchedSolve from lists of host (numpy) square matrices and right-hand sides (n×k)."
+    import numpy as np
+    a = np.concatenate([np.ascontiguousarray(m).reshape(-1) for m in mats]) if mats else np.zeros(0)
+    b = np.concatenate([np.ascontiguousarray(x).reshape(-1) for x in rhs]) if rhs else np.zeros(0)
+    ns = [m.shape[0] for m in mats]
+    ks = [x.shape[1] if x.ndim == 2 else 1 for x in rhs]
+    ta = torch.from_numpy(np.ascontiguousarray(a)).to(device=device, dtype=dtype)
+    tb = torch.from_numpy(np.ascontiguousarray(b)).to(device=device, dtype=dtype)
+    return ta, tb, ns, ks
+
+
+_TUNED = {}
+
+
+def tune(a, b, ns, ks, candidates=((32, 1), (16, 1), (32, 2), (16, 2)), reps=2):
+    """The batched kernel's autotuner: times each (panel width, workgroups per CU) candidate on copies of the systems
+    with HIP events on the current stream, best of `reps` graph replays, and returns the fastest, with the times (ms).
+    Results are remembered per dtype and workload shape (count, Σn³, Σn²k)."""
+    import numpy as np
+    ns64 = np.asarray(ns, dtype=np.int64); ks64 = np.asarray(ks, dtype=np.int64)
+    key = (a.dtype, len(ns64), int((ns64 ** 3).sum()), int((ns64 ** 2 * ks64).sum()))
+    if key in _TUNED: return _TUNED[key]
+    wa, wb = a.clone(), b.clone()
+    times = {}
+    for pw, wg in candidates:
+        plan = BatchedSolve(wa, wb, ns64, ks64, panel_width=pw, wgs_per_cu=wg)
+        best = float('inf')
+        for _ in range(reps + 1):          # the first run captures the graph and is not counted
+            wa.copy_(a); wb.copy_(b)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(); plan.run(); e1.record(); e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        times[(pw, wg)] = best
+        plan.close()
+    choice = min(times, key=times.get)
+    _TUNED[key] = (choice, times)
+    return choice, times

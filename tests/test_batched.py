@@ -1,0 +1,184 @@
+"""GPU parity of the batched uneven solves against the oracle, through the C ABI (hs_dev_*gesv_batched_*).
+
+Each cell of a batch must come out as the oracle's `solve` of that cell (rustymath's `LuWork::solve`,
+core/src/linalg/factor.rs:207-226), bit for bit: the `regular` decision, the pivot order, the factors and X. The cases
+cover every order where the reference's leaf and panel structure changes (16-column leaves, 32-row backward leaves),
+empty systems and systems without right-hand sides, zero, subnormal, NaN and tied pivots, both dtypes, every tuning
+choice, graph replays, and the full 10,000-system workload through size-independent checks (backward error of every
+system, a sample against the oracle).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from test_oracle import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+EDGE_NS = [0, 1, 2, 3, 7, 8, 9, 15, 16, 17, 23, 24, 31, 32, 33, 47, 48, 63, 64, 65, 96, 100, 127, 128, 129, 200, 255,
+           256, 257, 300, 383, 384, 400, 500, 511, 512]
+
+
+@pytest.fixture(scope='module')
+def dv():
+    import torch
+    torch.cuda.set_device(0)
+    from rustymath_amd import device
+    return device
+
+
+def solve_batch(dv, mats, rhs, dtype=np.float64, **knobs):
+    "Runs one batch through BatchedSolve; returns (factors, perms, regular flags, solutions) on the host."
+    import torch
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    a, b, ns, ks = dv.pack(mats, rhs, dtype=tdt)
+    plan = dv.BatchedSolve(a, b, ns, ks, **knobs)
+    plan.run()
+    torch.cuda.synchronize()
+    out = []
+    reg = plan.regular.cpu().numpy()
+    for i in range(len(ns)):
+        f, perm = plan.factors(i)
+        out.append((f.cpu().numpy(), perm.cpu().numpy(), bool(reg[i]), plan.x(i).cpu().numpy()))
+    plan.close()
+    return out
+
+
+def check_cell(a, b, got, what):
+    "One cell against the oracle: regular flag, pivots, factors (rows moved to their positions) and X, bit for bit."
+    f, perm, regular, x = got
+    n = a.shape[0]
+    want = oracle.solve(a, b)
+    assert regular == (want is not None), f'{what}: regular {regular}'
+    if n == 0: return
+    wf, wperm, _ = oracle.lu(a)
+    assert (perm == wperm).all(), f'{what}: pivots'
+    assert bits_equal(f[perm], wf.T), f'{what}: factors'
+    if regular: assert bits_equal(x, want), f'{what}: solution'
+    else: assert bits_equal(x, b), f'{what}: b must stay as it was'
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_every_order_matches_the_oracle(dv, dtype):
+    "Orders across the leaf (16) and backward-leaf (32) boundaries up to 512, 0 to 8 right-hand sides, N(0, 1) entries."
+    rng = np.random.default_rng(11)
+    mats = [rng.standard_normal((n, n)).astype(dtype) for n in EDGE_NS]
+    rhs = [rng.standard_normal((n, i % 9)).astype(dtype) for i, n in enumerate(EDGE_NS)]
+    got = solve_batch(dv, mats, rhs, dtype)
+    for i, (a, b) in enumerate(zip(mats, rhs)):
+        check_cell(a, b, got[i], f'n={a.shape[0]} k={b.shape[1]}')
+
+
+def special_cells(dtype):
+    "Matrices that exercise the reference's pivot rules: zero pivots, ties, subnormal pivots, NaN and infinity."
+    rng = np.random.default_rng(5)
+    cells = []
+    z = np.zeros((20, 20), dtype); cells.append(('zero matrix', z))
+    a = rng.standard_normal((40, 40)).astype(dtype); a[:, 5] = 0; cells.append(('zero column in a leaf', a))
+    a = rng.standard_normal((70, 70)).astype(dtype); a[:, 33] = 0; a[:, 34] = 0; cells.append(('zero columns at a panel edge', a))
+    a = np.sign(rng.standard_normal((64, 64))).astype(dtype); cells.append(('random signs (ties)', a))
+    a = (rng.integers(-2, 3, size=(90, 90))).astype(dtype); cells.append(('small integers (ties, zero pivots)', a))
+    tiny = np.ldexp(1.0, -1060 if dtype == np.float64 else -140)
+    a = (rng.standard_normal((48, 48)) * tiny).astype(dtype); cells.append(('subnormal pivots', a))
+    a = rng.standard_normal((36, 36)).astype(dtype); a[7, 3] = np.nan; cells.append(('a NaN', a))
+    a = rng.standard_normal((36, 36)).astype(dtype); a[:, 2] = np.nan; cells.append(('a NaN column', a))
+    a = rng.standard_normal((36, 36)).astype(dtype); a[10, 10] = np.inf; cells.append(('an infinity', a))
+    a = np.eye(33, dtype=dtype)[::-1].copy(); cells.append(('reversed identity', a))
+    a = np.triu(rng.standard_normal((50, 50))).astype(dtype); cells.append(('upper triangular', a))
+    a = (rng.standard_normal((60, 5)) @ rng.standard_normal((5, 60))).astype(dtype); cells.append(('rank 5', a))
+    return cells
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_special_matrices_match_the_oracle(dv, dtype):
+    rng = np.random.default_rng(6)
+    cells = special_cells(dtype)
+    mats = [a for _, a in cells]
+    rhs = []
+    for i, a in enumerate(mats):
+        b = rng.standard_normal((a.shape[0], 1 + i % 8))
+        if 'subnormal' in cells[i][0]: b = b * np.ldexp(1.0, -1060 if dtype == np.float64 else -140)
+        rhs.append(b.astype(dtype))
+    got = solve_batch(dv, mats, rhs, dtype)
+    for i, (name, a) in enumerate(cells):
+        check_cell(a, rhs[i], got[i], name)
+
+
+def test_tuning_choices_and_graph_replays_give_the_same_bits(dv):
+    """Panel width, workgroups per CU, a direct launch and repeated graph replays (with other GPU work between them)
+    change nothing in the results."""
+    import torch
+    rng = np.random.default_rng(9)
+    ns = rng.integers(1, 513, size=60)
+    mats = [rng.standard_normal((n, n)) for n in ns]
+    rhs = [rng.standard_normal((n, 1 + i % 8)) for i, n in enumerate(ns)]
+    a0, b0, ns_, ks_ = dv.pack(mats, rhs)
+    ref = None
+    for pw in (16, 32):
+        for wg in (1, 2):
+            a, b = a0.clone(), b0.clone()
+            plan = dv.BatchedSolve(a, b, ns_, ks_, panel_width=pw, wgs_per_cu=wg)
+            for rep, graph in enumerate((True, True, False, True)):
+                a.copy_(a0); b.copy_(b0)
+                plan.run(graph=graph)
+                noise = torch.randn(1000, 1000, device='cuda', dtype=torch.float64)
+                (noise @ noise).sum().item()      # unrelated eager work and a synchronization between replays
+                torch.cuda.synchronize()
+                got = (a.cpu().numpy(), b.cpu().numpy(), plan.perm.cpu().numpy(), plan.regular.cpu().numpy())
+                if ref is None:
+                    ref = got
+                    for i in range(0, len(ns), 7):
+                        check_cell(mats[i], rhs[i], (plan.factors(i)[0].cpu().numpy(), plan.factors(i)[1].cpu().numpy(),
+                                                      bool(got[3][i]), plan.x(i).cpu().numpy()), f'cell {i}')
+                else:
+                    assert all(bits_equal(x, y) for x, y in zip(got, ref)), (pw, wg, rep, graph)
+            plan.close()
+
+
+def test_full_workload_backward_errors_and_sample(dv):
+    """The 10,000-system workload (n uniform in [8, 512], k in [1, 8], A = G + sqrt(n) I, seed 3): every system is
+    regular and its normwise backward error ||B - A X|| / (||A|| ||X||) is at most 10 n eps64; a sample of cells,
+    smallest and largest orders included, equals the oracle bit for bit."""
+    import torch
+    from bench import batched_workload
+    a0, b0, ns, ks = batched_workload()
+    a, b = a0.clone(), b0.clone()
+    plan = dv.BatchedSolve(a, b, ns, ks)
+    plan.run()
+    torch.cuda.synchronize()
+    assert bool((plan.regular[:len(ns)] == 1).all())
+    eps = np.finfo(np.float64).eps
+    worst = 0.0
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        for k in np.unique(ks[idx]):
+            sel = idx[ks[idx] == k]
+            A = torch.stack([a0[plan.a_off[i]:plan.a_off[i] + n * n].view(n, n) for i in sel])
+            B = torch.stack([b0[plan.b_off[i]:plan.b_off[i] + n * k].view(n, k) for i in sel])
+            X = torch.stack([b[plan.b_off[i]:plan.b_off[i] + n * k].view(n, k) for i in sel])
+            r = (B - torch.bmm(A, X.contiguous())).flatten(1).norm(dim=1)
+            e = r / (A.flatten(1).norm(dim=1) * X.flatten(1).norm(dim=1))
+            worst = max(worst, float((e / (10 * n * eps)).max()))
+    assert worst <= 1.0, worst
+    order = np.argsort(ns)
+    sample = list(order[:3]) + list(order[-3:]) + list(np.random.default_rng(0).choice(len(ns), 14, replace=False))
+    for i in sample:
+        n, k = int(ns[i]), int(ks[i])
+        ai = a0[plan.a_off[i]:plan.a_off[i] + n * n].view(n, n).cpu().numpy()
+        bi = b0[plan.b_off[i]:plan.b_off[i] + n * k].view(n, k).cpu().numpy()
+        f, perm = plan.factors(i)
+        check_cell(ai, bi, (f.cpu().numpy(), perm.cpu().numpy(), True, plan.x(i).cpu().numpy()), f'system {i} n={n} k={k}')
+    plan.close()
+
+
+def test_autotuner_picks_a_candidate_and_remembers_it(dv):
+    import torch
+    rng = np.random.default_rng(2)
+    ns = rng.integers(8, 129, size=40)
+    mats = [rng.standard_normal((n, n)) + np.sqrt(n) * np.eye(n) for n in ns]
+    rhs = [rng.standard_normal((n, 2)) for n in ns]
+    a, b, ns_, ks_ = dv.pack(mats, rhs)
+    choice, times = dv.tune(a, b, ns_, ks_, reps=1)
+    assert choice in times and len(times) == 4 and all(t > 0 for t in times.values())
+    assert dv.tune(a, b, ns_, ks_)[0] == choice
+    torch.cuda.synchronize()

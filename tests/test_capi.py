@@ -41,3 +41,25 @@ def test_linalg_validates_shapes_on_the_host():
     with pytest.raises(ValueError): la.lu(np.ones((2, 3)))
     with pytest.raises(IndexError): la.lu_solve(np.eye(3), np.array([-1, 0, 1]), np.ones(3))
     with pytest.raises(TypeError): la.matmul(np.ones((2, 2), complex), np.ones((2, 2)))
+
+
+def test_batched_plan_rejects_bad_systems_on_the_host():
+    "The batched plan checks every system's order, column count and offsets, and its tuning knobs, before any HIP call."
+    import numpy as np
+    from rustymath_amd import _lib
+    lib = _lib.load()
+    p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    buf = np.zeros(64)
+    ptr, reg = p(buf), p(np.zeros(4, np.int32))
+    plan = ctypes.c_int64(0)
+    def call(n, k, off=0, pw=32, wg=1, count=1):
+        n = np.array([n], np.int64); k = np.array([k], np.int64); o = np.array([off], np.int64)
+        return lib.hs_dev_dgesv_batched_plan(count, p(n), p(k), p(o), p(o), p(o), ptr, ptr, ptr, reg, pw, wg,
+                                             ctypes.byref(plan))
+    assert call(4, 1, count=-1) == -1
+    assert call(513, 1) == -2
+    assert call(4, 9) == -3
+    assert call(4, 1, off=-1) == -4
+    assert call(4, 1, pw=24) == -11
+    assert call(4, 1, wg=3) == -12
+    assert lib.hs_dev_gesv_batched_free(12345) == -1

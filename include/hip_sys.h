@@ -94,18 +94,22 @@ int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, doub
  * worker (src/linalg/stacked.rs:88-101 over core/src/parallel.rs:118-192 and core/src/linalg/factor.rs:207-226),
  * as one persistent launch over all cells. A plan covers `count` systems in device buffers: A_i (n[i]×n[i],
  * row-major, n[i] <= 512) at a + a_off[i], B_i (n[i]×k[i], row-major, k[i] <= 8) at b + b_off[i], and perm_i
- * (n[i] int64) at perm + p_off[i]; n, k and the offsets are host arrays. A run overwrites each A_i with its
- * factors (row perm_i[j] holds position j, rows are not moved) and, where regular[i] = 1, B_i with X_i, bit for
- * bit the reference's `solve` of that cell; regular[i] = 0 marks a cell the reference would solve by least
- * squares, whose B_i is left as it was. panel_width (16 or 32) and wgs_per_cu (1 or 2) are tuning knobs that
- * leave the results unchanged. A run with graph = 1 replays a hipGraph captured at the plan's first run.
- * The plan keeps the pointers; runs that overlap on one plan race. */
+ * (n[i] int64) at perm + p_off[i]; n, k and the offsets are host arrays.
+ * mode 0: a run overwrites each A_i with its factors (row perm_i[j] holds position j, rows are not moved) and,
+ *   where regular[i] = 1, B_i with X_i, bit for bit the reference's `solve` of that cell; regular[i] = 0 marks a
+ *   cell the reference would solve by least squares, whose B_i is left as it was.
+ * mode 1 / 2: solves A_i X = B_i (the reference's `lu_solve`, factor.rs:633-641) or A_iᵀ X = B_i (Uᵀ, Lᵀ, then
+ *   the inverse permutation; no counterpart in the reference, for adjoints) in place on B_i with the factors
+ *   and perm that a mode-0 run left; `regular` may be NULL.
+ * panel_width (16 or 32) and wgs_per_cu (1 or 2) are tuning knobs that leave the results unchanged. A run with
+ * graph = 1 replays a hipGraph captured at the plan's first run. The plan keeps the pointers; runs that overlap
+ * on one plan race. */
 int hs_dev_dgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
                               const int64_t* b_off, const int64_t* p_off, double* a, double* b, int64_t* perm,
-                              int* regular, int panel_width, int wgs_per_cu, int64_t* plan);
+                              int* regular, int mode, int panel_width, int wgs_per_cu, int64_t* plan);
 int hs_dev_sgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
                               const int64_t* b_off, const int64_t* p_off, float* a, float* b, int64_t* perm,
-                              int* regular, int panel_width, int wgs_per_cu, int64_t* plan);
+                              int* regular, int mode, int panel_width, int wgs_per_cu, int64_t* plan);
 int hs_dev_gesv_batched_run(int64_t plan, int graph, void* stream);
 int hs_dev_gesv_batched_free(int64_t plan);
 
@@ -116,6 +120,10 @@ int hs_profile_panel(int on, int64_t* out);
 /* Profiling of the MFMA product: on = 1 clears and starts bracketing every product launch with HIP events on its
  * stream, on = 0 stops; gives the launches' summed time (ms), their count and their algorithmic flops (2 m n k). */
 int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops);
+/* Profiling of the batched solves: on = 1 clears and starts per-phase timers in workgroup 0 (100 MHz ticks: setup,
+ * panel columns, U rows and trailing update, regularity check and L, U), so that runs launch directly rather than
+ * replay their graph; on = 0 stops. `out` (8 values, may be NULL) gets the sums, out[7] the systems timed. */
+int hs_profile_batched(int on, int64_t* out);
 
 #ifdef __cplusplus
 }

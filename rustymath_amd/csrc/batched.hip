@@ -6,19 +6,24 @@
 // sorted by decreasing cost (longest processing time first) and runs the same arithmetic, item for item:
 //
 //   - the factorization is right-looking over panels made of whole reference leaves (lu_panel halves down
-//     to 16 columns), at most 32 columns each. A panel's rows sit one per thread in registers, shifted so
-//     v[0] is the current column; rows never move: a swap relabels positions (`perm[position] = row`), as in
-//     the large-matrix panel kernel. Every item takes its updates in pivot-column order, one fused
-//     multiply-add each; multipliers are a · (1/pivot), a quotient for a subnormal pivot; a zero pivot
-//     skips the rest of its own leaf;
-//   - right of a panel, each thread owns a column: it solves the panel's unit-lower triangle for that
-//     column's U rows in registers, then updates the trailing rows with the panel's multipliers, which the
-//     workgroup stages in LDS a chunk of rows at a time;
-//   - the solve substitutes in the reference's three orders (csrc/solve.hip has the same three).
+//     to 16 columns), at most BW columns each. A panel's rows sit one per thread in registers, shifted so
+//     v[0] is the current column; rows never move: a swap relabels positions (`perm[position] = row`). Each
+//     column costs one barrier: every wave publishes its best candidate row, and all threads pick the
+//     winner from LDS. Every item takes its updates in pivot-column order, one fused multiply-add each;
+//     multipliers are a · (1/pivot), a quotient for a subnormal pivot; a zero pivot skips the rest of its
+//     own leaf;
+//   - right of a panel, each thread solves one column's U rows with the panel's unit-lower triangle, in
+//     order, and the trailing rows take the panel's product on MFMA: each wave owns 32×32 blocks and adds
+//     the panel's terms four at a time. gfx950's FP64 and FP32 16x16x4 MFMA rounds every step of its
+//     four-term sum as a fused multiply-add in order of k (measured), so the product is the in-order fold;
+//     padding terms are (+0)·(−0) = −0, which leaves every sum unchanged;
+//   - the solve substitutes in blocks of 32 rows: one wave per right-hand side solves a diagonal block
+//     with register broadcasts, then every row outside it adds the block's terms, in the reference's three
+//     orders (csrc/solve.hip has the same three) — and the transposed solve for adjoints.
 //
 // A is overwritten by its factors without moving rows: row perm[i] of the matrix holds position i of the
 // factors, where perm is rustymath's `perm` (factor.rs:132-134). B is overwritten by X where the system is
-// regular. The result of a system does not depend on which workgroup ran it or when.
+// regular. The result of a system does not depend on which workgroup ran it, when, or the tuning choices.
 #include "hs_kernels.h"
 #include <vector>
 #include <algorithm>
@@ -30,15 +35,22 @@ namespace hs {
 
 namespace {
 
-constexpr int BT = 512;    // threads of a workgroup = the largest order
-constexpr int LR = 64;     // trailing rows whose multipliers are staged in LDS at once
-constexpr int MAXRHS = 8;  // right-hand sides per system
-constexpr int NS = 32;     // `backward`'s leaf height (factor.rs:693-707)
+constexpr int BT = 512;             // threads of a workgroup = the largest order
+constexpr int WAVES = BT / 64;
+constexpr int MAXRHS = 8;           // right-hand sides per system (one wave each in the substitutions)
+constexpr int NS = 32;              // `backward`'s leaf height (factor.rs:693-707)
+constexpr int SB = 32;              // substitution block height
+
+enum Mode { FACTOR_SOLVE = 0, SOLVE = 1, SOLVE_T = 2 };
 
 struct Desc {
     long long n, k, a_off, b_off, p_off;
 };
 
+template <typename T>
+struct Bits { typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type U; };
+
+// The first greatest (value, key): larger value, then smaller key.
 template <typename T>
 __device__ __forceinline__ void wave_best(T& v, int& r) {
 #pragma unroll
@@ -49,30 +61,110 @@ __device__ __forceinline__ void wave_best(T& v, int& r) {
     }
 }
 
+// Lane l's value of v (l uniform).
 template <typename T>
-struct Bits { typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type U; };
+__device__ __forceinline__ T readlane(T v, int l) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    }
+}
 
-// BW: the widest panel (16 or 32 columns, a tuning choice; the results do not depend on it).
+// The trailing update of one panel on MFMA: A22 (rows and columns [c1, n)) += (−L21) U12, with L21 the panel's
+// multipliers (columns [c0, c0 + w)) and U12 its rows of U. Each wave owns 32×32 blocks (2×2 MFMA tiles).
+template <typename T, int BW>
+__device__ __forceinline__ void trailing_update(T* __restrict__ a, int n, int c0, int w, int c1, const short* perm,
+                                                int wave, int lane) {
+    typedef typename Mfma<T>::acc V;
+    const int m = n - c1, nb = (m + 31) / 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int blk = wave; blk < nb * nb; blk += WAVES) {
+        const int rb = c1 + 32 * (blk / nb), cb = c1 + 32 * (blk % nb);
+        V acc[2][2];
+        int crow[2][4];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = rb + 16 * ti + Mfma<T>::row(lane, e);
+                crow[ti][e] = r < n ? perm[r] * n : -1;
+            }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                const int col = cb + 16 * tj + lr;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[ti][tj][e] = (crow[ti][e] >= 0 && col < n) ? a[crow[ti][e] + col] : T(0);
+            }
+        int arow[2];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) { const int r = rb + 16 * ti + lr; arow[ti] = r < n ? perm[r] * n + c0 : -1; }
+#pragma unroll
+        for (int s = 0; s < BW / 4; ++s) {
+            if (4 * s < w) {
+                const int kk = 4 * s + lk;
+                const int brow = kk < w ? perm[c0 + kk] * n : -1;
+                T av[2], bv[2];
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti) av[ti] = (kk < w && arow[ti] >= 0) ? -a[arow[ti] + kk] : T(0);
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj) {
+                    const int col = cb + 16 * tj + lr;
+                    bv[tj] = (brow >= 0 && col < n) ? a[brow + col] : T(-0.0);
+                }
+#pragma unroll
+                for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                    for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = Mfma<T>::step(av[ti], bv[tj], acc[ti][tj]);
+            }
+        }
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) {
+                const int col = cb + 16 * tj + lr;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (crow[ti][e] >= 0 && col < n) a[crow[ti][e] + col] = acc[ti][tj][e];
+            }
+    }
+}
+
+// BW: the widest panel (16 or 32 columns, a tuning choice; the results do not depend on it). `mode`: factor and
+// solve (perm and regular written), or solve A X = B / Aᵀ X = B with the factors and perm of an earlier run.
 template <typename T, int BW>
 __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc* __restrict__ desc, const int* __restrict__ order,
                                                           T* __restrict__ a_all, T* __restrict__ b_all,
                                                           long long* __restrict__ perm_all, int* __restrict__ regular_all,
-                                                          int* next) {
+                                                          int* next, int mode, long long* prof) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int WAVES = BT / 64;
+    // Where `prof` is not null, thread 0 of workgroup 0 adds each phase's 100 MHz ticks into prof[0..4] and counts
+    // its systems in prof[7]: setup, panel columns, U rows and trailing update, regularity + forward, backward.
+    const bool timed = prof != nullptr && blockIdx.x == 0 && tid == 0;
+    long long tl = timed ? (long long)__builtin_amdgcn_s_memrealtime() : 0;
+#define HS_BPHASE(q) do { if (timed) { long long t_ = __builtin_amdgcn_s_memrealtime(); prof[q] += t_ - tl; tl = t_; } } while (0)
     __shared__ int s_sys;
     __shared__ unsigned long long s_max;
     __shared__ short perm[BT];       // position -> row
     __shared__ short leaf_end[BT];   // end of the reference leaf of each column
     __shared__ short panel_end[BT];  // for a panel's first column, the panel's end
-    __shared__ T s_piv[BW];
-    __shared__ T s_val[WAVES];
-    __shared__ int s_key[WAVES];
-    __shared__ T s_L[BW][BW + 1];    // the panel's unit-lower triangle
-    __shared__ T s_buf[BT * MAXRHS]; // trailing multipliers [LR][BW], or the solve's x [BT][MAXRHS]
-    T (*s_l2)[BW] = reinterpret_cast<T (*)[BW]>(s_buf);
-    T (*s_x)[MAXRHS] = reinterpret_cast<T (*)[MAXRHS]>(s_buf);
-    static_assert(LR * BW <= BT * MAXRHS && BW >= 16, "LDS carve-up; panels hold whole 16-column leaves");
+    __shared__ T s_wval[2][WAVES];   // each wave's best candidate, double-buffered by column
+    __shared__ int s_wkey[2][WAVES];
+    // Work area: the panel's candidate rows [2][WAVES][BW], its rows of U [BW][BW] and unit-lower triangle
+    // [BW][BW + 1]; or the substitutions' x [BT][MAXRHS].
+    constexpr int PANEL_ITEMS = 2 * WAVES * BW + BW * BW + BW * (BW + 1);
+    constexpr int WORK = PANEL_ITEMS > BT * MAXRHS ? PANEL_ITEMS : BT * MAXRHS;
+    __shared__ T s_work[WORK];
+    T (*s_cand)[WAVES][BW] = reinterpret_cast<T (*)[WAVES][BW]>(s_work);
+    T (*s_u11)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * BW);
+    T (*s_l11)[BW + 1] = reinterpret_cast<T (*)[BW + 1]>(s_work + 2 * WAVES * BW + BW * BW);
+    T (*s_x)[MAXRHS] = reinterpret_cast<T (*)[MAXRHS]>(s_work);
+    static_assert(BW == 16 || BW == 32, "panels hold whole 16-column leaves");
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
     const T eps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07f);
 
@@ -81,224 +173,321 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
         __syncthreads();
         const int q = s_sys;
         if (q >= count) break;
+        if (timed) prof[7] += 1;
         const int sys = order[q];
         const Desc d = desc[sys];
         const int n = (int)d.n, k = (int)d.k;
         T* a = a_all + d.a_off;
         T* b = b_all + d.b_off;
+        long long* gperm = perm_all + d.p_off;
+        bool solve;
 
-        if (tid < n) {
-            perm[tid] = (short)tid;
-            // The reference leaf holding column tid: lu_panel's halving while wider than 16 (factor.rs:592-616).
-            int l0 = 0, l1 = n;
-            while (l1 - l0 > 16) { const int mid = (l0 + l1) / 2; if (tid < mid) l1 = mid; else l0 = mid; }
-            leaf_end[tid] = (short)l1;
-        }
-        {
-            // `greatest` magnitude of the matrix, NaN ignored (factor.rs:488), for `regular`.
-            T m = T(0);
-            for (int i = tid; i < n * n; i += BT) { const T v = fabs(a[i]); m = v > m ? v : m; }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) { const T o = __shfl_xor(m, off); m = o > m ? o : m; }
-            if (lane == 0) atomicMax(&s_max, (unsigned long long)__builtin_bit_cast(typename Bits<T>::U, m));
-        }
-        __syncthreads();
-        if (tid == 0) {
-            // Panels: greedy unions of whole leaves, at most BW columns.
-            int pstart = 0;
-            for (int c = 0; c < n; c = leaf_end[c]) {
-                if (leaf_end[c] - pstart > BW) { panel_end[pstart] = (short)c; pstart = c; }
+        if (mode == FACTOR_SOLVE) {
+            if (tid < n) {
+                perm[tid] = (short)tid;
+                // The reference leaf holding column tid: lu_panel's halving while wider than 16 (factor.rs:592-616).
+                int l0 = 0, l1 = n;
+                while (l1 - l0 > 16) { const int mid = (l0 + l1) / 2; if (tid < mid) l1 = mid; else l0 = mid; }
+                leaf_end[tid] = (short)l1;
             }
-            if (n > 0) panel_end[pstart] = (short)n;
-        }
-        __syncthreads();
-
-        // ---- lu_factor ----
-        for (int c0 = 0; c0 < n;) {
-            const int c1 = panel_end[c0], w = c1 - c0, rows = n - c0;
-            const bool has = tid < rows;
-            const int row = has ? perm[c0 + tid] : 0;
-            int pos = tid;
-            T v[BW];
+            {
+                // `greatest` magnitude of the matrix, NaN ignored (factor.rs:488), for `regular`.
+                T mq[8];
 #pragma unroll
-            for (int j = 0; j < BW; ++j) v[j] = (has && j < w) ? a[row * n + c0 + j] : T(0);
+                for (int t = 0; t < 8; ++t) mq[t] = T(0);
+                const int nn = n * n;
+                for (int i = tid; i < nn; i += 8 * BT) {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const int j = i + t * BT;
+                        const T v = j < nn ? fabs(a[j]) : T(0);
+                        mq[t] = v > mq[t] ? v : mq[t];
+                    }
+                }
+                T m = mq[0];
+#pragma unroll
+                for (int t = 1; t < 8; ++t) m = mq[t] > m ? mq[t] : m;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) { const T o = __shfl_xor(m, off); m = o > m ? o : m; }
+                if (lane == 0) atomicMax(&s_max, (unsigned long long)__builtin_bit_cast(typename Bits<T>::U, m));
+            }
+            __syncthreads();
+            if (tid == 0) {
+                // Panels: greedy unions of whole leaves, at most BW columns.
+                int pstart = 0;
+                for (int c = 0; c < n; c = leaf_end[c]) {
+                    if (leaf_end[c] - pstart > BW) { panel_end[pstart] = (short)c; pstart = c; }
+                }
+                if (n > 0) panel_end[pstart] = (short)n;
+            }
+            __syncthreads();
+            HS_BPHASE(0);
+
+            // ---- lu_factor ----
+            for (int c0 = 0; c0 < n;) {
+                const int c1 = panel_end[c0], w = c1 - c0, rows = n - c0;
+                const bool has = tid < rows;
+                const int row = has ? perm[c0 + tid] : 0;
+                int pos = tid;
+                T v[BW];
+#pragma unroll
+                for (int j = 0; j < BW; ++j) v[j] = (has && j < w) ? a[row * n + c0 + j] : T(0);
 #pragma clang loop unroll(disable)
-            for (int c = 0; c < w; ++c) {
-                // First greatest magnitude at positions >= c, NaN ignored; all NaN -> position c (factor.rs:476-490).
-                T val = T(-1);
-                int key = 0x7fffffff;
-                if (has && pos >= c) { const T x = fabs(v[0]); val = x != x ? T(-1) : x; key = pos; }
-                wave_best(val, key);
-                if (lane == 0) { s_val[wave] = val; s_key[wave] = key; }
-                __syncthreads();
-                T wb = s_val[0];
-                int p = s_key[0];
+                for (int c = 0; c < w; ++c) {
+                    const int buf = c & 1;
+                    // First greatest magnitude at positions >= c, NaN ignored; all NaN -> position c (factor.rs:476-490).
+                    T val = T(-1);
+                    int key = 0x7fffffff;
+                    if (has && pos >= c) { const T x = fabs(v[0]); val = x != x ? T(-1) : x; key = pos; }
+                    T bv = val;
+                    int bk = key;
+                    wave_best(bv, bk);
+                    if (bk != 0x7fffffff && key == bk) {
 #pragma unroll
-                for (int u = 1; u < WAVES; ++u)
-                    if (s_val[u] > wb || (s_val[u] == wb && s_key[u] < p)) { wb = s_val[u]; p = s_key[u]; }
-                if (has && pos == p) {
-#pragma unroll
-                    for (int j = 0; j < BW; ++j) s_piv[j] = v[j];
-                }
-                __syncthreads();
-                const T pivot = s_piv[0];
-                if (has && pos == p) {
-#pragma unroll
-                    for (int j = 0; j < BW; ++j) if (j < w - c) a[row * n + c0 + c + j] = v[j];
-                    pos = c;
-                } else if (has && pos == c) {
-                    pos = p;
-                }
-                if (has && pos > c) {
-                    // divide (factor.rs:506-508), nothing for a zero pivot, whose leaf then skips its update (:606).
-                    const bool zero = pivot == T(0);
-                    T l = v[0];
-                    if (!zero) l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
-                    a[row * n + c0 + c] = l;
-                    const int skip = zero ? leaf_end[c0 + c] - c0 - c : 1;
-#pragma unroll
-                    for (int j = 1; j < BW; ++j) {
-                        const T nv = __builtin_fma(l, -s_piv[j], v[j]);
-                        v[j - 1] = j >= skip ? nv : v[j];
+                        for (int j = 0; j < BW; ++j) s_cand[buf][wave][j] = v[j];
                     }
-                    v[BW - 1] = T(0);
+                    if (lane == 0) { s_wval[buf][wave] = bv; s_wkey[buf][wave] = bk; }
+                    __syncthreads();
+                    T wb = s_wval[buf][0];
+                    int p = s_wkey[buf][0], win = 0;
+#pragma unroll
+                    for (int u = 1; u < WAVES; ++u) {
+                        const T vu = s_wval[buf][u];
+                        const int ku = s_wkey[buf][u];
+                        if (vu > wb || (vu == wb && ku < p)) { wb = vu; p = ku; win = u; }
+                    }
+                    const T* piv = s_cand[buf][win];
+                    if (tid < BW) s_u11[c][tid] = piv[tid];
+                    const T pivot = piv[0];
+                    if (has && pos == p) pos = c;
+                    else if (has && pos == c) pos = p;
+                    if (has && pos > c) {
+                        // divide (factor.rs:506-508), nothing for a zero pivot, whose leaf then skips its update (:606).
+                        T l = v[0];
+                        if (pivot != T(0)) {
+                            l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
+#pragma unroll
+                            for (int j = 1; j < BW; ++j) v[j - 1] = __builtin_fma(l, -piv[j], v[j]);
+                        } else {
+                            const int skip = leaf_end[c0 + c] - c0 - c;
+#pragma unroll
+                            for (int j = 1; j < BW; ++j) v[j - 1] = j >= skip ? __builtin_fma(l, -piv[j], v[j]) : v[j];
+                        }
+                        v[BW - 1] = T(0);
+                        a[row * n + c0 + c] = l;
+                    }
                 }
-            }
-            if (has) perm[c0 + pos] = (short)row;
-            __syncthreads();
-            if (c1 < n) {
+                if (has) perm[c0 + pos] = (short)row;
+                __syncthreads();
+                HS_BPHASE(1);
+                // The panel's rows of U, from LDS to their rows.
                 for (int i = tid; i < BW * BW; i += BT) {
-                    const int r = i / BW, cq = i % BW;
-                    s_L[r][cq] = (r < w && cq < w) ? a[perm[c0 + r] * n + c0 + cq] : T(0);
+                    const int cc = i / BW, j = i % BW;
+                    if (cc < w && j < w - cc) a[perm[c0 + cc] * n + c0 + cc + j] = s_u11[cc][j];
                 }
-                // Threads split into CG column groups of 64 and G row groups.
-                const int cols = n - c1, CG = (cols + 63) / 64, G = WAVES / CG;
-                const int grp = wave / CG, col = c1 + (wave % CG) * 64 + lane;
-                const bool live = grp < G && col < n;
-                // U12 for this column: the panel's unit-lower solve, in order (every group of the column alike,
-                // all reading before the barrier, group 0 writing after it).
-                T u[BW];
+                if (c1 < n) {
+                    for (int i = tid; i < BW * BW; i += BT) {
+                        const int r = i / BW, cq = i % BW;
+                        s_l11[r][cq] = (r < w && cq < r) ? a[perm[c0 + r] * n + c0 + cq] : T(0);
+                    }
+                    // U12: the panel's unit-lower solve for each column right of it, in order, one column per thread.
+                    // u[j] holds row qq + j of the column: step qq finishes row qq and shifts the rest down.
+                    const int col = c1 + tid;
+                    const bool live = col < n;
+                    T u[BW];
 #pragma unroll
-                for (int r = 0; r < BW; ++r) u[r] = (live && r < w) ? a[perm[c0 + r] * n + col] : T(0);
-                __syncthreads();
+                    for (int r = 0; r < BW; ++r) u[r] = (live && r < w) ? a[perm[c0 + r] * n + col] : T(0);
+                    __syncthreads();
+#pragma clang loop unroll(disable)
+                    for (int qq = 0; qq < w; ++qq) {
+                        const T x = u[0];
+                        if (live) a[perm[c0 + qq] * n + col] = x;
 #pragma unroll
-                for (int qq = 0; qq < BW; ++qq)
-#pragma unroll
-                    for (int r = qq + 1; r < BW; ++r) u[r] = __builtin_fma(-s_L[r][qq], u[qq], u[r]);
-                if (live && grp == 0) {
-#pragma unroll
-                    for (int r = 0; r < BW; ++r) if (r < w) a[perm[c0 + r] * n + col] = u[r];
-                }
-                // Trailing rows [c1, n): each item takes the panel's w terms in order, one fused multiply-add each.
-                for (int r0 = c1; r0 < n; r0 += LR) {
-                    const int nr = n - r0 < LR ? n - r0 : LR;
-                    __syncthreads();  // the previous chunk's multipliers are no longer read
-                    for (int i = tid; i < LR * BW; i += BT) {
-                        const int rr = i / BW, p = i % BW;
-                        s_l2[rr][p] = (rr < nr && p < w) ? -a[perm[r0 + rr] * n + c0 + p] : T(0);
+                        for (int j = 1; j < BW; ++j) u[j - 1] = __builtin_fma(-s_l11[qq + j < BW ? qq + j : BW - 1][qq], x, u[j]);
+                        u[BW - 1] = T(0);
                     }
                     __syncthreads();
-                    if (live) {
-                        for (int rr = grp; rr < nr; rr += G) {
-                            T* item = a + perm[r0 + rr] * n + col;
-                            T acc = *item;
-#pragma unroll
-                            for (int p = 0; p < BW; ++p)
-                                if (p < w) acc = __builtin_fma(s_l2[rr][p], u[p], acc);
-                            *item = acc;
-                        }
-                    }
+                    trailing_update<T, BW>(a, n, c0, w, c1, perm, wave, lane);
                 }
                 __syncthreads();
+                HS_BPHASE(2);
+                c0 = c1;
             }
-            c0 = c1;
+
+            // ---- regular (factor.rs:221-226): no |u_ii| at or below eps · n · greatest ----
+            const T largest = __builtin_bit_cast(T, (typename Bits<T>::U)s_max);
+            const T negligible = eps * T(n) * largest;
+            const int ok = __syncthreads_and(!(tid < n) || !(fabs(a[perm[tid] * n + tid]) <= negligible));
+            if (tid < n) gperm[tid] = perm[tid];
+            if (tid == 0) regular_all[sys] = ok && n > 0;
+            solve = ok && n > 0 && k > 0;
+        } else {
+            if (tid < n) perm[tid] = (short)gperm[tid];
+            __syncthreads();
+            solve = n > 0 && k > 0;
         }
 
-        // ---- regular (factor.rs:221-226): no |u_ii| at or below eps · n · greatest ----
-        const T largest = __builtin_bit_cast(T, (typename Bits<T>::U)s_max);
-        const T negligible = eps * T(n) * largest;
-        const int ok = __syncthreads_and(!(tid < n) || !(fabs(a[perm[tid] * n + tid]) <= negligible));
-        if (tid < n) perm_all[d.p_off + tid] = perm[tid];
-        if (tid == 0) regular_all[sys] = ok && n > 0;
-        if (!ok || n == 0 || k == 0) { __syncthreads(); continue; }
-
-        // ---- lu_solve (factor.rs:633-641): x = rows perm of b, then L, then U ----
-        const bool mine = tid < n;
-        const T* frow = a + (mine ? perm[tid] : 0) * n;  // this thread's factor row (position tid)
-        T acc[MAXRHS];
-#pragma unroll
-        for (int c = 0; c < MAXRHS; ++c) acc[c] = (mine && c < k) ? b[perm[tid] * k + c] : T(0);
-        // L: increasing order, unit diagonal.
-        for (int p = 0; p < n; ++p) {
-            if (tid == p) {
-#pragma unroll
-                for (int c = 0; c < MAXRHS; ++c) s_x[p][c] = acc[c];
+        if (solve) {
+            // ---- lu_solve (factor.rs:633-641): x = rows perm of b, then L, then U; or, transposed, Uᵀ then Lᵀ,
+            // then the inverse permutation ----
+            const bool tr = mode == SOLVE_T;
+            // Item (i, j) of the triangle being solved: F(i, j), or F(j, i) for the transposed solve.
+            auto ent = [&](int i, int j) -> T { return tr ? a[perm[j] * n + i] : a[perm[i] * n + j]; };
+            for (int i = tid; i < n * MAXRHS; i += BT) {
+                const int r = i / MAXRHS, c = i % MAXRHS;
+                s_x[r][c] = c < k ? b[(tr ? r : perm[r]) * k + c] : T(0);
             }
             __syncthreads();
-            if (mine && tid > p) {
-                const T lp = -frow[p];
+            // Forward: L (unit) or Uᵀ (a reciprocal multiply at each row's turn), increasing order.
+            for (int p0 = 0; p0 < n; p0 += SB) {
+                const int h = n - p0 < SB ? n - p0 : SB;
+                if (wave < k && lane < SB) {
+                    const bool act = lane < h;
+                    const int r = p0 + lane;
+                    T x = act ? s_x[r][wave] : T(0);
+                    T mj[SB];
 #pragma unroll
-                for (int c = 0; c < MAXRHS; ++c) if (c < k) acc[c] = __builtin_fma(s_x[p][c], lp, acc[c]);
-            }
-        }
-        __syncthreads();
-        if (k == 1) {
-            // One column: decreasing order, then a quotient (`single`, factor.rs:725-748).
-            for (int p = n - 1; p >= 0; --p) {
-                if (tid == p) { acc[0] = acc[0] / frow[p]; s_x[p][0] = acc[0]; }
-                __syncthreads();
-                if (mine && tid < p) acc[0] = __builtin_fma(s_x[p][0], -frow[p], acc[0]);
-            }
-        } else {
-            // Several columns: `backward`'s halving above NS rows (factor.rs:693-707). Leaves run bottom to top;
-            // once the leaf starting at a split's middle is done, the split's top half adds its bottom half.
-            for (int end = n; end > 0;) {
-                int r0 = 0, r1 = n;
-                while (r1 - r0 > NS) { const int mid = (r0 + r1) / 2; if (end <= mid) r1 = mid; else r0 = mid; }
-                for (int t = r1 - 1; t >= r0; --t) {
-                    if (tid == t) {
-                        // The rows after it in the leaf, in increasing order, then a reciprocal multiply.
-                        for (int j = t + 1; j < r1; ++j) {
-                            const T uu = -frow[j];
+                    for (int j = 0; j < SB; ++j) mj[j] = (act && j < lane) ? ent(r, p0 + j) : T(0);
+                    const T dg = (tr && act) ? ent(r, r) : T(1);
 #pragma unroll
-                            for (int c = 0; c < MAXRHS; ++c) if (c < k) acc[c] = __builtin_fma(s_x[j][c], uu, acc[c]);
-                        }
-                        const T dd = frow[t];
-                        const bool nrm = fabs(dd) >= tiny;
-                        const T rcp = T(1) / dd;
-#pragma unroll
-                        for (int c = 0; c < MAXRHS; ++c) {
-                            if (c < k) acc[c] = nrm ? acc[c] * rcp : acc[c] / dd;
-                            s_x[t][c] = acc[c];
+                    for (int j = 0; j < SB; ++j) {
+                        if (j < h) {
+                            if (tr && lane == j) x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
+                            const T xj = readlane(x, j);
+                            if (act && lane > j) x = __builtin_fma(xj, -mj[j], x);
                         }
                     }
-                    __syncthreads();
+                    if (act) s_x[r][wave] = x;
                 }
-                int a0 = 0, a1 = n;
-                while (a1 - a0 > NS) {
-                    const int mid = (a0 + a1) / 2;
-                    if (mid == r0) {
-                        if (tid >= a0 && tid < mid) {
-                            for (int j = mid; j < a1; ++j) {
-                                const T uu = -frow[j];
+                __syncthreads();
+                if (tid >= p0 + h && tid < n) {
+                    T e[SB];
 #pragma unroll
-                                for (int c = 0; c < MAXRHS; ++c) if (c < k) acc[c] = __builtin_fma(s_x[j][c], uu, acc[c]);
+                    for (int j = 0; j < SB; ++j) e[j] = j < h ? -ent(tid, p0 + j) : T(0);
+#pragma unroll
+                    for (int c = 0; c < MAXRHS; ++c) {
+                        if (c < k) {
+                            T acc = s_x[tid][c];
+#pragma unroll
+                            for (int j = 0; j < SB; ++j) if (j < h) acc = __builtin_fma(s_x[p0 + j][c], e[j], acc);
+                            s_x[tid][c] = acc;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            HS_BPHASE(3);
+            if (tr || k == 1) {
+                // Backward, decreasing order: U with a quotient at each row's turn (`single`, factor.rs:725-748), or
+                // Lᵀ (unit). Diagonal blocks sit right-aligned in the wave: lane SB - 1 holds the block's last row.
+                for (int q1 = n; q1 > 0; q1 -= SB) {
+                    const int q0 = q1 - SB > 0 ? q1 - SB : 0, h = q1 - q0, off = SB - h;
+                    if (wave < k && lane < SB) {
+                        const bool act = lane >= off;
+                        const int r = q0 + lane - off;
+                        T x = act ? s_x[r][wave] : T(0);
+                        T mj[SB];
+#pragma unroll
+                        for (int j = 0; j < SB; ++j) mj[j] = (act && j > lane) ? ent(r, q0 + j - off) : T(0);
+                        const T dg = (!tr && act) ? ent(r, r) : T(1);
+#pragma unroll
+                        for (int s = 0; s < SB; ++s) {
+                            const int L = SB - 1 - s;
+                            if (s < h) {
+                                if (!tr && lane == L) x = x / dg;
+                                const T xl = readlane(x, L);
+                                if (act && lane < L) x = __builtin_fma(xl, -mj[L], x);
                             }
                         }
-                        __syncthreads();
-                        break;
+                        if (act) s_x[r][wave] = x;
                     }
-                    if (r0 < mid) a1 = mid; else a0 = mid;
+                    __syncthreads();
+                    if (tid < q0) {
+                        T e[SB];
+#pragma unroll
+                        for (int j = 0; j < SB; ++j) e[j] = j < h ? -ent(tid, q0 + j) : T(0);
+#pragma unroll
+                        for (int c = 0; c < MAXRHS; ++c) {
+                            if (c < k) {
+                                T acc = s_x[tid][c];
+#pragma unroll
+                                for (int j = SB - 1; j >= 0; --j) if (j < h) acc = __builtin_fma(s_x[q0 + j][c], e[j], acc);
+                                s_x[tid][c] = acc;
+                            }
+                        }
+                    }
+                    __syncthreads();
                 }
-                end = r0;
+            } else {
+                // Several columns: `backward`'s halving above NS rows (factor.rs:693-707). Leaves run bottom to top,
+                // each row adding the rows after it in its leaf in increasing order, then a reciprocal multiply; once
+                // the leaf starting at a split's middle is done, the split's top half adds its bottom half.
+                for (int end = n; end > 0;) {
+                    int r0 = 0, r1 = n;
+                    while (r1 - r0 > NS) { const int mid = (r0 + r1) / 2; if (end <= mid) r1 = mid; else r0 = mid; }
+                    const int h = r1 - r0, off = SB - h;
+                    if (wave < k && lane < SB) {
+                        const bool act = lane >= off;
+                        const int r = r0 + lane - off;
+                        T x = act ? s_x[r][wave] : T(0);
+                        T mj[SB];
+#pragma unroll
+                        for (int j = 0; j < SB; ++j) mj[j] = (act && j > lane) ? a[perm[r] * n + r0 + j - off] : T(0);
+                        const T dg = act ? a[perm[r] * n + r] : T(1);
+                        T xs[SB];
+#pragma unroll
+                        for (int s = 0; s < SB; ++s) {
+                            const int L = SB - 1 - s;
+                            if (s < h) {
+                                if (lane == L) {
+#pragma unroll
+                                    for (int j = L + 1; j < SB; ++j) x = __builtin_fma(xs[j], -mj[j], x);
+                                    x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
+                                }
+                                xs[L] = readlane(x, L);
+                            }
+                        }
+                        if (act) s_x[r][wave] = x;
+                    }
+                    __syncthreads();
+                    int a0 = 0, a1 = n;
+                    while (a1 - a0 > NS) {
+                        const int mid = (a0 + a1) / 2;
+                        if (mid == r0) {
+                            if (tid >= a0 && tid < mid) {
+                                for (int j0 = mid; j0 < a1; j0 += SB) {
+                                    T e[SB];
+#pragma unroll
+                                    for (int j = 0; j < SB; ++j) e[j] = j0 + j < a1 ? -a[perm[tid] * n + j0 + j] : T(0);
+#pragma unroll
+                                    for (int c = 0; c < MAXRHS; ++c) {
+                                        if (c < k) {
+                                            T acc = s_x[tid][c];
+#pragma unroll
+                                            for (int j = 0; j < SB; ++j)
+                                                if (j0 + j < a1) acc = __builtin_fma(s_x[j0 + j][c], e[j], acc);
+                                            s_x[tid][c] = acc;
+                                        }
+                                    }
+                                }
+                            }
+                            __syncthreads();
+                            break;
+                        }
+                        if (r0 < mid) a1 = mid; else a0 = mid;
+                    }
+                    end = r0;
+                }
+            }
+            for (int i = tid; i < n * MAXRHS; i += BT) {
+                const int r = i / MAXRHS, c = i % MAXRHS;
+                if (c < k) b[(tr ? perm[r] : r) * k + c] = s_x[r][c];
             }
         }
-        if (mine) {
-#pragma unroll
-            for (int c = 0; c < MAXRHS; ++c) if (c < k) b[tid * k + c] = acc[c];
-        }
         __syncthreads();
+        HS_BPHASE(4);
     }
+#undef HS_BPHASE
 }
 
 __global__ void zero_int(int* p) { *p = 0; }
@@ -307,6 +496,7 @@ __global__ void zero_int(int* p) { *p = 0; }
 struct Plan {
     int count = 0;
     bool f64 = true;
+    int mode = FACTOR_SOLVE;
     Desc* d_desc = nullptr;
     int* d_order = nullptr;
     int* d_next = nullptr;
@@ -318,19 +508,21 @@ struct Plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+long long* g_bprof = nullptr;  // the batched kernel's phase ticks, while profiling is on
+bool g_bprof_on = false;
 std::mutex g_plans_mu;
 std::map<long long, Plan*> g_plans;
 long long g_next_plan = 1;
 
 template <typename T>
-int enqueue(Plan* p, hipStream_t s) {
+int enqueue(Plan* p, hipStream_t s, long long* prof) {
     zero_int<<<1, 1, 0, s>>>(p->d_next);
     if (p->bw == 16)
         gesv_batched_kernel<T, 16><<<p->grid, BT, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, (T*)p->b, p->perm,
-                                                         p->regular, p->d_next);
+                                                         p->regular, p->d_next, p->mode, prof);
     else
         gesv_batched_kernel<T, 32><<<p->grid, BT, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, (T*)p->b, p->perm,
-                                                         p->regular, p->d_next);
+                                                         p->regular, p->d_next, p->mode, prof);
     HS_HIP(hipGetLastError());
     return OK;
 }
@@ -344,10 +536,11 @@ Plan* find_plan(long long handle) {
 }  // namespace
 
 int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
-                 const int64_t* p_off, bool f64, void* a, void* b, int64_t* perm, int* regular, int panel_width,
+                 const int64_t* p_off, bool f64, void* a, void* b, int64_t* perm, int* regular, int mode, int panel_width,
                  int wgs_per_cu, long long* handle) {
-    if (panel_width != 16 && panel_width != 32) { set_error("batched panel width must be 16 or 32"); return -11; }
-    if (wgs_per_cu < 1 || wgs_per_cu > 2) { set_error("batched workgroups per CU must be 1 or 2"); return -12; }
+    if (mode < FACTOR_SOLVE || mode > SOLVE_T) { set_error("batched mode must be 0, 1 or 2"); return -11; }
+    if (panel_width != 16 && panel_width != 32) { set_error("batched panel width must be 16 or 32"); return -12; }
+    if (wgs_per_cu < 1 || wgs_per_cu > 2) { set_error("batched workgroups per CU must be 1 or 2"); return -13; }
     std::vector<Desc> desc(count);
     std::vector<int> order(count);
     for (int64_t i = 0; i < count; ++i) {
@@ -355,15 +548,20 @@ int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_
         if (k[i] < 0 || k[i] > MAXRHS) { set_error("batched systems need 0 <= k <= 8"); return -3; }
         if (a_off[i] < 0 || b_off[i] < 0 || p_off[i] < 0) { set_error("batched offsets must be >= 0"); return -4; }
         desc[i] = Desc{n[i], k[i], a_off[i], b_off[i], p_off[i]};
+        if (n[i] > 0 && (!a || !perm)) { set_error("batched systems need A and perm buffers"); return !a ? -7 : -9; }
+        if (n[i] * k[i] > 0 && !b) { set_error("batched right-hand sides need a B buffer"); return -8; }
     }
+    if (count > 0 && mode == FACTOR_SOLVE && !regular) { set_error("batched plans need a regular-flag buffer"); return -10; }
     // Longest processing time first: the factorization's n³ and the solve's n² k.
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        const double cx = (double)n[x] * n[x] * (n[x] + 3.0 * k[x]), cy = (double)n[y] * n[y] * (n[y] + 3.0 * k[y]);
+        const double fx = mode == FACTOR_SOLVE ? (double)n[x] : 0.0, fy = mode == FACTOR_SOLVE ? (double)n[y] : 0.0;
+        const double cx = (double)n[x] * n[x] * (fx + 3.0 * k[x]), cy = (double)n[y] * n[y] * (fy + 3.0 * k[y]);
         return cx > cy;
     });
     Plan* p = new Plan;
-    p->count = (int)count; p->f64 = f64; p->a = a; p->b = b; p->perm = (long long*)perm; p->regular = regular;
+    p->count = (int)count; p->f64 = f64; p->mode = mode; p->a = a; p->b = b; p->perm = (long long*)perm;
+    p->regular = regular;
     p->grid = device_cus() * wgs_per_cu;
     p->bw = panel_width;
     const size_t cnt = count > 0 ? (size_t)count : 1;
@@ -386,13 +584,16 @@ int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_
 int batched_run(long long handle, bool use_graph, hipStream_t s) {
     Plan* p = find_plan(handle);
     if (!p) { set_error("unknown batched plan"); return -1; }
-    if (!use_graph) return p->f64 ? enqueue<double>(p, s) : enqueue<float>(p, s);
+    if (!use_graph || g_bprof_on) {
+        long long* prof = g_bprof_on ? g_bprof : nullptr;
+        return p->f64 ? enqueue<double>(p, s, prof) : enqueue<float>(p, s, prof);
+    }
     if (!p->exec) {
         // The queue reset and the persistent launch, captured once; every run replays the graph.
         hipStream_t cap;
         HS_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         HS_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        const int rc = p->f64 ? enqueue<double>(p, cap) : enqueue<float>(p, cap);
+        const int rc = p->f64 ? enqueue<double>(p, cap, nullptr) : enqueue<float>(p, cap, nullptr);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(cap, &g);
         hipStreamDestroy(cap);
@@ -402,6 +603,15 @@ int batched_run(long long handle, bool use_graph, hipStream_t s) {
         HS_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
     }
     HS_HIP(hipGraphLaunch(p->exec, s));
+    return OK;
+}
+
+int batched_profile(int on, long long* out) {
+    if (!g_bprof) HS_HIP(hipMalloc(&g_bprof, 8 * sizeof(long long)));
+    HS_HIP(hipDeviceSynchronize());
+    if (out) HS_HIP(hipMemcpy(out, g_bprof, 8 * sizeof(long long), hipMemcpyDeviceToHost));
+    if (on) HS_HIP(hipMemset(g_bprof, 0, 8 * sizeof(long long)));
+    g_bprof_on = on != 0;
     return OK;
 }
 

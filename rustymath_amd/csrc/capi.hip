@@ -360,24 +360,26 @@ int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, doub
 }
 
 int hs_dev_dgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
-                              const int64_t* p_off, double* a, double* b, int64_t* perm, int* regular, int panel_width,
+                              const int64_t* p_off, double* a, double* b, int64_t* perm, int* regular, int mode, int panel_width,
                               int wgs_per_cu, int64_t* plan) {
     if (count < 0) return -1;
-    if (count > 0 && (!n || !k || !a_off || !b_off || !p_off || !a || !b || !perm || !regular)) return -2;
-    if (!plan) return -13;
+    if (count > 0 && (!n || !k || !a_off || !b_off || !p_off)) return -2;
+    if (!plan) return -14;
     long long h = 0;
-    const int rc = batched_plan(count, n, k, a_off, b_off, p_off, true, a, b, perm, regular, panel_width, wgs_per_cu, &h);
+    const int rc = batched_plan(count, n, k, a_off, b_off, p_off, true, a, b, perm, regular, mode, panel_width, wgs_per_cu,
+                                &h);
     *plan = h;
     return rc;
 }
 int hs_dev_sgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
-                              const int64_t* p_off, float* a, float* b, int64_t* perm, int* regular, int panel_width,
+                              const int64_t* p_off, float* a, float* b, int64_t* perm, int* regular, int mode, int panel_width,
                               int wgs_per_cu, int64_t* plan) {
     if (count < 0) return -1;
-    if (count > 0 && (!n || !k || !a_off || !b_off || !p_off || !a || !b || !perm || !regular)) return -2;
-    if (!plan) return -13;
+    if (count > 0 && (!n || !k || !a_off || !b_off || !p_off)) return -2;
+    if (!plan) return -14;
     long long h = 0;
-    const int rc = batched_plan(count, n, k, a_off, b_off, p_off, false, a, b, perm, regular, panel_width, wgs_per_cu, &h);
+    const int rc = batched_plan(count, n, k, a_off, b_off, p_off, false, a, b, perm, regular, mode, panel_width, wgs_per_cu,
+                                &h);
     *plan = h;
     return rc;
 }
@@ -387,6 +389,7 @@ int hs_dev_gesv_batched_run(int64_t plan, int graph, void* stream) {
 int hs_dev_gesv_batched_free(int64_t plan) { return batched_free(plan); }
 
 int hs_profile_panel(int on, int64_t* out) { return panel_profile(on, (long long*)out); }
+int hs_profile_batched(int on, int64_t* out) { return batched_profile(on, (long long*)out); }
 int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops) { return gemm_profile(on, ms, launches, flops); }
 
 }  // extern "C"

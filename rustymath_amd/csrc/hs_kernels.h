@@ -40,13 +40,17 @@ int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x,
 // Batched solves (batched.hip): a plan for `count` systems with orders n[i] <= 512 and k[i] <= 8 right-hand
 // sides, A_i row-major at a + a_off[i], B_i row-major n×k at b + b_off[i] (overwritten by X_i where regular),
 // perm_i at perm + p_off[i]; regular[i] = 1 where the LU solve ran, 0 where rustymath would take least
-// squares. Host arrays describe device buffers. Tuning knobs: panel width (16 or 32) and workgroups per CU
-// (1 or 2). A run replays the plan's hipGraph (or launches directly) on `s`.
+// squares. Host arrays describe device buffers. Mode 0 factors and solves; modes 1 and 2 solve A X = B and
+// Aᵀ X = B with the factors and perm that a mode-0 run left. Tuning knobs: panel width (16 or 32) and
+// workgroups per CU (1 or 2). A run replays the plan's hipGraph (or launches directly) on `s`.
 int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
-                 const int64_t* p_off, bool f64, void* a, void* b, int64_t* perm, int* regular, int panel_width,
-                 int wgs_per_cu, long long* handle);
+                 const int64_t* p_off, bool f64, void* a, void* b, int64_t* perm, int* regular, int mode,
+                 int panel_width, int wgs_per_cu, long long* handle);
 int batched_run(long long handle, bool use_graph, hipStream_t s);
 int batched_free(long long handle);
+// The batched kernel's per-phase profile (workgroup 0, 100 MHz ticks): on = 1 clears and enables (runs then
+// launch directly, not from the graph), 0 disables; out[8] gets the sums and, in out[7], the systems timed.
+int batched_profile(int on, long long* out);
 
 // The MFMA product's live profile: on = 1 clears and starts, 0 stops; gives the summed event time (ms), the
 // number of launches and their algorithmic flops (2 m n k each) since the last call.

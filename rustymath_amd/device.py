@@ -86,10 +86,13 @@ class BatchedSolve:
 
     `a` and `b` are flat device buffers holding the systems back to back: A_i row-major (n_i² items) and B_i row-major
     (n_i k_i items), in the order of `ns`/`ks` (see `pack`). A run overwrites them: A_i with its factors, rows left in
-    place (row perm_i[j] holds position j), and B_i with X_i where `regular[i]`. `panel_width` (16 or 32) and
+    place (row perm_i[j] holds position j), and B_i with X_i where `regular[i]`. Modes 'lu_solve' and 'solve_t' take
+    such factors and their `perm` and solve A_i X = B_i or A_iᵀ X = B_i in place on b. `panel_width` (16 or 32) and
     `wgs_per_cu` (1 or 2) change the speed, never the results; `tune` picks them."""
 
-    def __init__(self, a, b, ns, ks, panel_width=32, wgs_per_cu=1):
+    MODES = {'solve': 0, 'lu_solve': 1, 'solve_t': 2}
+
+    def __init__(self, a, b, ns, ks, panel_width=32, wgs_per_cu=1, mode='solve', perm=None):
         import ctypes
         import numpy as np
         if not (a.is_cuda and b.is_cuda): raise ValueError('batched buffers must be GPU tensors')
@@ -103,15 +106,19 @@ class BatchedSolve:
         self.p_off = np.concatenate([[0], np.cumsum(ns)[:-1]]).astype(np.int64) if len(ns) else ns.copy()
         if a.numel() < int((ns * ns).sum()) or b.numel() < int((ns * ks).sum()):
             raise ValueError('the buffers are smaller than the systems they should hold')
-        self.a, self.b = a, b
-        self.perm = torch.empty(max(int(ns.sum()), 1), dtype=torch.int64, device=a.device)
+        if mode not in self.MODES: raise ValueError(f'mode must be one of {sorted(self.MODES)}')
+        if mode != 'solve' and (perm is None or perm.dtype != torch.int64 or perm.numel() < int(ns.sum())):
+            raise ValueError("modes 'lu_solve' and 'solve_t' take the int64 perm of an earlier 'solve' run")
+        self.a, self.b, self.mode = a, b, mode
+        self.perm = perm if perm is not None else torch.empty(max(int(ns.sum()), 1), dtype=torch.int64, device=a.device)
         self.regular = torch.zeros(max(len(ns), 1), dtype=torch.int32, device=a.device)
         self.panel_width, self.wgs_per_cu = panel_width, wgs_per_cu
         h = ctypes.c_int64(0)
         p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
         f = getattr(_lib.load(), f'hs_dev_{_sfx(a.dtype)}gesv_batched_plan')
         _lib.check(f(len(ns), p(ns), p(ks), p(self.a_off), p(self.b_off), p(self.p_off), a.data_ptr(), b.data_ptr(),
-                     self.perm.data_ptr(), self.regular.data_ptr(), panel_width, wgs_per_cu, ctypes.byref(h)),
+                     self.perm.data_ptr(), self.regular.data_ptr(), self.MODES[mode], panel_width, wgs_per_cu,
+                     ctypes.byref(h)),
                    'gesv_batched_plan')
         self.handle = h.value
 

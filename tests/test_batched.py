@@ -182,3 +182,40 @@ def test_autotuner_picks_a_candidate_and_remembers_it(dv):
     assert choice in times and len(times) == 4 and all(t > 0 for t in times.values())
     assert dv.tune(a, b, ns_, ks_)[0] == choice
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_factors_are_reused_for_solves_and_transposed_solves(dv, dtype):
+    """A factor-and-solve run leaves factors and perm that the 'lu_solve' mode reuses for new right-hand sides (the
+    oracle's lu_solve on the same factors, bit for bit) and the 'solve_t' mode for Aᵀ Y = W (the large-matrix
+    transposed solve's bits, and a small residual)."""
+    import torch
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rng = np.random.default_rng(4)
+    ns = [1, 5, 16, 33, 64, 100, 257, 512]
+    mats = [rng.standard_normal((n, n)).astype(dtype) for n in ns]
+    rhs = [rng.standard_normal((n, 1 + i % 8)).astype(dtype) for i, n in enumerate(ns)]
+    a, b, ns_, ks_ = dv.pack(mats, rhs, dtype=tdt)
+    plan = dv.BatchedSolve(a, b, ns_, ks_)
+    plan.run()
+    rhs2 = [rng.standard_normal((n, 8 - i % 8)).astype(dtype) for i, n in enumerate(ns)]
+    _, b2, _, ks2 = dv.pack(mats, rhs2, dtype=tdt)
+    again = dv.BatchedSolve(a, b2, ns_, ks2, mode='lu_solve', perm=plan.perm)
+    again.run()
+    _, b3, _, ks3 = dv.pack(mats, rhs2, dtype=tdt)
+    trans = dv.BatchedSolve(a, b3, ns_, ks3, mode='solve_t', perm=plan.perm)
+    trans.run()
+    torch.cuda.synchronize()
+    for i, n in enumerate(ns):
+        wf, wperm, _ = oracle.lu(mats[i])
+        assert bits_equal(again.x(i).cpu().numpy(), oracle.lu_solve(wf, wperm, rhs2[i])), f'lu_solve n={n}'
+        f = torch.from_numpy(mats[i]).cuda()
+        f, perm = dv.getrf(f)
+        w = torch.from_numpy(rhs2[i]).cuda()
+        dv.getrs(f, perm, w, trans=True)
+        y = trans.x(i)
+        assert bits_equal(y.cpu().numpy(), w.cpu().numpy()), f'solve_t n={n}'
+        r = np.linalg.norm(mats[i].T.astype(np.float64) @ y.cpu().numpy() - rhs2[i])
+        scale = np.linalg.norm(mats[i].astype(np.float64)) * np.linalg.norm(y.cpu().numpy())
+        assert r <= 10 * n * np.finfo(dtype).eps * scale
+    for p in (plan, again, trans): p.close()

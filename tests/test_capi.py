@@ -54,12 +54,15 @@ def test_batched_plan_rejects_bad_systems_on_the_host():
     plan = ctypes.c_int64(0)
     def call(n, k, off=0, pw=32, wg=1, count=1):
         n = np.array([n], np.int64); k = np.array([k], np.int64); o = np.array([off], np.int64)
-        return lib.hs_dev_dgesv_batched_plan(count, p(n), p(k), p(o), p(o), p(o), ptr, ptr, ptr, reg, pw, wg,
+        return lib.hs_dev_dgesv_batched_plan(count, p(n), p(k), p(o), p(o), p(o), ptr, ptr, ptr, reg, mode, pw, wg,
                                              ctypes.byref(plan))
+    mode = 0
     assert call(4, 1, count=-1) == -1
     assert call(513, 1) == -2
     assert call(4, 9) == -3
     assert call(4, 1, off=-1) == -4
-    assert call(4, 1, pw=24) == -11
-    assert call(4, 1, wg=3) == -12
+    assert call(4, 1, pw=24) == -12
+    assert call(4, 1, wg=3) == -13
+    mode = 3
+    assert call(4, 1) == -11
     assert lib.hs_dev_gesv_batched_free(12345) == -1

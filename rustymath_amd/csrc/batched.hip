@@ -55,9 +55,11 @@ template <typename T>
 __device__ __forceinline__ void wave_best(T& v, int& r) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        T v2 = __shfl_xor(v, off);
-        int r2 = __shfl_xor(r, off);
-        if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
+        const T v2 = __shfl_xor(v, off);
+        const int r2 = __shfl_xor(r, off);
+        const bool take = (v2 > v) | ((v2 == v) & (r2 < r));
+        v = take ? v2 : v;
+        r = take ? r2 : r;
     }
 }
 
@@ -74,64 +76,96 @@ __device__ __forceinline__ T readlane(T v, int l) {
     }
 }
 
+// One system's matrix as a buffer resource: loads at an index past its n² items return +0 and stores there are
+// dropped (the buffer unit's range check), so guarded loads need no branch and stay in flight together.
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+template <typename T>
+struct Mat {
+    __amdgpu_buffer_rsrc_t r;
+    unsigned items;
+    __device__ __forceinline__ Mat(T* p, int n) : items((unsigned)n * (unsigned)n) {
+        r = __builtin_amdgcn_make_buffer_rsrc(p, 0, (int)(items * sizeof(T)), 0x00020000);
+    }
+    __device__ __forceinline__ T ld(unsigned i) const {
+        if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, i * 8u, 0, 0));
+        else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4u, 0, 0));
+    }
+    __device__ __forceinline__ void st(unsigned i, T v) const {
+        if constexpr (sizeof(T) == 8) __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, v), r, i * 8u, 0, 0);
+        else __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, i * 4u, 0, 0);
+    }
+    // i where ok, else an index past the end.
+    __device__ __forceinline__ unsigned at(bool ok, int i) const { return ok ? (unsigned)i : items; }
+};
+
 // The trailing update of one panel on MFMA: A22 (rows and columns [c1, n)) += (−L21) U12, with L21 the panel's
 // multipliers (columns [c0, c0 + w)) and U12 its rows of U. Each wave owns 32×32 blocks (2×2 MFMA tiles).
 template <typename T, int BW>
-__device__ __forceinline__ void trailing_update(T* __restrict__ a, int n, int c0, int w, int c1, const short* perm,
+__device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, int w, int c1, const short* perm,
                                                 int wave, int lane) {
     typedef typename Mfma<T>::acc V;
-    const int m = n - c1, nb = (m + 31) / 32;
+    const int m = n - c1, nb = (m + 31) / 32, last = n - 1;
     const int lr = lane & 15, lk = lane >> 4;
+    // The MFMA takes l · (−u): terms outside the matrix or past the panel load as +0 on the L side and −0 on the
+    // U side, so padding adds (+0)·(−0) = −0, and (−u)·l is the reference's term (its α = −1 folds into U12).
     for (int blk = wave; blk < nb * nb; blk += WAVES) {
         const int rb = c1 + 32 * (blk / nb), cb = c1 + 32 * (blk % nb);
         V acc[2][2];
-        int crow[2][4];
+        unsigned cidx[2][2][4];
 #pragma unroll
         for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int r = rb + 16 * ti + Mfma<T>::row(lane, e);
-                crow[ti][e] = r < n ? perm[r] * n : -1;
-            }
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-            for (int tj = 0; tj < 2; ++tj) {
-                const int col = cb + 16 * tj + lr;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[ti][tj][e] = (crow[ti][e] >= 0 && col < n) ? a[crow[ti][e] + col] : T(0);
-            }
-        int arow[2];
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti) { const int r = rb + 16 * ti + lr; arow[ti] = r < n ? perm[r] * n + c0 : -1; }
-#pragma unroll
-        for (int s = 0; s < BW / 4; ++s) {
-            if (4 * s < w) {
-                const int kk = 4 * s + lk;
-                const int brow = kk < w ? perm[c0 + kk] * n : -1;
-                T av[2], bv[2];
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti) av[ti] = (kk < w && arow[ti] >= 0) ? -a[arow[ti] + kk] : T(0);
+                const int prow = perm[r < n ? r : last] * n;
 #pragma unroll
                 for (int tj = 0; tj < 2; ++tj) {
                     const int col = cb + 16 * tj + lr;
-                    bv[tj] = (brow >= 0 && col < n) ? a[brow + col] : T(-0.0);
+                    cidx[ti][tj][e] = A.at(r < n && col < n, prow + col);
                 }
-#pragma unroll
-                for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-                    for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = Mfma<T>::step(av[ti], bv[tj], acc[ti][tj]);
             }
-        }
 #pragma unroll
         for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[ti][tj][e] = A.ld(cidx[ti][tj][e]);
+        int arow[2];
+        bool aok[2];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) {
+            const int r = rb + 16 * ti + lr;
+            aok[ti] = r < n;
+            arow[ti] = perm[r < n ? r : last] * n + c0;
+        }
+        // All BW / 4 steps, also past a narrower panel's end (padding then); every operand is loaded before the
+        // first MFMA, so that the block's loads are in flight together.
+        T av[BW / 4][2], bv[BW / 4][2];
+#pragma unroll
+        for (int s = 0; s < BW / 4; ++s) {
+            const int kk = 4 * s + lk;
+            const bool kok = kk < w;
+            const int brow = perm[c0 + (kok ? kk : 0)] * n;
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti) av[s][ti] = A.ld(A.at(kok && aok[ti], arow[ti] + kk));
+#pragma unroll
             for (int tj = 0; tj < 2; ++tj) {
                 const int col = cb + 16 * tj + lr;
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                    if (crow[ti][e] >= 0 && col < n) a[crow[ti][e] + col] = acc[ti][tj][e];
+                bv[s][tj] = -A.ld(A.at(kok && col < n, brow + col));
             }
+        }
+#pragma unroll
+        for (int s = 0; s < BW / 4; ++s)
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = Mfma<T>::step(av[s][ti], bv[s][tj], acc[ti][tj]);
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) A.st(cidx[ti][tj][e], acc[ti][tj][e]);
     }
 }
 
@@ -179,6 +213,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
         const int n = (int)d.n, k = (int)d.k;
         T* a = a_all + d.a_off;
         T* b = b_all + d.b_off;
+        const Mat<T> A(a, n);
         long long* gperm = perm_all + d.p_off;
         bool solve;
 
@@ -227,11 +262,11 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
             for (int c0 = 0; c0 < n;) {
                 const int c1 = panel_end[c0], w = c1 - c0, rows = n - c0;
                 const bool has = tid < rows;
-                const int row = has ? perm[c0 + tid] : 0;
+                const int row = perm[c0 + (has ? tid : 0)];
                 int pos = tid;
                 T v[BW];
 #pragma unroll
-                for (int j = 0; j < BW; ++j) v[j] = (has && j < w) ? a[row * n + c0 + j] : T(0);
+                for (int j = 0; j < BW; ++j) v[j] = A.ld(A.at(has && j < w, row * n + c0 + j));
 #pragma clang loop unroll(disable)
                 for (int c = 0; c < w; ++c) {
                     const int buf = c & 1;
@@ -248,13 +283,18 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     }
                     if (lane == 0) { s_wval[buf][wave] = bv; s_wkey[buf][wave] = bk; }
                     __syncthreads();
-                    T wb = s_wval[buf][0];
-                    int p = s_wkey[buf][0], win = 0;
+                    T wv[WAVES];
+                    int wk[WAVES];
+#pragma unroll
+                    for (int u = 0; u < WAVES; ++u) { wv[u] = s_wval[buf][u]; wk[u] = s_wkey[buf][u]; }
+                    T wb = wv[0];
+                    int p = wk[0], win = 0;
 #pragma unroll
                     for (int u = 1; u < WAVES; ++u) {
-                        const T vu = s_wval[buf][u];
-                        const int ku = s_wkey[buf][u];
-                        if (vu > wb || (vu == wb && ku < p)) { wb = vu; p = ku; win = u; }
+                        const bool take = (wv[u] > wb) | ((wv[u] == wb) & (wk[u] < p));
+                        wb = take ? wv[u] : wb;
+                        p = take ? wk[u] : p;
+                        win = take ? u : win;
                     }
                     const T* piv = s_cand[buf][win];
                     if (tid < BW) s_u11[c][tid] = piv[tid];
@@ -294,9 +334,10 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     // u[j] holds row qq + j of the column: step qq finishes row qq and shifts the rest down.
                     const int col = c1 + tid;
                     const bool live = col < n;
+                    const int colc = live ? col : n - 1;
                     T u[BW];
 #pragma unroll
-                    for (int r = 0; r < BW; ++r) u[r] = (live && r < w) ? a[perm[c0 + r] * n + col] : T(0);
+                    for (int r = 0; r < BW; ++r) u[r] = A.ld(A.at(live && r < w, perm[c0 + (r < w ? r : 0)] * n + colc));
                     __syncthreads();
 #pragma clang loop unroll(disable)
                     for (int qq = 0; qq < w; ++qq) {
@@ -307,7 +348,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                         u[BW - 1] = T(0);
                     }
                     __syncthreads();
-                    trailing_update<T, BW>(a, n, c0, w, c1, perm, wave, lane);
+                    trailing_update<T, BW>(A, n, c0, w, c1, perm, wave, lane);
                 }
                 __syncthreads();
                 HS_BPHASE(2);
@@ -332,7 +373,8 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
             // then the inverse permutation ----
             const bool tr = mode == SOLVE_T;
             // Item (i, j) of the triangle being solved: F(i, j), or F(j, i) for the transposed solve.
-            auto ent = [&](int i, int j) -> T { return tr ? a[perm[j] * n + i] : a[perm[i] * n + j]; };
+            auto ent_at = [&](bool ok, int i, int j) -> unsigned { return A.at(ok, tr ? perm[j] * n + i : perm[i] * n + j); };
+            auto ent = [&](int i, int j) -> T { return A.ld(ent_at(true, i, j)); };
             for (int i = tid; i < n * MAXRHS; i += BT) {
                 const int r = i / MAXRHS, c = i % MAXRHS;
                 s_x[r][c] = c < k ? b[(tr ? r : perm[r]) * k + c] : T(0);
@@ -343,12 +385,13 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                 const int h = n - p0 < SB ? n - p0 : SB;
                 if (wave < k && lane < SB) {
                     const bool act = lane < h;
-                    const int r = p0 + lane;
+                    const int r = p0 + (act ? lane : 0);
                     T x = act ? s_x[r][wave] : T(0);
                     T mj[SB];
 #pragma unroll
-                    for (int j = 0; j < SB; ++j) mj[j] = (act && j < lane) ? ent(r, p0 + j) : T(0);
-                    const T dg = (tr && act) ? ent(r, r) : T(1);
+                    for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j < lane, r, p0 + (j < h ? j : 0)));
+                    const T dgl = ent(r, r);
+                    const T dg = (tr && act) ? dgl : T(1);
 #pragma unroll
                     for (int j = 0; j < SB; ++j) {
                         if (j < h) {
@@ -363,7 +406,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                 if (tid >= p0 + h && tid < n) {
                     T e[SB];
 #pragma unroll
-                    for (int j = 0; j < SB; ++j) e[j] = j < h ? -ent(tid, p0 + j) : T(0);
+                    for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, tid, p0 + (j < h ? j : 0)));
 #pragma unroll
                     for (int c = 0; c < MAXRHS; ++c) {
                         if (c < k) {
@@ -384,12 +427,13 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     const int q0 = q1 - SB > 0 ? q1 - SB : 0, h = q1 - q0, off = SB - h;
                     if (wave < k && lane < SB) {
                         const bool act = lane >= off;
-                        const int r = q0 + lane - off;
+                        const int r = act ? q0 + lane - off : q0;
                         T x = act ? s_x[r][wave] : T(0);
                         T mj[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) mj[j] = (act && j > lane) ? ent(r, q0 + j - off) : T(0);
-                        const T dg = (!tr && act) ? ent(r, r) : T(1);
+                        for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j > lane, r, j >= off ? q0 + j - off : q0));
+                        const T dgl = ent(r, r);
+                        const T dg = (!tr && act) ? dgl : T(1);
 #pragma unroll
                         for (int s = 0; s < SB; ++s) {
                             const int L = SB - 1 - s;
@@ -405,7 +449,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     if (tid < q0) {
                         T e[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) e[j] = j < h ? -ent(tid, q0 + j) : T(0);
+                        for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, tid, q0 + (j < h ? j : 0)));
 #pragma unroll
                         for (int c = 0; c < MAXRHS; ++c) {
                             if (c < k) {
@@ -428,12 +472,13 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     const int h = r1 - r0, off = SB - h;
                     if (wave < k && lane < SB) {
                         const bool act = lane >= off;
-                        const int r = r0 + lane - off;
+                        const int r = act ? r0 + lane - off : r0;
                         T x = act ? s_x[r][wave] : T(0);
                         T mj[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) mj[j] = (act && j > lane) ? a[perm[r] * n + r0 + j - off] : T(0);
-                        const T dg = act ? a[perm[r] * n + r] : T(1);
+                        for (int j = 0; j < SB; ++j) mj[j] = A.ld(A.at(act && j > lane, perm[r] * n + (j >= off ? r0 + j - off : r0)));
+                        const T dgl = a[perm[r] * n + r];
+                        const T dg = act ? dgl : T(1);
                         T xs[SB];
 #pragma unroll
                         for (int s = 0; s < SB; ++s) {
@@ -458,7 +503,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                                 for (int j0 = mid; j0 < a1; j0 += SB) {
                                     T e[SB];
 #pragma unroll
-                                    for (int j = 0; j < SB; ++j) e[j] = j0 + j < a1 ? -a[perm[tid] * n + j0 + j] : T(0);
+                                    for (int j = 0; j < SB; ++j) e[j] = -A.ld(A.at(j0 + j < a1, perm[tid] * n + j0 + j));
 #pragma unroll
                                     for (int c = 0; c < MAXRHS; ++c) {
                                         if (c < k) {

@@ -54,6 +54,84 @@ def batched_workload(count=10000, seed=3, nmin=8, nmax=512, kmax=8, dtype=None):
     return a.to(dtype), b.to(dtype), ns, ks
 
 
+def batched_bench(reps=3):
+    """The batched uneven-solve workload on the GPU: the autotuned plan's graph replayed `reps` times on fresh copies
+    of the systems, timed with HIP events; solves/s, GFLOP/s (Σ 2/3 n³ + 2 n² k) and the FP64 MFMA fraction."""
+    import numpy as np
+    import torch
+    from rustymath_amd import device as dv
+    a0, b0, ns, ks = batched_workload()
+    flops = float((2 / 3 * ns.astype(np.float64) ** 3 + 2.0 * ns * ns * ks).sum())
+    (pw, wg), tuned = dv.tune(a0, b0, ns, ks)
+    a, b = a0.clone(), b0.clone()
+    plan = dv.BatchedSolve(a, b, ns, ks, panel_width=pw, wgs_per_cu=wg)
+    times = []
+    for r in range(reps + 1):
+        a.copy_(a0); b.copy_(b0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(); plan.run(); e1.record(); e1.synchronize()
+        if r: times.append(e0.elapsed_time(e1) / 1e3)
+    ok = bool((plan.regular[:len(ns)] == 1).all())
+    plan.close()
+    t = min(times)
+    big = ns >= 256
+    return {'workload': f'{len(ns)} float64 systems, n uniform in [8, 512], k uniform in [1, 8], A = G + sqrt(n) I, '
+                        'seed 3; one persistent launch replayed from a hipGraph',
+            'solves_per_s': round(len(ns) / t, 1), 'ms': round(t * 1e3, 3), 'gflops': round(flops / t / 1e9, 2),
+            'frac_fp64_peak': round(flops / t / PEAK['fp64_mfma'], 4), 'all_regular': ok,
+            'tuned': {'panel_width': pw, 'wgs_per_cu': wg},
+            'tuning_ms': {f'{k[0]}x{k[1]}': round(v, 3) for k, v in tuned.items()},
+            'systems_n_ge_256': int(big.sum())}
+
+
+def batched_cpu_baseline(sample=800, threads=16):
+    """The oracle's per-cell solve (rustymath's LuWork::solve restated) over a sample of the batched workload's
+    distribution, one system per host thread as rustymath's stacks run, in solves/s."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+    import oracle
+    oracle.set_threads(1)
+    rng = np.random.default_rng(3)
+    ns = rng.integers(8, 513, size=sample); ks = rng.integers(1, 9, size=sample)
+    mats = [rng.standard_normal((n, n)) + np.sqrt(n) * np.eye(n) for n in ns]
+    rhs = [rng.standard_normal((n, k)) for n, k in zip(ns, ks)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda i: oracle.solve(mats[i], rhs[i]), range(sample)))
+    t = time.perf_counter() - t0
+    oracle.set_threads(threads)
+    flops = float((2 / 3 * ns.astype(np.float64) ** 3 + 2.0 * ns * ns * ks).sum())
+    return {'value': round(sample / t, 1), 'unit': 'solves/s', 'gflops': round(flops / t / 1e9, 2), 'cores': threads,
+            'kind': 'port', 'sample': f'{sample} systems of the same distribution (numpy seed 3), oracle solve, one '
+                                      f'system per thread on {threads} threads'}
+
+
+def adjoint_bench(reps=3):
+    """loss = sum(w ⊙ solve(A, C D)) and its gradients w.r.t. A, C, D through the hand-written adjoint: 256 systems
+    of order 256 and one of order 4096, C n×n, D and w n×16, A = G + sqrt(n) I; milliseconds per loss + gradient."""
+    import torch
+    from rustymath_amd import adjoint
+    out = {}
+    for n, batch in ((256, 256), (4096, 0)):
+        g = torch.Generator(device='cuda'); g.manual_seed(3)
+        shape = (batch,) if batch else ()
+        a = torch.randn(*shape, n, n, dtype=torch.float64, device='cuda', generator=g)
+        a += n ** 0.5 * torch.eye(n, dtype=torch.float64, device='cuda')
+        c = torch.randn(*shape, n, n, dtype=torch.float64, device='cuda', generator=g)
+        d = torch.randn(*shape, n, 16, dtype=torch.float64, device='cuda', generator=g)
+        w = torch.randn(*shape, n, 16, dtype=torch.float64, device='cuda', generator=g)
+        best = math.inf
+        for r in range(reps + 1):
+            xs = [x.clone().requires_grad_(True) for x in (a, c, d)]
+            torch.cuda.synchronize(); t0 = time.perf_counter()
+            (w * adjoint.solve(xs[0], adjoint.matmul(xs[1], xs[2]))).sum().backward()
+            torch.cuda.synchronize()
+            if r: best = min(best, time.perf_counter() - t0)
+        out[f'{max(batch, 1)}x{n}_ms'] = round(best * 1e3, 3)
+    out['workload'] = 'loss = sum(w * solve(A, C D)) + gradients in A, C, D; C n x n, D and w n x 16; float64'
+    return out
+
+
 def cpu_baseline(n=6144, runs=3, threads=16):
     """The oracle's LU solve (the C restatement of rustymath's blocked CPU kernels) on a bounded sample: an FP64 N(0, 1)
     system of order n on `threads` host threads, best of `runs`, in GFLOP/s (2/3 n³ + 2 n²)."""
@@ -150,6 +228,9 @@ def main():
     except AttributeError:
         pass
 
+    batched = batched_bench()
+    adj = adjoint_bench()
+
     out = {
         'metric': METRIC, 'value': round(flops / t / 1e9, 2), 'unit': 'GFLOP/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(t * 1e3, 3), 'higher_is_better': True, 'scaling': 'weak',
@@ -158,10 +239,11 @@ def main():
                                f's = logspace(0, -6)), cond 1e6, b N(0,1), seed 3', 'n': n, 'rhs': 1},
         'fp64_lu_solve_gflops': round(flops / t64 / 1e9, 2), 'speedup_vs_fp64_lu': round(t64 / t, 3),
         'refinement_solves': iters, 'backward_error': berr, 'backward_error_fp64_lu': r64, 'tolerance': tol,
-        'forward_error_vs_fp64_lu': fwd, 'roofline': roof,
+        'forward_error_vs_fp64_lu': fwd, 'roofline': roof, 'batched': batched, 'adjoint': adj,
     }
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline()
+        out['batched']['cpu_baseline'] = batched_cpu_baseline()
     print(json.dumps(out))
 
 

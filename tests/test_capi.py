@@ -66,3 +66,11 @@ def test_batched_plan_rejects_bad_systems_on_the_host():
     mode = 3
     assert call(4, 1) == -11
     assert lib.hs_dev_gesv_batched_free(12345) == -1
+
+
+def test_mixed_precision_solve_rejects_bad_arguments_on_the_host():
+    from rustymath_amd import _lib
+    lib = _lib.load()
+    assert lib.hs_dev_dsgesv(-1, None, 1, None, None, 10, 0.0, None, None, None) == -1
+    assert lib.hs_dev_dsgesv(4, None, 5, None, None, 10, 0.0, None, None, None) == -3
+    assert lib.hs_dev_dsgesv(4, None, 4, None, None, 0, 0.0, None, None, None) == -6

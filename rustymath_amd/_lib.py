@@ -53,6 +53,13 @@ def load():
     "The loaded library, with argument types set; raises where it has not been built."
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64.so.7; with one soname the first one loaded serves the whole process, and
+        # torch's device code needs its own. Loading torch first makes this library share that runtime (and the
+        # same device pointers and streams).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(PATH):
             raise RuntimeError(f'{PATH} is not built: run `python -m rustymath_amd.build` (hipcc, gfx950)')
         lib = ctypes.CDLL(PATH)

@@ -56,19 +56,22 @@ template <typename T> struct Slot {
     T cand[PW];
 };
 
-// Prefers the larger magnitude, then the smaller row: `largest`'s first maximum (factor.rs:476).
+// Prefers the larger magnitude, then the smaller position: `largest`'s first maximum (factor.rs:476). Selects,
+// not branches, so that a reduction step costs a few instructions.
 template <typename T>
-__device__ __forceinline__ void better(T& v, long long& r, T v2, long long r2) {
-    if (v2 > v || (v2 == v && r2 < r)) { v = v2; r = r2; }
+__device__ __forceinline__ bool better(T v, int r, T v2, int r2) {
+    return (v2 > v) | ((v2 == v) & (r2 < r));
 }
 
 template <typename T>
-__device__ __forceinline__ void wave_best(T& v, long long& r) {
+__device__ __forceinline__ void wave_best(T& v, int& r) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        T v2 = __shfl_xor(v, off);
-        long long r2 = __shfl_xor(r, off);
-        better(v, r, v2, r2);
+        const T v2 = __shfl_xor(v, off);
+        const int r2 = __shfl_xor(r, off);
+        const bool take = better(v, r, v2, r2);
+        v = take ? v2 : v;
+        r = take ? r2 : r;
     }
 }
 
@@ -79,9 +82,9 @@ __device__ __forceinline__ long long ticks() { return __builtin_amdgcn_s_memreal
 //
 // One thread holds one row in registers, shifted so that v[0] is always the current column: at column c,
 // v[j] is the row's item in column c + j. The rows never move between threads. A swap only relabels
-// positions (each thread knows its row's current position `pos`), and every finished item (the row's
-// multiplier at its column, or the pivot row's U items at its pivot step) goes to `scratch[column][row]`,
-// from which each row is written once, at its final position, when the panel is done.
+// positions (each thread knows its row's current position `pos`); each multiplier goes to
+// `scratch[column][row]` and each pivot row's U items to LDS (every workgroup sees every pivot row), from
+// which each row is written once, at its final position, when the panel is done.
 //
 // With one workgroup, a column's pivot takes workgroup barriers only. With several (launch indices 0, 8,
 // 16, ..., one XCD under round-robin dispatch, one workgroup per CU, so all are resident; the other launch
@@ -112,58 +115,80 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
 
     const int64_t phys = (int64_t)g * PT + tid;  // this thread's row, relative to c0
     const bool has = phys < rows;
-    int64_t pos = phys;
+    int pos = (int)phys;
     T v[PW];
     {
-        const T* src = a + (c0 + phys) * lda + c0;
+        const T* src = a + (c0 + (has ? phys : 0)) * lda + c0;
 #pragma unroll
-        for (int j = 0; j < PW; ++j) v[j] = (has && j < w) ? src[j] : T(0);
+        for (int j = 0; j < PW; ++j) {
+            const T x = src[j < w ? j : 0];
+            v[j] = (has && j < w) ? x : T(0);
+        }
     }
 
-    __shared__ T s_val[WAVES];
-    __shared__ long long s_key[WAVES];
+    // Double-buffered by column, so that a workgroup alone needs one barrier per column: each wave's best
+    // candidate (magnitude, position, row), the pivot rows of U, and the pivot row of the current column.
+    __shared__ T s_val[2][WAVES];
+    __shared__ int s_key[2][WAVES];
+    __shared__ T s_cand[2][WAVES][PW];
+    __shared__ T s_u11[PW][PW];
     __shared__ T s_piv[PW];
-    __shared__ long long s_p;
-    __shared__ int s_gp, s_abort;
+    __shared__ int s_p, s_gp, s_abort;
     if (tid == 0) { s_abort = 0; if (g == 0) moves[0] = 0; }
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
 
 #pragma clang loop unroll(disable)
     for (int c = 0; c < w; ++c) {
-        // This workgroup's candidate.
+        const int buf = c & 1;
+        // This wave's candidate, published with its row.
         T val = T(-1);
-        long long key = 0x7fffffffffffffffll;
+        int key = 0x7fffffff;
         if (has && pos >= c) {
-            T x = fabs(v[0]);
+            const T x = fabs(v[0]);
             val = x != x ? T(-1) : x;
             key = pos;
         }
-        wave_best(val, key);
-        if (lane == 0) { s_val[wave] = val; s_key[wave] = key; }
+        T bv = val;
+        int bk = key;
+        wave_best(bv, bk);
+        if (bk != 0x7fffffff && key == bk) {
+#pragma unroll
+            for (int j = 0; j < PW; ++j) s_cand[buf][wave][j] = v[j];
+        }
+        if (lane == 0) { s_val[buf][wave] = bv; s_key[buf][wave] = bk; }
         __syncthreads();
         HS_PHASE(0);
-        T wb = s_val[0];
-        long long wk = s_key[0];
+        T wv[WAVES];
+        int wk[WAVES];
 #pragma unroll
-        for (int q = 1; q < WAVES; ++q) better(wb, wk, s_val[q], s_key[q]);
-        if (has && pos == wk) {
+        for (int q = 0; q < WAVES; ++q) { wv[q] = s_val[buf][q]; wk[q] = s_key[buf][q]; }
+        T wb = wv[0];
+        int wbk = wk[0], win = 0;
 #pragma unroll
-            for (int j = 0; j < PW; ++j) s_piv[j] = v[j];
+        for (int q = 1; q < WAVES; ++q) {
+            const bool take = better(wb, wbk, wv[q], wk[q]);
+            wb = take ? wv[q] : wb;
+            wbk = take ? wk[q] : wbk;
+            win = take ? q : win;
         }
-        __syncthreads();
+        const T* cand = s_cand[buf][win];
+        const T* piv = cand;
+        int p = wbk;
         HS_PHASE(1);
 
-        long long p = wk;
         if (nwg > 1) {
             Slot<T>* mine = slots + (c & 1) * MAX_WG + g;
-            if (tid < w - c) put(&mine->cand[tid], s_piv[tid]);
-            if (tid == 0) { put(&mine->val, wb); put64(&mine->row, wk); }
-            __syncthreads();  // every store of this workgroup has completed (s_waitcnt before the barrier)
+            if (tid < w - c) put(&mine->cand[tid], cand[tid]);
+            if (tid == 0) { put(&mine->val, wb); put64(&mine->row, wbk); }
+            // Every slot store of this workgroup reaches L2 before the flag can: the barrier alone does not wait
+            // for stores.
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
             const unsigned long long target = epoch * 1024ull + (unsigned long long)c + 1ull;
             if (tid == 0) __hip_atomic_store(&flags[g], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (wave == 0) {
                 T gv = T(-1);
-                long long gk = 0x7fffffffffffffffll;
+                int gk = 0x7fffffff, who = lane;
                 bool ok = true;
                 if (lane < nwg) {
                     uint64_t spins = 0;
@@ -173,21 +198,23 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     if (ok) {
                         const Slot<T>* sl = slots + (c & 1) * MAX_WG + lane;
                         gv = get(&sl->val);
-                        gk = get64(&sl->row);
+                        gk = (int)get64(&sl->row);
                     }
                 }
                 const bool all = __all(ok);
                 // Reduce (magnitude, position) and remember which workgroup holds the winner.
-                long long who = lane;
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) {
-                    T v2 = __shfl_xor(gv, off);
-                    long long k2 = __shfl_xor(gk, off), w2 = __shfl_xor(who, off);
-                    if (v2 > gv || (v2 == gv && k2 < gk)) { gv = v2; gk = k2; who = w2; }
+                    const T v2 = __shfl_xor(gv, off);
+                    const int k2 = __shfl_xor(gk, off), w2 = __shfl_xor(who, off);
+                    const bool take = better(gv, gk, v2, k2);
+                    gv = take ? v2 : gv;
+                    gk = take ? k2 : gk;
+                    who = take ? w2 : who;
                 }
                 if (lane == 0) {
                     s_p = gk;
-                    s_gp = (int)who;
+                    s_gp = who;
                     if (!all) { s_abort = 1; atomicExch(status, ERR_LIMIT); }
                 }
             }
@@ -195,47 +222,58 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
             HS_PHASE(2);
             if (s_abort) return;
             p = s_p;
-            if (s_gp != g && tid < w - c) s_piv[tid] = get(&slots[(c & 1) * MAX_WG + s_gp].cand[tid]);
+            const int gp = s_gp;
+            if (tid < PW) s_piv[tid] = gp == g ? cand[tid] : (tid < w - c ? get(&slots[(c & 1) * MAX_WG + gp].cand[tid]) : T(0));
             __syncthreads();
+            piv = s_piv;
             HS_PHASE(3);
         }
 
-        // The winner takes position c and its U items are final; the row at position c moves to p.
-        const T pivot = s_piv[0];
-        if (has && pos == p) {
-#pragma unroll
-            for (int j = 0; j < PW; ++j)
-                if (j < w - c) scratch[(int64_t)(c + j) * rows + phys] = v[j];
-            pos = c;
-        } else if (has && pos == c) {
-            pos = p;
-        }
+        // The winner takes position c; its row of U is kept in LDS. The row at position c moves to p.
+        if (tid < PW) s_u11[c][tid] = piv[tid];
+        const T pivot = piv[0];
+        if (has && pos == p) pos = c;
+        else if (has && pos == c) pos = p;
         if (g == 0 && tid == 0) pivots[c0 + c] = c0 + p;
 
         // Rows below the pivot: divide (a quotient for a subnormal pivot, nothing for a zero one), record the
         // multiplier, update the rest of the row and shift it one column left.
         if (has && pos > c) {
-            const bool zero = pivot == T(0);
+            // Every item takes fma(l, -u, a). A zero pivot (rare; the same for all threads) leaves l undivided and
+            // skips columns c + j, j < skip <= 16 (a leaf's width): only the first sixteen items have a second,
+            // guarded path, so that the common path has no per-item condition.
             T l = v[0];
-            if (!zero) l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
-            scratch[(int64_t)c * rows + phys] = l;
-            const int skip = zero ? leaf_end[c0 + c] - (int)c0 - c : 1;  // columns c + j, j < skip, not updated
+            const bool zero = pivot == T(0);
+            if (!zero) {
+                l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
 #pragma unroll
-            for (int j = 1; j < PW; ++j) {
-                const T nv = __builtin_fma(l, -s_piv[j], v[j]);
-                v[j - 1] = j >= skip ? nv : v[j];
+                for (int j = 1; j < 16; ++j) v[j - 1] = __builtin_fma(l, -piv[j], v[j]);
+            } else {
+                const int skip = leaf_end[c0 + c] - (int)c0 - c;
+#pragma unroll
+                for (int j = 1; j < 16; ++j) v[j - 1] = j >= skip ? __builtin_fma(l, -piv[j], v[j]) : v[j];
+            }
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int j0 = 16; j0 < PW; j0 += 16) {
+#pragma unroll
+                for (int j = j0; j < j0 + 16 && j < PW; ++j) v[j - 1] = __builtin_fma(l, -piv[j], v[j]);
+                asm volatile("" ::: "memory");  // sixteen items of the pivot row in registers at a time
             }
             v[PW - 1] = T(0);
+            scratch[(int64_t)c * rows + phys] = l;
         }
         HS_PHASE(4);
     }
     if (timed)
         for (int q = 0; q < 8; ++q) atomicAdd((unsigned long long*)&prof[q], (unsigned long long)tp[q]);
 #undef HS_PHASE
+    __syncthreads();  // every row of U is in s_u11
 
     if (has) {
+        // The row at its final position: multipliers from scratch left of it, its row of U (pivot rows) from LDS.
         T* dst = a + (c0 + pos) * lda + c0;
-        for (int j = 0; j < w; ++j) dst[j] = scratch[(int64_t)j * rows + phys];
+        for (int j = 0; j < w; ++j) dst[j] = j < pos ? scratch[(int64_t)j * rows + phys] : s_u11[pos][j - pos];
         if (pos != phys) {
             const long long k = atomicAdd((unsigned long long*)moves, 1ull);
             moves[1 + 2 * k] = c0 + pos;

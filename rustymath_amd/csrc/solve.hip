@@ -13,6 +13,7 @@
 // the solved rows of other blocks in the order above, one fused multiply-add each, as soon as their
 // block publishes them; then the block substitutes within itself and publishes its rows.
 #include "hs_kernels.h"
+#include <atomic>
 #include <vector>
 
 namespace hs {
@@ -187,6 +188,157 @@ void backward_leaves(int r0, int r1, std::vector<std::pair<int, int>>& out) {
     out.push_back({r0, r1});
 }
 
+// One right-hand side (the refinement's corrections, single-column lu_solve): one wave per 32-row block, in
+// solve order. Lane t holds row r0 + t. A solved row is published as 8-byte words that carry the call's
+// epoch beside the value (two words for FP64), stored and loaded with device-coherent (sc1) atomics, so one
+// load both tells that a row is ready and brings its value; the tile of matrix items a block needs from the
+// next producer is loaded before the wait. A row adds the producers' rows in the reference's order, in
+// increasing (L) or decreasing (U, `single`) column order, and the block then substitutes within itself
+// with register broadcasts: a quotient by the diagonal at each row's turn for U.
+template <typename T>
+__device__ __forceinline__ void publish1(unsigned long long* pub, int64_t row, T v, unsigned long long epoch) {
+    if constexpr (sizeof(T) == 4) {
+        const unsigned long long w = (epoch << 32) | __builtin_bit_cast(unsigned, v);
+        __hip_atomic_store(&pub[row], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        __hip_atomic_store(&pub[2 * row], (epoch << 32) | (u & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&pub[2 * row + 1], (epoch << 32) | (u >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+template <typename T>
+__device__ __forceinline__ bool fetch1(unsigned long long* pub, int64_t row, unsigned long long epoch, T& v) {
+    if constexpr (sizeof(T) == 4) {
+        const unsigned long long w = __hip_atomic_load(&pub[row], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = __builtin_bit_cast(T, (unsigned)w);
+        return (w >> 32) == epoch;
+    } else {
+        const unsigned long long w0 = __hip_atomic_load(&pub[2 * row], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long w1 = __hip_atomic_load(&pub[2 * row + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = __builtin_bit_cast(T, (w1 << 32) | (w0 & 0xffffffffull));
+        return (w0 >> 32) == epoch && (w1 >> 32) == epoch;
+    }
+}
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int l) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    }
+}
+
+constexpr int B1 = 32;
+
+template <typename T>
+__global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T* __restrict__ f, int64_t ld, T* x,
+                                                   unsigned long long* pub, unsigned long long epoch, int* status) {
+    const int nb = (int)((n + B1 - 1) / B1), b = blockIdx.x, lane = threadIdx.x;
+    const bool fwd = mode == FWD_UNIT;
+    const int q = fwd ? b : nb - 1 - b;  // this block's rows [q B1, ...)
+    const int64_t r0 = (int64_t)q * B1;
+    const int h = (int)(n - r0 < B1 ? n - r0 : B1);
+    const bool act = lane < h;
+    const int64_t i = r0 + (act ? lane : 0);
+    T acc = act ? x[i] : T(0);
+    // Tile of producer block pq: e[j] = F(i, pq B1 + j) (column-major factors), padded with zeros.
+    auto load_tile = [&](int pq, T* e) {
+        const int64_t p0 = (int64_t)pq * B1;
+#pragma unroll
+        for (int j = 0; j < B1; ++j) {
+            const int64_t pc = p0 + j < n ? p0 + j : n - 1;
+            const T v = f[pc * ld + i];
+            e[j] = p0 + j < n ? v : T(0);
+        }
+    };
+    const int np = fwd ? q : nb - 1 - q;
+    T e[B1], en[B1], d[B1];
+    load_tile(q, d);  // the diagonal tile d[j] = F(i, r0 + j), off the critical path
+    if (np > 0) load_tile(fwd ? 0 : nb - 1, e);
+    for (int t = 0; t < np; ++t) {
+        const int pq = fwd ? t : nb - 1 - t;
+        if (t + 1 < np) load_tile(fwd ? t + 1 : nb - 2 - t, en);
+        const int64_t p0 = (int64_t)pq * B1;
+        const int ph = (int)(n - p0 < B1 ? n - p0 : B1);
+        T xv = T(0);
+        bool ok = lane >= ph;
+        uint64_t spins = 0;
+        while (true) {
+            if (!ok) ok = fetch1(pub, p0 + lane, epoch, xv);
+            if (__all(ok)) break;
+            __builtin_amdgcn_s_sleep(2);  // many blocks wait on the same producer: poll its words gently
+            if (++spins > SPIN_LIMIT) { if (lane == 0) atomicExch(status, ERR_LIMIT); return; }
+        }
+        if (fwd) {
+#pragma unroll
+            for (int j = 0; j < B1; ++j)
+                if (j < ph) acc = __builtin_fma(lane_value(xv, j), -e[j], acc);
+        } else {
+#pragma unroll
+            for (int j = B1 - 1; j >= 0; --j)
+                if (j < ph) acc = __builtin_fma(lane_value(xv, j), -e[j], acc);
+        }
+#pragma unroll
+        for (int j = 0; j < B1; ++j) e[j] = en[j];
+    }
+    // Within the block.
+    if (fwd) {
+#pragma unroll
+        for (int j = 0; j < B1; ++j) {
+            if (j < h) {
+                const T xj = lane_value(acc, j);
+                if (lane > j) acc = __builtin_fma(xj, -d[j], acc);
+            }
+        }
+    } else {
+        T dg = T(1);
+#pragma unroll
+        for (int j = 0; j < B1; ++j) if (j == lane) dg = d[j];
+#pragma unroll
+        for (int j = B1 - 1; j >= 0; --j) {
+            if (j < h) {
+                if (lane == j) acc = acc / dg;
+                const T xj = lane_value(acc, j);
+                if (lane < j) acc = __builtin_fma(xj, -d[j], acc);
+            }
+        }
+    }
+    if (act) {
+        publish1(pub, i, acc, epoch);
+        x[i] = acc;
+    }
+}
+
+// The epoch of each single-column pass: increasing across calls, so that words left by an earlier pass never
+// match (the published words are also cleared before each pass).
+std::atomic<unsigned long long> g_epoch1{1};
+
+// L then U (lu_solve's order for one column) in place on x (n items, contiguous), with one status check.
+template <typename T>
+int trsv1_lu(int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
+    const int nb = (int)((n + B1 - 1) / B1);
+    const size_t words = (size_t)n * (sizeof(T) == 8 ? 2 : 1);
+    char* buf = (char*)pool_get(words * 8 + 64);
+    if (!buf) return hip_status(hipErrorOutOfMemory, "pool_get");
+    unsigned long long* pub = (unsigned long long*)buf;
+    int* status = (int*)(buf + words * 8);
+    HS_HIP(hipMemsetAsync(buf, 0, words * 8 + sizeof(int), s));
+    const unsigned long long e1 = (g_epoch1.fetch_add(2) % 0xfffffff0ull) | 1ull;  // odd, and e1 + 1 < 2^32
+    trsv1_kernel<T><<<nb, 64, 0, s>>>(FWD_UNIT, n, f, ld, x, pub, e1, status);
+    HS_HIP(hipGetLastError());
+    trsv1_kernel<T><<<nb, 64, 0, s>>>(BWD_SINGLE, n, f, ld, x, pub, e1 + 1, status);
+    HS_HIP(hipGetLastError());
+    int st = 0;
+    HS_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HS_HIP(hipStreamSynchronize(s));
+    pool_put(buf);
+    if (st) { set_error("triangular solve wait ran out"); return ERR_LIMIT; }
+    return OK;
+}
+
 // One triangular pass of `mode` over all n rows of x (n×k, k <= MAXK) in place.
 template <typename T>
 int trsv_pass(int mode, int64_t n, int k, const T* f, int64_t ld, T* x, int64_t ldx, hipStream_t s) {
@@ -266,7 +418,8 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
     if (!trans) {
         gather_rows<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, (int)k, b, ldb, perm, w, k);
         HS_HIP(hipGetLastError());
-        for (int64_t c0 = 0; c0 < k && rc == OK; c0 += MAXK) {
+        if (k == 1) rc = trsv1_lu<T>(n, lu, lda, w, s);
+        for (int64_t c0 = 0; c0 < k && rc == OK && k > 1; c0 += MAXK) {
             int kc = (int)std::min<int64_t>(MAXK, k - c0);
             rc = trsv_pass<T>(FWD_UNIT, n, kc, lu, lda, w + c0, k, s);
             if (rc == OK) rc = trsv_pass<T>(k == 1 ? BWD_SINGLE : BWD_LEAVES, n, kc, lu, lda, w + c0, k, s);

@@ -65,85 +65,144 @@ __device__ __forceinline__ bool tile_of(int w, int tiles_m, int tiles_n, int& tm
     return true;
 }
 
-template <typename T, bool TA, bool TB>
-__global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, int64_t k, T alpha,
-                                                         const T* __restrict__ A, int64_t lda,
-                                                         const T* __restrict__ B, int64_t ldb, T beta,
-                                                         T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
-    constexpr int P = Pad<T>::v;
-    __shared__ T As[2][BK][BM + P];
-    __shared__ T Bs[2][BK][BN + P];
+// One 128x128 tile of c = alpha op(a) op(b) + beta c. EDGE selects the path for tiles that cross the last row or
+// column: there every address is clamped into the matrix and the value replaced afterwards, so no load sits behind a
+// per-lane branch (a guarded load makes the compiler wait for each load before the next). Interior tiles load
+// without checks; a k tail (the last step when BK does not divide k) takes the clamped loads in either path.
+template <typename T, bool TA, bool TB, bool EDGE>
+__device__ __forceinline__ void gemm_tile(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A, int64_t lda,
+                                          const T* __restrict__ B, int64_t ldb, T beta, T* __restrict__ C, int64_t ldc,
+                                          int64_t m0, int64_t n0, T (*As)[BK][BM + Pad<T>::v], T (*Bs)[BK][BN + Pad<T>::v]) {
     typedef typename Mfma<T>::acc Acc;
-
-    int tm, tn;
-    if (!tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn)) return;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     const bool scale_a = alpha != T(1);
 
     // Accumulators start at 0, or at beta c (c itself where beta is 1), as `native_gemm` scales c first.
     Acc acc[4][4];
+    if (beta != T(0)) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
-                T v = T(0);
-                if (beta != T(0) && r < m && col < n) {
-                    v = C[r * ldc + col];
-                    if (beta != T(1)) v = beta * v;
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
+                    T v;
+                    if (EDGE) {
+                        const int64_t rr = r < m ? r : m - 1, cc = col < n ? col : n - 1;
+                        v = C[rr * ldc + cc];
+                    } else {
+                        v = C[r * ldc + col];
+                    }
+                    acc[i][j][e] = beta != T(1) ? beta * v : v;
                 }
-                acc[i][j][e] = v;
-            }
-
-    // Global loads: each thread brings 8 items of A's tile and 8 of B's per step of BK.
-    T ra[8], rb[8];
-    auto load = [&](int64_t p0) {
-        // alpha scales only the items that exist, so that padding stays +0 whatever alpha's sign.
-        if (!TA) {  // A row-major m×k: thread -> row t/2, steps (t%2)*8 ..+8
-            int64_t r = m0 + (t >> 1), p = p0 + (t & 1) * 8;
+    } else {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                T v = (r < m && p + q < k) ? A[r * lda + p + q] : T(0);
-                ra[q] = (scale_a && r < m && p + q < k) ? alpha * v : v;
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
+    }
+
+    // Global loads: each thread brings 8 items of A's tile and 8 of B's per step of BK. Items outside the matrix are
+    // +0 on A's side and, past k, -0 on B's side, so that padding terms add (+0)(-0) = -0, which changes no sum. alpha
+    // scales only the items that exist (bits of amask), when they are stored to LDS: the loads of the next step stay in
+    // flight while the MFMAs of this one run.
+    T ra[8], rb[8];
+    unsigned amask = 0xffu;
+    auto load = [&](int64_t p0) {
+        const bool checked = EDGE || p0 + BK > k;
+        if (!TA) {  // A row-major m×k: thread -> row t/2, steps (t%2)*8 ..+8
+            const int64_t r = m0 + (t >> 1), p = p0 + (t & 1) * 8;
+            if (!checked) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) ra[q] = A[r * lda + p + q];
+            } else {
+                const int64_t rr = r < m ? r : m - 1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int64_t pp = p + q < k ? p + q : k - 1;
+                    const T v = A[rr * lda + pp];
+                    ra[q] = (r < m && p + q < k) ? v : T(0);
+                }
+            }
+            amask = 0xffu;
+            if (checked) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) amask &= (r < m && p + q < k) ? 0xffu : ~(1u << q);
             }
         } else {  // A stored k×m: thread -> step t/16, rows (t%16)*8 ..+8
-            int64_t p = p0 + (t >> 4), r = m0 + (t & 15) * 8;
+            const int64_t p = p0 + (t >> 4), r = m0 + (t & 15) * 8;
+            if (!checked) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                T v = (p < k && r + q < m) ? A[p * lda + r + q] : T(0);
-                ra[q] = (scale_a && p < k && r + q < m) ? alpha * v : v;
+                for (int q = 0; q < 8; ++q) ra[q] = A[p * lda + r + q];
+            } else {
+                const int64_t pp = p < k ? p : k - 1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int64_t rr = r + q < m ? r + q : m - 1;
+                    const T v = A[pp * lda + rr];
+                    ra[q] = (p < k && r + q < m) ? v : T(0);
+                }
+            }
+            amask = 0xffu;
+            if (checked) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) amask &= (p < k && r + q < m) ? 0xffu : ~(1u << q);
             }
         }
         if (!TB) {  // B row-major k×n: thread -> step t/16, columns (t%16)*8 ..+8
-            int64_t p = p0 + (t >> 4), c = n0 + (t & 15) * 8;
+            const int64_t p = p0 + (t >> 4), c = n0 + (t & 15) * 8;
+            if (!checked) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) rb[q] = p < k ? ((c + q < n) ? B[p * ldb + c + q] : T(0)) : T(-0.0);
+                for (int q = 0; q < 8; ++q) rb[q] = B[p * ldb + c + q];
+            } else {
+                const int64_t pp = p < k ? p : k - 1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int64_t cc = c + q < n ? c + q : n - 1;
+                    const T v = B[pp * ldb + cc];
+                    rb[q] = p < k ? ((c + q < n) ? v : T(0)) : T(-0.0);
+                }
+            }
         } else {  // B stored n×k: thread -> column t/2, steps (t%2)*8 ..+8
-            int64_t c = n0 + (t >> 1), p = p0 + (t & 1) * 8;
+            const int64_t c = n0 + (t >> 1), p = p0 + (t & 1) * 8;
+            if (!checked) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) rb[q] = p + q < k ? ((c < n) ? B[c * ldb + p + q] : T(0)) : T(-0.0);
+                for (int q = 0; q < 8; ++q) rb[q] = B[c * ldb + p + q];
+            } else {
+                const int64_t cc = c < n ? c : n - 1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int64_t pp = p + q < k ? p + q : k - 1;
+                    const T v = B[cc * ldb + pp];
+                    rb[q] = p + q < k ? ((c < n) ? v : T(0)) : T(-0.0);
+                }
+            }
         }
     };
     auto stash = [&](int buf) {
+        if (scale_a) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) ra[q] = ((amask >> q) & 1u) ? alpha * ra[q] : ra[q];
+        }
         if (!TA) {
-            int r = t >> 1, p = (t & 1) * 8;
+            const int r = t >> 1, p = (t & 1) * 8;
 #pragma unroll
             for (int q = 0; q < 8; ++q) As[buf][p + q][r] = ra[q];
         } else {
-            int p = t >> 4, r = (t & 15) * 8;
+            const int p = t >> 4, r = (t & 15) * 8;
 #pragma unroll
             for (int q = 0; q < 8; ++q) As[buf][p][r + q] = ra[q];
         }
         if (!TB) {
-            int p = t >> 4, c = (t & 15) * 8;
+            const int p = t >> 4, c = (t & 15) * 8;
 #pragma unroll
             for (int q = 0; q < 8; ++q) Bs[buf][p][c + q] = rb[q];
         } else {
-            int c = t >> 1, p = (t & 1) * 8;
+            const int c = t >> 1, p = (t & 1) * 8;
 #pragma unroll
             for (int q = 0; q < 8; ++q) Bs[buf][p + q][c] = rb[q];
         }
@@ -179,9 +238,26 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, in
         for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
-                if (r < m && col < n) C[r * ldc + col] = acc[i][j][e];
+                const int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
+                if (!EDGE || (r < m && col < n)) C[r * ldc + col] = acc[i][j][e];
             }
+}
+
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, int64_t k, T alpha,
+                                                         const T* __restrict__ A, int64_t lda,
+                                                         const T* __restrict__ B, int64_t ldb, T beta,
+                                                         T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
+    constexpr int P = Pad<T>::v;
+    __shared__ T As[2][BK][BM + P];
+    __shared__ T Bs[2][BK][BN + P];
+    int tm, tn;
+    if (!tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn)) return;
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    if (m0 + BM <= m && n0 + BN <= n)
+        gemm_tile<T, TA, TB, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
+    else
+        gemm_tile<T, TA, TB, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
 }
 
 // `scale` (core/src/linalg.rs:1000-1008): c = beta c, zeros for beta = 0.

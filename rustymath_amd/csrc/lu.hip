@@ -118,12 +118,14 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     int pos = (int)phys;
     T v[PW];
     {
+        // All loads first, from addresses clamped into the panel, then the selects: a load behind a per-lane
+        // condition would be issued and waited for one at a time.
         const T* src = a + (c0 + (has ? phys : 0)) * lda + c0;
 #pragma unroll
-        for (int j = 0; j < PW; ++j) {
-            const T x = src[j < w ? j : 0];
-            v[j] = (has && j < w) ? x : T(0);
-        }
+        for (int j = 0; j < PW; ++j) v[j] = src[j < w ? j : 0];
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int j = 0; j < PW; ++j) v[j] = (has && j < w) ? v[j] : T(0);
     }
 
     // Double-buffered by column, so that a workgroup alone needs one barrier per column: each wave's best
@@ -272,8 +274,15 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
 
     if (has) {
         // The row at its final position: multipliers from scratch left of it, its row of U (pivot rows) from LDS.
+        // The scratch items are all loaded before any is used (see the first loads).
         T* dst = a + (c0 + pos) * lda + c0;
-        for (int j = 0; j < w; ++j) dst[j] = j < pos ? scratch[(int64_t)j * rows + phys] : s_u11[pos][j - pos];
+#pragma unroll
+        for (int j = 0; j < PW; ++j) v[j] = scratch[(int64_t)(j < w ? j : 0) * rows + phys];
+        asm volatile("" ::: "memory");
+        const int upos = pos < PW ? pos : PW - 1;
+#pragma unroll
+        for (int j = 0; j < PW; ++j)
+            if (j < w) dst[j] = j < pos ? v[j] : s_u11[upos][(j - pos) & (PW - 1)];
         if (pos != phys) {
             const long long k = atomicAdd((unsigned long long*)moves, 1ull);
             moves[1 + 2 * k] = c0 + pos;
@@ -293,14 +302,25 @@ __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t ld
     for (int q = threadIdx.x; q < 1 + 4 * PW; q += blockDim.x) mv[q] = moves[q];
     __syncthreads();
     const int nm = (int)mv[0];
+    if (nm == 0) return;
     const int tid = threadIdx.x, col_l = tid & 63, grp = tid >> 6;
     int64_t col = (int64_t)blockIdx.x * 64 + col_l;  // over the columns outside [c0, c1)
     if (col >= c0) col += c1 - c0;
     const bool live = col < n;
+    const int64_t lc = live ? col : 0;
+    // All sources in flight at once: past the last move, a load repeats the first source (addresses stay inside
+    // the matrix and no load waits behind a per-lane condition).
+    T vals[2 * PW / 4];
 #pragma unroll
-    for (int q = 0; q < 2 * PW / 4; ++q) {  // all sources in flight at once
+    for (int q = 0; q < 2 * PW / 4; ++q) {
         const int k = grp + 4 * q;
-        if (k < nm && live) tmp[k][col_l] = a[mv[2 + 2 * k] * lda + col];
+        vals[q] = a[mv[2 + 2 * (k < nm ? k : 0)] * lda + lc];
+    }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int q = 0; q < 2 * PW / 4; ++q) {
+        const int k = grp + 4 * q;
+        if (k < nm) tmp[k][col_l] = vals[q];
     }
     __syncthreads();
 #pragma unroll
@@ -320,13 +340,24 @@ __global__ __launch_bounds__(TC) void trsm_block_kernel(T* __restrict__ a, int64
     __shared__ T X[PW][TC + 1];
     const int t = threadIdx.x;
     const int64_t col0 = c0 + w + (int64_t)blockIdx.x * TC;
+    // All loads in flight at once, from addresses clamped into the matrix; zeros then pad a narrow panel.
+    T lv[PW * PW / TC], xv[PW];
+    const int64_t col = col0 + t < n ? col0 + t : n - 1;
 #pragma unroll
-    for (int q = 0; q < PW * PW / TC; ++q) {  // all loads in flight at once; zeros pad a narrow panel
+    for (int q = 0; q < PW * PW / TC; ++q) {
         const int idx = t + q * TC, r = idx / PW, c = idx % PW;
-        L[r][c] = (r < w && c < w) ? a[(c0 + r) * lda + c0 + c] : T(0);
+        lv[q] = a[(c0 + (r < w ? r : w - 1)) * lda + c0 + (c < w ? c : w - 1)];
     }
 #pragma unroll
-    for (int r = 0; r < PW; ++r) X[r][t] = (r < w && col0 + t < n) ? a[(c0 + r) * lda + col0 + t] : T(0);
+    for (int r = 0; r < PW; ++r) xv[r] = a[(c0 + (r < w ? r : w - 1)) * lda + col];
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int q = 0; q < PW * PW / TC; ++q) {
+        const int idx = t + q * TC, r = idx / PW, c = idx % PW;
+        L[r][c] = (r < w && c < w) ? lv[q] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < PW; ++r) X[r][t] = (r < w && col0 + t < n) ? xv[r] : T(0);
     __syncthreads();
     T x[PW];
 #pragma unroll

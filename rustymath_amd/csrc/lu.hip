@@ -30,48 +30,106 @@ template <typename T> struct Bits;
 template <> struct Bits<double> { typedef unsigned long long u; };
 template <> struct Bits<float> { typedef unsigned int u; };
 
+// A candidate's key: larger magnitude first, then the smaller position, NaN never chosen over a number:
+// `largest`'s first maximum (core/src/linalg/factor.rs:476). kv is the bit pattern of |x| plus one (the order of
+// non-negative IEEE values is the order of their bit patterns), 0 for NaN; a row that is no candidate has kv = 0
+// and position INT_MAX, so an all-NaN column picks its first row, as `largest` does.
 template <typename T>
-__device__ __forceinline__ void put(T* p, T v) {
-    typedef typename Bits<T>::u U;
-    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned long long mag_key(T x) {
+    if constexpr (sizeof(T) == 8) {
+        const double ax = fabs(x);
+        return ax != ax ? 0ull : __builtin_bit_cast(unsigned long long, ax) + 1ull;
+    } else {
+        const float ax = fabsf(x);
+        return ax != ax ? 0ull : (unsigned long long)__builtin_bit_cast(unsigned, ax) + 1ull;
+    }
 }
-template <typename T>
-__device__ __forceinline__ T get(const T* p) {
-    typedef typename Bits<T>::u U;
-    U u = __hip_atomic_load(reinterpret_cast<U*>(const_cast<T*>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(T, u);
+__device__ __forceinline__ bool better(unsigned long long kv, int kp, unsigned long long kv2, int kp2) {
+    return (kv2 > kv) | ((kv2 == kv) & (kp2 < kp));
 }
-__device__ __forceinline__ void put64(long long* p, long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ long long get64(const long long* p) {
-    return __hip_atomic_load(const_cast<long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// One workgroup's published candidate for one column: the magnitude (-1 for none or NaN), its row, the
-// candidate row's items, and (from the workgroup holding it) the current row at the pivot position.
-template <typename T> struct Slot {
-    T val;
-    long long row;
-    T cand[PW];
-};
-
-// Prefers the larger magnitude, then the smaller position: `largest`'s first maximum (factor.rs:476). Selects,
-// not branches, so that a reduction step costs a few instructions.
-template <typename T>
-__device__ __forceinline__ bool better(T v, int r, T v2, int r2) {
-    return (v2 > v) | ((v2 == v) & (r2 < r));
-}
-
-template <typename T>
-__device__ __forceinline__ void wave_best(T& v, int& r) {
+__device__ __forceinline__ void wave_best(unsigned long long& kv, int& kp, int& who) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        const T v2 = __shfl_xor(v, off);
-        const int r2 = __shfl_xor(r, off);
-        const bool take = better(v, r, v2, r2);
-        v = take ? v2 : v;
-        r = take ? r2 : r;
+        const unsigned long long v2 = __shfl_xor(kv, off);
+        const int p2 = __shfl_xor(kp, off), w2 = __shfl_xor(who, off);
+        const bool take = better(kv, kp, v2, p2);
+        kv = take ? v2 : kv;
+        kp = take ? p2 : kp;
+        who = take ? w2 : who;
+    }
+}
+
+// The exchange between the workgroups of one panel. Every word a workgroup publishes carries a tag (the panel's
+// launch number within the factorization times 64, plus the column, plus one) beside its payload, so that one
+// device-coherent load both tells that a word is current and brings its value: no flag, no fence, no wait for
+// store completion. Per column parity and workgroup: the key (FP32: one word, tag << 46 | kv << 15 | 32767 - pos;
+// FP64: three words, tag << 32 with the high and low halves of kv and with 32767 - pos) and the candidate row
+// (one word per FP32 item, tag << 32 | bits; two per FP64 item, tag << 32 | half). The buffer is cleared when a
+// factorization starts; tags within it never repeat.
+constexpr int KEY_WORDS = 4;
+constexpr int ROW_WORDS = 2 * PW;
+struct Exchange {
+    unsigned long long key[2][MAX_WG][KEY_WORDS];
+    unsigned long long row[2][MAX_WG][ROW_WORDS];
+};
+
+__device__ __forceinline__ void xput(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long xget(const unsigned long long* p) {
+    return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ void put_key(unsigned long long* k, unsigned tag, unsigned long long kv, int kp) {
+    const unsigned long long pp = (unsigned long long)(32767 - (kp < 32767 ? kp : 32767));
+    if constexpr (sizeof(T) == 4) {
+        xput(&k[0], ((unsigned long long)tag << 46) | (kv << 15) | pp);
+    } else {
+        const unsigned long long t = (unsigned long long)tag << 32;
+        xput(&k[0], t | (kv >> 32));
+        xput(&k[1], t | (kv & 0xffffffffull));
+        xput(&k[2], t | pp);
+    }
+}
+// Reads a key; false while any of its words is older than `tag`.
+template <typename T>
+__device__ __forceinline__ bool get_key(const unsigned long long* k, unsigned tag, unsigned long long& kv, int& kp) {
+    if constexpr (sizeof(T) == 4) {
+        const unsigned long long w = xget(&k[0]);
+        kv = (w >> 15) & 0x7fffffffffull;
+        const int pp = (int)(w & 0x7fff);
+        kp = pp == 0 ? 0x7fffffff : 32767 - pp;
+        return (unsigned)(w >> 46) == tag;
+    } else {
+        const unsigned long long w0 = xget(&k[0]), w1 = xget(&k[1]), w2 = xget(&k[2]);
+        kv = ((w0 & 0xffffffffull) << 32) | (w1 & 0xffffffffull);
+        const int pp = (int)(w2 & 0x7fff);
+        kp = pp == 0 ? 0x7fffffff : 32767 - pp;
+        return (unsigned)(w0 >> 32) == tag && (unsigned)(w1 >> 32) == tag && (unsigned)(w2 >> 32) == tag;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void put_item(unsigned long long* r, int j, unsigned tag, T v) {
+    const unsigned long long t = (unsigned long long)tag << 32;
+    if constexpr (sizeof(T) == 4) {
+        xput(&r[j], t | __builtin_bit_cast(unsigned, v));
+    } else {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        xput(&r[2 * j], t | (u & 0xffffffffull));
+        xput(&r[2 * j + 1], t | (u >> 32));
+    }
+}
+template <typename T>
+__device__ __forceinline__ bool get_item(const unsigned long long* r, int j, unsigned tag, T& v) {
+    if constexpr (sizeof(T) == 4) {
+        const unsigned long long w = xget(&r[j]);
+        v = __builtin_bit_cast(T, (unsigned)w);
+        return (unsigned)(w >> 32) == tag;
+    } else {
+        const unsigned long long w0 = xget(&r[2 * j]), w1 = xget(&r[2 * j + 1]);
+        v = __builtin_bit_cast(T, (w1 << 32) | (w0 & 0xffffffffull));
+        return (unsigned)(w0 >> 32) == tag && (unsigned)(w1 >> 32) == tag;
     }
 }
 
@@ -86,12 +144,15 @@ __device__ __forceinline__ long long ticks() { return __builtin_amdgcn_s_memreal
 // `scratch[column][row]` and each pivot row's U items to LDS (every workgroup sees every pivot row), from
 // which each row is written once, at its final position, when the panel is done.
 //
-// With one workgroup, a column's pivot takes workgroup barriers only. With several (launch indices 0, 8,
-// 16, ..., one XCD under round-robin dispatch, one workgroup per CU, so all are resident; the other launch
-// indices return at once), every column takes one exchange: each workgroup publishes its best candidate
-// (magnitude, position, the candidate row) with device-coherent (sc1) stores, raises its flag, waits for
-// every other flag, and all pick the same pivot. Candidates compare by magnitude, NaN never chosen, then by
-// position: `largest`'s first maximum (core/src/linalg/factor.rs:476).
+// A column's critical path is short: once the pivot row of column c is known, each row computes its multiplier
+// and only its next item x0 (column c + 1); its pivot candidacy for column c + 1 goes out at once, and the
+// rest of the row's update for column c runs while the candidates are compared. Per column: a wave reduction,
+// a barrier, the workgroups' best keys compared, the winner's row fetched, a second barrier.
+//
+// With one workgroup, the winner's row goes through LDS. With several (launch indices 0, 8, 16, ..., one XCD
+// under round-robin dispatch, one workgroup per CU, so all are resident; the other launch indices return at
+// once), each workgroup publishes its best key and, from the thread holding it, its row as tagged words
+// (Exchange); wave 0 of every workgroup reads all keys, picks the winner and fetches its row.
 //
 // Writes pivots[c0 + c] (absolute rows) and `moves` (count, then pairs dst, src: the rows whose position
 // changed), for the row-swap kernel. `leaf_end[c]` is the end of the reference leaf of column c; a zero
@@ -101,14 +162,12 @@ template <typename T>
 __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int64_t rows, int w,
                                                    int64_t* __restrict__ pivots, long long* __restrict__ moves,
                                                    const int* __restrict__ leaf_end, T* __restrict__ scratch,
-                                                   char* state, unsigned long long epoch, int nwg, int* status,
+                                                   Exchange* __restrict__ ex, unsigned tag0, int nwg, int* status,
                                                    long long* prof) {
     if (nwg > 1 && (blockIdx.x & 7)) return;
     const int g = nwg > 1 ? (int)(blockIdx.x >> 3) : 0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int WAVES = PT / 64;
-    unsigned long long* flags = reinterpret_cast<unsigned long long*>(state);
-    Slot<T>* slots = reinterpret_cast<Slot<T>*>(state + MAX_WG * sizeof(unsigned long long));
     const bool timed = prof != nullptr && g == 0 && threadIdx.x == 0;
     long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tl = timed ? ticks() : 0;
 #define HS_PHASE(q) do { if (timed) { long long t_ = ticks(); tp[q] += t_ - tl; tl = t_; } } while (0)
@@ -116,120 +175,143 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     const int64_t phys = (int64_t)g * PT + tid;  // this thread's row, relative to c0
     const bool has = phys < rows;
     int pos = (int)phys;
+    // The row's items, every load issued before any is used, from addresses clamped into the panel (a load behind a
+    // per-lane condition would be issued and waited for one at a time). Items past the panel's width only ever meet
+    // other such items (an item in column j takes the pivot row's item in column j), are never written, and so may
+    // hold any value.
     T v[PW];
     {
-        // All loads first, from addresses clamped into the panel, then the selects: a load behind a per-lane
-        // condition would be issued and waited for one at a time.
         const T* src = a + (c0 + (has ? phys : 0)) * lda + c0;
 #pragma unroll
-        for (int j = 0; j < PW; ++j) v[j] = src[j < w ? j : 0];
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int j = 0; j < PW; ++j) v[j] = (has && j < w) ? v[j] : T(0);
+        for (int j = 0; j < PW; ++j) v[j] = src[j < w ? j : w - 1];
     }
 
-    // Double-buffered by column, so that a workgroup alone needs one barrier per column: each wave's best
-    // candidate (magnitude, position, row), the pivot rows of U, and the pivot row of the current column.
-    __shared__ T s_val[2][WAVES];
-    __shared__ int s_key[2][WAVES];
-    __shared__ T s_cand[2][WAVES][PW];
+    __shared__ unsigned long long s_kv[2][WAVES];
+    __shared__ int s_kp[2][WAVES];
+    __shared__ T s_piv[2][PW];   // the pivot rows of the last two columns
+    __shared__ T s_hand[WAVES][PW + 2];
     __shared__ T s_u11[PW][PW];
-    __shared__ T s_piv[PW];
-    __shared__ int s_p, s_gp, s_abort;
-    if (tid == 0) { s_abort = 0; if (g == 0) moves[0] = 0; }
+    __shared__ int s_p[2], s_abort;
+    if (tid == 0) s_abort = 0;
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+
+    // x0: this row's item in the current column, fully updated; the rest of v may still owe the previous column's
+    // update (pending), with multiplier l_prev, the pivot row in s_piv[buf ^ 1] and the zero-pivot skip.
+    T x0 = v[0], l_prev = T(0);
+    bool pending = false;
+    int skip_prev = 0;
 
 #pragma clang loop unroll(disable)
     for (int c = 0; c < w; ++c) {
         const int buf = c & 1;
-        // This wave's candidate, published with its row.
-        T val = T(-1);
-        int key = 0x7fffffff;
-        if (has && pos >= c) {
-            const T x = fabs(v[0]);
-            val = x != x ? T(-1) : x;
-            key = pos;
-        }
-        T bv = val;
-        int bk = key;
-        wave_best(bv, bk);
-        if (bk != 0x7fffffff && key == bk) {
-#pragma unroll
-            for (int j = 0; j < PW; ++j) s_cand[buf][wave][j] = v[j];
-        }
-        if (lane == 0) { s_val[buf][wave] = bv; s_key[buf][wave] = bk; }
+        const unsigned tag = tag0 + (unsigned)c;
+        // This row's candidacy for column c and the wave's best.
+        const bool cand = has && pos >= c;
+        unsigned long long kv = cand ? mag_key(x0) : 0ull;
+        int kp = cand ? pos : 0x7fffffff, who = wave;
+        wave_best(kv, kp, who);
+        if (lane == 0) { s_kv[buf][wave] = kv; s_kp[buf][wave] = kp; }
         __syncthreads();
         HS_PHASE(0);
-        T wv[WAVES];
-        int wk[WAVES];
-#pragma unroll
-        for (int q = 0; q < WAVES; ++q) { wv[q] = s_val[buf][q]; wk[q] = s_key[buf][q]; }
-        T wb = wv[0];
-        int wbk = wk[0], win = 0;
+        unsigned long long bkv = s_kv[buf][0];
+        int bkp = s_kp[buf][0];
 #pragma unroll
         for (int q = 1; q < WAVES; ++q) {
-            const bool take = better(wb, wbk, wv[q], wk[q]);
-            wb = take ? wv[q] : wb;
-            wbk = take ? wk[q] : wbk;
-            win = take ? q : win;
+            const unsigned long long v2 = s_kv[buf][q];
+            const int p2 = s_kp[buf][q];
+            const bool take = better(bkv, bkp, v2, p2);
+            bkv = take ? v2 : bkv;
+            bkp = take ? p2 : bkp;
         }
-        const T* cand = s_cand[buf][win];
-        const T* piv = cand;
-        int p = wbk;
+        if (nwg > 1 && tid == 0) put_key<T>(ex->key[buf][g], tag, bkv, bkp);
+
+        // The workgroup's best row goes out before the rest of the update runs: the holding lane writes its items,
+        // still owing the previous column's update, to its wave's LDS row, and lane j applies that update to item j
+        // (one fused multiply-add, the same one the holder would apply) and publishes it.
+        {
+            const bool holder = has && pos == bkp;
+            if (__ballot(holder)) {
+                if (holder) {
+#pragma unroll
+                    for (int j = 0; j < PW; ++j) s_hand[wave][j] = v[j];
+                    s_hand[wave][PW] = x0;
+                    s_hand[wave][PW + 1] = l_prev;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const T* h = s_hand[wave];
+                T item;
+                if (c == 0) {
+                    item = h[lane];
+                } else {
+                    const int j = lane + 1 < PW ? lane + 1 : PW - 1;
+                    const T u = __builtin_fma(h[PW + 1], -s_piv[buf ^ 1][j], h[j]);
+                    item = lane == 0 ? h[PW] : (lane == PW - 1 ? T(0) : (j >= skip_prev ? u : h[j]));
+                }
+                if (nwg > 1) put_item<T>(ex->row[buf][g], lane, tag, item);
+                else s_piv[buf][lane] = item;
+            }
+        }
+
+        // The previous column's update of the rest of the row.
+        if (pending) {
+            const T* pp = s_piv[buf ^ 1];
+            if (skip_prev <= 2) {
+#pragma unroll
+                for (int j0 = 2; j0 < PW; j0 += 16) {
+#pragma unroll
+                    for (int j = j0; j < j0 + 16 && j < PW; ++j) v[j - 1] = __builtin_fma(l_prev, -pp[j], v[j]);
+                    asm volatile("" ::: "memory");  // sixteen items of the pivot row in registers at a time
+                }
+            } else {  // a zero pivot: the rest of its leaf (at most 16 columns) keeps its items
+#pragma unroll
+                for (int j = 2; j < 16; ++j) v[j - 1] = j >= skip_prev ? __builtin_fma(l_prev, -pp[j], v[j]) : v[j];
+#pragma unroll
+                for (int j = 16; j < PW; ++j) v[j - 1] = __builtin_fma(l_prev, -pp[j], v[j]);
+            }
+            v[PW - 1] = T(0);
+            pending = false;
+        }
+        v[0] = x0;
         HS_PHASE(1);
 
         if (nwg > 1) {
-            Slot<T>* mine = slots + (c & 1) * MAX_WG + g;
-            if (tid < w - c) put(&mine->cand[tid], cand[tid]);
-            if (tid == 0) { put(&mine->val, wb); put64(&mine->row, wbk); }
-            // Every slot store of this workgroup reaches L2 before the flag can: the barrier alone does not wait
-            // for stores.
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
-            const unsigned long long target = epoch * 1024ull + (unsigned long long)c + 1ull;
-            if (tid == 0) __hip_atomic_store(&flags[g], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (wave == 0) {
-                T gv = T(-1);
-                int gk = 0x7fffffff, who = lane;
+                unsigned long long gkv = 0ull;
+                int gkp = 0x7fffffff, gw = lane;
                 bool ok = true;
                 if (lane < nwg) {
                     uint64_t spins = 0;
-                    while (__hip_atomic_load(&flags[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                    while (!get_key<T>(ex->key[buf][lane], tag, gkv, gkp)) {
                         if (++spins > SPIN_LIMIT) { ok = false; break; }
                     }
-                    if (ok) {
-                        const Slot<T>* sl = slots + (c & 1) * MAX_WG + lane;
-                        gv = get(&sl->val);
-                        gk = (int)get64(&sl->row);
+                }
+                bool all = __all(ok);
+                wave_best(gkv, gkp, gw);
+                T item = T(0);
+                if (all) {
+                    uint64_t spins = 0;
+                    while (!get_item<T>(ex->row[buf][gw], lane, tag, item)) {
+                        if (++spins > SPIN_LIMIT) { ok = false; break; }
                     }
+                    all = __all(ok);
                 }
-                const bool all = __all(ok);
-                // Reduce (magnitude, position) and remember which workgroup holds the winner.
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    const T v2 = __shfl_xor(gv, off);
-                    const int k2 = __shfl_xor(gk, off), w2 = __shfl_xor(who, off);
-                    const bool take = better(gv, gk, v2, k2);
-                    gv = take ? v2 : gv;
-                    gk = take ? k2 : gk;
-                    who = take ? w2 : who;
-                }
+                s_piv[buf][lane] = item;
                 if (lane == 0) {
-                    s_p = gk;
-                    s_gp = who;
+                    s_p[buf] = gkp;
                     if (!all) { s_abort = 1; atomicExch(status, ERR_LIMIT); }
                 }
             }
-            __syncthreads();
-            HS_PHASE(2);
-            if (s_abort) return;
-            p = s_p;
-            const int gp = s_gp;
-            if (tid < PW) s_piv[tid] = gp == g ? cand[tid] : (tid < w - c ? get(&slots[(c & 1) * MAX_WG + gp].cand[tid]) : T(0));
-            __syncthreads();
-            piv = s_piv;
-            HS_PHASE(3);
+        } else if (tid == 0) {
+            s_p[buf] = bkp;
         }
+        HS_PHASE(2);
+        __syncthreads();
+        HS_PHASE(3);
+        if (s_abort) return;
+        const int p = s_p[buf];
+        const T* piv = s_piv[buf];
 
         // The winner takes position c; its row of U is kept in LDS. The row at position c moves to p.
         if (tid < PW) s_u11[c][tid] = piv[tid];
@@ -238,32 +320,18 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         else if (has && pos == c) pos = p;
         if (g == 0 && tid == 0) pivots[c0 + c] = c0 + p;
 
-        // Rows below the pivot: divide (a quotient for a subnormal pivot, nothing for a zero one), record the
-        // multiplier, update the rest of the row and shift it one column left.
+        // Rows below the pivot: the multiplier (a quotient for a subnormal pivot, the item itself for a zero one,
+        // whose update then skips the rest of its leaf) and the next item; the rest of the row waits for the next
+        // column's candidates to be out.
         if (has && pos > c) {
-            // Every item takes fma(l, -u, a). A zero pivot (rare; the same for all threads) leaves l undivided and
-            // skips columns c + j, j < skip <= 16 (a leaf's width): only the first sixteen items have a second,
-            // guarded path, so that the common path has no per-item condition.
-            T l = v[0];
             const bool zero = pivot == T(0);
-            if (!zero) {
-                l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
-#pragma unroll
-                for (int j = 1; j < 16; ++j) v[j - 1] = __builtin_fma(l, -piv[j], v[j]);
-            } else {
-                const int skip = leaf_end[c0 + c] - (int)c0 - c;
-#pragma unroll
-                for (int j = 1; j < 16; ++j) v[j - 1] = j >= skip ? __builtin_fma(l, -piv[j], v[j]) : v[j];
-            }
-            asm volatile("" ::: "memory");
-#pragma unroll
-            for (int j0 = 16; j0 < PW; j0 += 16) {
-#pragma unroll
-                for (int j = j0; j < j0 + 16 && j < PW; ++j) v[j - 1] = __builtin_fma(l, -piv[j], v[j]);
-                asm volatile("" ::: "memory");  // sixteen items of the pivot row in registers at a time
-            }
-            v[PW - 1] = T(0);
+            const int skip = zero ? leaf_end[c0 + c] - (int)c0 - c : 0;
+            const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * (T(1) / pivot) : x0 / pivot);
+            x0 = 1 >= skip ? __builtin_fma(l, -piv[1], v[1]) : v[1];
             scratch[(int64_t)c * rows + phys] = l;
+            l_prev = l;
+            skip_prev = skip;
+            pending = true;
         }
         HS_PHASE(4);
     }
@@ -277,12 +345,15 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         // The scratch items are all loaded before any is used (see the first loads).
         T* dst = a + (c0 + pos) * lda + c0;
 #pragma unroll
-        for (int j = 0; j < PW; ++j) v[j] = scratch[(int64_t)(j < w ? j : 0) * rows + phys];
+        for (int j = 0; j < PW; ++j) v[j] = scratch[(int64_t)(j < w ? j : w - 1) * rows + phys];
         asm volatile("" ::: "memory");
         const int upos = pos < PW ? pos : PW - 1;
+        if (w == PW) {
 #pragma unroll
-        for (int j = 0; j < PW; ++j)
-            if (j < w) dst[j] = j < pos ? v[j] : s_u11[upos][(j - pos) & (PW - 1)];
+            for (int j = 0; j < PW; ++j) dst[j] = j < pos ? v[j] : s_u11[upos][(j - pos) & (PW - 1)];
+        } else {
+            for (int j = 0; j < w; ++j) dst[j] = j < pos ? scratch[(int64_t)j * rows + phys] : s_u11[upos][j - pos];
+        }
         if (pos != phys) {
             const long long k = atomicAdd((unsigned long long*)moves, 1ull);
             moves[1 + 2 * k] = c0 + pos;
@@ -330,16 +401,17 @@ __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t ld
     }
 }
 
-// U12 = L11⁻¹ A12 for rows [c0, c0 + w), columns [c0 + w, n): a workgroup takes 128 columns, stages
+// U12 = L11⁻¹ A12 for rows [c0, c0 + w), columns [cs, n): a workgroup takes 128 columns, stages
 // them and L11 through LDS with coalesced row reads, and each thread solves one column in registers,
 // each row taking its terms in order of the pivot column (factor.rs:668-689's leaf order).
 constexpr int TC = 128;
 template <typename T>
-__global__ __launch_bounds__(TC) void trsm_block_kernel(T* __restrict__ a, int64_t lda, int64_t n, int64_t c0, int w) {
+__global__ __launch_bounds__(TC) void trsm_block_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int w, int64_t cs,
+                                                        int64_t n) {
     __shared__ T L[PW][PW + 1];
     __shared__ T X[PW][TC + 1];
     const int t = threadIdx.x;
-    const int64_t col0 = c0 + w + (int64_t)blockIdx.x * TC;
+    const int64_t col0 = cs + (int64_t)blockIdx.x * TC;
     // All loads in flight at once, from addresses clamped into the matrix; zeros then pad a narrow panel.
     T lv[PW * PW / TC], xv[PW];
     const int64_t col = col0 + t < n ? col0 + t : n - 1;
@@ -427,36 +499,30 @@ void leaves(int64_t c0, int64_t c1, std::vector<int64_t>& ends) {
     ends.push_back(c1);
 }
 
-// Per-device state of the panel exchange: slots and flags, allocated once and never freed while the
-// library is loaded; the epoch makes each launch's flag targets larger than any value left behind.
-struct PanelState {
-    char* state = nullptr;  // MAX_WG flags, then 2 x MAX_WG slots
+// Per-device profile counters of the panel kernel.
+struct PanelProfile {
     long long* prof = nullptr;
-    bool prof_on = false;
-    int* status = nullptr;
-    long long* moves = nullptr;
-    unsigned long long epoch = 0;
+    bool on = false;
 };
 std::mutex g_panel_mu;
-PanelState g_panel[16];
+PanelProfile g_panel[16];
 
-int panel_state(PanelState*& st) {
+int panel_prof(PanelProfile*& st) {
     int dev = 0;
     HS_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_panel_mu);
     st = &g_panel[dev & 15];
-    if (!st->state) {
-        const size_t bytes = MAX_WG * sizeof(unsigned long long) + 2 * MAX_WG * sizeof(Slot<double>);
-        HS_HIP(hipMalloc(&st->state, bytes));
-        HS_HIP(hipMemset(st->state, 0, bytes));
-        HS_HIP(hipMalloc(&st->status, sizeof(int)));
+    if (!st->prof) {
         HS_HIP(hipMalloc(&st->prof, 8 * sizeof(long long)));
         HS_HIP(hipMemset(st->prof, 0, 8 * sizeof(long long)));
-        HS_HIP(hipMalloc(&st->moves, (1 + 4 * PW) * sizeof(long long)));
-        HS_HIP(hipMemset(st->status, 0, sizeof(int)));
         HS_HIP(hipDeviceSynchronize());
     }
     return OK;
+}
+
+__global__ void fill_words(unsigned long long* p, int64_t n, unsigned long long v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 }  // namespace
@@ -464,22 +530,30 @@ int panel_state(PanelState*& st) {
 // The panel kernel's per-phase profile (ticks of 10 ns, summed over launches): on = 1 clears and enables,
 // on = 0 disables; out (8 values) receives the sums where not null.
 int panel_profile(int on, long long* out) {
-    PanelState* st;
-    int e = panel_state(st);
+    PanelProfile* st;
+    int e = panel_prof(st);
     if (e) return e;
     if (out) HS_HIP(hipMemcpy(out, st->prof, 8 * sizeof(long long), hipMemcpyDeviceToHost));
     if (on) HS_HIP(hipMemset(st->prof, 0, 8 * sizeof(long long)));
-    st->prof_on = on != 0;
+    st->on = on != 0;
     return OK;
 }
 
+// Right-looking LU over blocks of up to OB = 256 columns, each made of panels of at most 64 (whole reference leaves).
+// Within a block, each panel is factored, its row moves applied to every other column, and only the block's own
+// columns right of it updated (a narrow product, k = the panel's width). After the block, the rows of U right of it
+// are solved panel by panel (a product over the block's earlier panels, then the panel's unit-lower triangle), and
+// the trailing matrix takes one product with k up to 256, which reads and writes it a quarter as often as per-panel
+// updates would. Every item still takes its terms in order of the pivot column, one fused multiply-add each.
 template <typename T>
 int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
     if (n == 0) return OK;
-    PanelState* st;
-    int e = panel_state(st);
+    if (n > (int64_t)MAX_WG * PT) { set_error("n above 16384: the panel kernel holds at most 32 x 512 rows"); return -1; }
+    PanelProfile* st;
+    int e = panel_prof(st);
     if (e) return e;
-    // Panels: consecutive whole reference leaves, at most PW columns each.
+    constexpr int64_t OB = 4 * PW;
+    // Panels: consecutive whole reference leaves, at most PW columns each; blocks: consecutive panels, at most OB.
     std::vector<int64_t> ends;
     leaves(0, n, ends);
     std::vector<int> leaf_end(n);
@@ -496,49 +570,81 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
         }
         panels.push_back({start, cur});
     }
+    std::vector<std::pair<size_t, size_t>> blocks;  // [first panel, end panel)
+    for (size_t i = 0; i < panels.size();) {
+        size_t j = i + 1;
+        while (j < panels.size() && panels[j].second - panels[i].first <= OB) ++j;
+        blocks.push_back({i, j});
+        i = j;
+    }
+    const size_t np = panels.size();
+    const size_t move_words = 1 + 4 * PW;
+    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 1;
     int* d_leaf = (int*)pool_get(n * sizeof(int));
     T* scratch = (T*)pool_get(sizeof(T) * (size_t)n * PW);
-    if (!d_leaf || !scratch) { pool_put(d_leaf); pool_put(scratch); return hip_status(hipErrorOutOfMemory, "pool_get"); }
+    char* xbuf = (char*)pool_get(clear_words * 8);
+    if (!d_leaf || !scratch || !xbuf) {
+        pool_put(d_leaf); pool_put(scratch); pool_put(xbuf);
+        return hip_status(hipErrorOutOfMemory, "pool_get");
+    }
+    Exchange* ex = (Exchange*)xbuf;
+    long long* moves = (long long*)(xbuf + sizeof(Exchange));
+    int* status = (int*)(moves + np * move_words);
     HS_HIP(hipMemcpyAsync(d_leaf, leaf_end.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+    fill_words<<<(unsigned)((clear_words + 255) / 256), 256, 0, s>>>((unsigned long long*)xbuf, (int64_t)clear_words, 0ull);
+    HS_HIP(hipGetLastError());
 
     int rc = OK;
-    for (auto [c0, c1] : panels) {
-        const int w = (int)(c1 - c0);
-        const int64_t rows = n - c0;
-        if (rows > (int64_t)MAX_WG * PT) { set_error("n above 16384: the panel kernel holds at most 32 x 512 rows"); rc = -1; break; }
-        const int nwg = (int)((rows + PT - 1) / PT);
-        unsigned long long epoch;
-        {
-            std::lock_guard<std::mutex> lk(g_panel_mu);
-            epoch = ++st->epoch;
-        }
-        const unsigned grid = nwg > 1 ? (unsigned)(8 * nwg) : 1u;
-        panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, st->moves, d_leaf, scratch, st->state, epoch, nwg,
-                                             st->status, st->prof_on ? st->prof : nullptr);
+    auto trsm = [&](int64_t c0, int w, int64_t cs, int64_t ce) {
+        if (ce <= cs) return OK;
+        trsm_block_kernel<T><<<(unsigned)((ce - cs + TC - 1) / TC), TC, 0, s>>>(a, lda, c0, w, cs, ce);
         HS_HIP(hipGetLastError());
-        const int64_t outside = n - w;
-        if (outside > 0) {
-            swap_kernel<T><<<(unsigned)((outside + 63) / 64), 256, 0, s>>>(a, lda, n, c0, c1, st->moves);
+        return OK;
+    };
+    for (auto [p0, p1] : blocks) {
+        const int64_t B0 = panels[p0].first, B1 = panels[p1 - 1].second;
+        for (size_t i = p0; i < p1 && rc == OK; ++i) {
+            const auto [c0, c1] = panels[i];
+            const int w = (int)(c1 - c0);
+            const int64_t rows = n - c0;
+            const int nwg = (int)((rows + PT - 1) / PT);
+            const unsigned grid = nwg > 1 ? (unsigned)(8 * nwg) : 1u;
+            long long* mv = moves + i * move_words;
+            panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, mv, d_leaf, scratch, ex,
+                                                 (unsigned)(i * PW + 1), nwg, status, st->on ? st->prof : nullptr);
             HS_HIP(hipGetLastError());
+            const int64_t outside = n - w;
+            if (outside > 0) {
+                swap_kernel<T><<<(unsigned)((outside + 63) / 64), 256, 0, s>>>(a, lda, n, c0, c1, mv);
+                HS_HIP(hipGetLastError());
+            }
+            if (c1 < B1) {
+                if ((rc = trsm(c0, w, c1, B1))) break;
+                rc = gemm<T>(false, false, n - c1, B1 - c1, w, T(-1), a + c1 * lda + c0, lda, a + c0 * lda + c1, lda, T(1),
+                             a + c1 * lda + c1, lda, s);
+            }
         }
-        if (c1 < n) {
-            const int64_t cols = n - c1;
-            trsm_block_kernel<T><<<(unsigned)((cols + TC - 1) / TC), TC, 0, s>>>(a, lda, n, c0, w);
-            HS_HIP(hipGetLastError());
-            rc = gemm<T>(false, false, n - c1, n - c1, w, T(-1), a + c1 * lda + c0, lda, a + c0 * lda + c1, lda, T(1),
-                         a + c1 * lda + c1, lda, s);
-            if (rc) break;
+        if (rc || B1 >= n) continue;
+        // Rows of U right of the block, panel by panel, then the trailing product.
+        for (size_t i = p0; i < p1 && rc == OK; ++i) {
+            const auto [c0, c1] = panels[i];
+            if (c0 > B0)
+                rc = gemm<T>(false, false, c1 - c0, n - B1, c0 - B0, T(-1), a + c0 * lda + B0, lda, a + B0 * lda + B1, lda,
+                             T(1), a + c0 * lda + B1, lda, s);
+            if (rc == OK) rc = trsm(c0, (int)(c1 - c0), B1, n);
         }
+        if (rc == OK)
+            rc = gemm<T>(false, false, n - B1, n - B1, B1 - B0, T(-1), a + B1 * lda + B0, lda, a + B0 * lda + B1, lda, T(1),
+                         a + B1 * lda + B1, lda, s);
     }
     if (rc == OK) {
         dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);
         transpose_square_inplace<T><<<grid, block, 0, s>>>(a, lda, n);
         HS_HIP(hipGetLastError());
-        int status = 0;
-        HS_HIP(hipMemcpyAsync(&status, st->status, sizeof(int), hipMemcpyDeviceToHost, s));
+        int stv = 0;
+        HS_HIP(hipMemcpyAsync(&stv, status, sizeof(int), hipMemcpyDeviceToHost, s));
         HS_HIP(hipStreamSynchronize(s));
-        if (status) {
-            HS_HIP(hipMemsetAsync(st->status, 0, sizeof(int), s));
+        if (stv) {
             set_error("panel exchange wait ran out (workgroups not co-resident?)");
             rc = ERR_LIMIT;
         }
@@ -547,6 +653,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
     }
     pool_put(d_leaf);
     pool_put(scratch);
+    pool_put(xbuf);
     return rc;
 }
 

@@ -47,15 +47,58 @@ __device__ __forceinline__ unsigned long long mag_key(T x) {
 __device__ __forceinline__ bool better(unsigned long long kv, int kp, unsigned long long kv2, int kp2) {
     return (kv2 > kv) | ((kv2 == kv) & (kp2 < kp));
 }
-__device__ __forceinline__ void wave_best(unsigned long long& kv, int& kp, int& who) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long v2 = __shfl_xor(kv, off);
-        const int p2 = __shfl_xor(kp, off), w2 = __shfl_xor(who, off);
-        const bool take = better(kv, kp, v2, p2);
-        kv = take ? v2 : kv;
-        kp = take ? p2 : kp;
-        who = take ? w2 : who;
+// The wave's best key: the largest kv through DPP row shifts and broadcasts (the maximum lands in lane 63), then
+// among the lanes holding it the smallest position (one lane, save for ties). Returns wave-uniform values; `lane_of`
+// gets the winning lane (-1 where no lane is a candidate: kp = INT_MAX everywhere).
+__device__ __forceinline__ unsigned long long dpp_max_u64(unsigned long long x) {
+    unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+#define HS_DPP_STEP(ctrl, rm, bm)                                                                      \
+    do {                                                                                               \
+        const unsigned lo2 = (unsigned)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, rm, bm, false);   \
+        const unsigned hi2 = (unsigned)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, rm, bm, false);   \
+        const bool gt = (hi2 > hi) | ((hi2 == hi) & (lo2 > lo));                                      \
+        lo = gt ? lo2 : lo;                                                                            \
+        hi = gt ? hi2 : hi;                                                                            \
+    } while (0)
+    HS_DPP_STEP(0x111, 0xf, 0xf);  // row_shr:1
+    HS_DPP_STEP(0x112, 0xf, 0xf);  // row_shr:2
+    HS_DPP_STEP(0x114, 0xf, 0xf);  // row_shr:4
+    HS_DPP_STEP(0x118, 0xf, 0xf);  // row_shr:8: lane 15 of each row holds the row's maximum
+    HS_DPP_STEP(0x142, 0xa, 0xf);  // row_bcast:15 into rows 1 and 3
+    HS_DPP_STEP(0x143, 0xc, 0xf);  // row_bcast:31 into rows 2 and 3: lane 63 holds the wave's maximum
+#undef HS_DPP_STEP
+    const unsigned blo = (unsigned)__builtin_amdgcn_readlane((int)lo, 63);
+    const unsigned bhi = (unsigned)__builtin_amdgcn_readlane((int)hi, 63);
+    return ((unsigned long long)bhi << 32) | blo;
+}
+__device__ __forceinline__ void wave_best(unsigned long long& kv, int& kp, int& lane_of) {
+    const unsigned long long best = dpp_max_u64(kv);
+    unsigned long long m = __ballot(kv == best && kp != 0x7fffffff);
+    if (m == 0ull) { kv = 0ull; kp = 0x7fffffff; lane_of = -1; return; }
+    int L = __builtin_ctzll(m);
+    int bp = __builtin_amdgcn_readlane(kp, L);
+    m &= m - 1;
+    while (m) {  // ties of magnitude (rare): the smallest position
+        const int L2 = __builtin_ctzll(m);
+        const int p2 = __builtin_amdgcn_readlane(kp, L2);
+        if (p2 < bp) { bp = p2; L = L2; }
+        m &= m - 1;
+    }
+    kv = best;
+    kp = bp;
+    lane_of = L;
+}
+
+// Lane l's value of v (l uniform), as a scalar operand.
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int l) {
+    if constexpr (sizeof(T) == 8) {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
     }
 }
 
@@ -188,28 +231,31 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
 
     __shared__ unsigned long long s_kv[2][WAVES];
     __shared__ int s_kp[2][WAVES];
-    __shared__ T s_piv[2][PW];   // the pivot rows of the last two columns
+    __shared__ T s_piv[2][PW];  // the pivot rows of the last two columns
     __shared__ T s_hand[WAVES][PW + 2];
     __shared__ T s_u11[PW][PW];
-    __shared__ int s_p[2], s_abort;
+    __shared__ int s_p[2], s_abort, s_skip[PW];
     if (tid == 0) s_abort = 0;
+    if (tid < PW) s_skip[tid] = tid < w ? leaf_end[c0 + tid] - (int)c0 - tid : 0;  // a zero pivot's skip, by column
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
 
     // x0: this row's item in the current column, fully updated; the rest of v may still owe the previous column's
-    // update (pending), with multiplier l_prev, the pivot row in s_piv[buf ^ 1] and the zero-pivot skip.
+    // update (pending), with multiplier l_prev, the pivot row prev and the zero-pivot skip.
     T x0 = v[0], l_prev = T(0);
+    const T* prev = s_piv[1];
     bool pending = false;
     int skip_prev = 0;
 
 #pragma clang loop unroll(disable)
     for (int c = 0; c < w; ++c) {
         const int buf = c & 1;
+        const int rem = w - c;  // items of the current row that lie in the panel
         const unsigned tag = tag0 + (unsigned)c;
         // This row's candidacy for column c and the wave's best.
         const bool cand = has && pos >= c;
         unsigned long long kv = cand ? mag_key(x0) : 0ull;
-        int kp = cand ? pos : 0x7fffffff, who = wave;
-        wave_best(kv, kp, who);
+        int kp = cand ? pos : 0x7fffffff, hl = 0;
+        wave_best(kv, kp, hl);
         if (lane == 0) { s_kv[buf][wave] = kv; s_kp[buf][wave] = kp; }
         __syncthreads();
         HS_PHASE(0);
@@ -246,23 +292,34 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     item = h[lane];
                 } else {
                     const int j = lane + 1 < PW ? lane + 1 : PW - 1;
-                    const T u = __builtin_fma(h[PW + 1], -s_piv[buf ^ 1][j], h[j]);
+                    const T u = __builtin_fma(h[PW + 1], -prev[j], h[j]);
                     item = lane == 0 ? h[PW] : (lane == PW - 1 ? T(0) : (j >= skip_prev ? u : h[j]));
                 }
-                if (nwg > 1) put_item<T>(ex->row[buf][g], lane, tag, item);
-                else s_piv[buf][lane] = item;
+                if (nwg > 1) {
+                    if (lane < rem) put_item<T>(ex->row[buf][g], lane, tag, item);
+                } else {
+                    s_piv[buf][lane] = item;
+                }
             }
         }
 
-        // The previous column's update of the rest of the row.
+        // The previous column's update of the rest of the row, sixteen items at a time, only as far as the panel
+        // reaches (items past it are never used).
+        // The pivot row reaches every lane as scalars: one LDS read per wave (lane j takes item j, with every lane
+        // active, so that each lane's register holds its item), then lane j's item is read out as a uniform operand,
+        // rather than every lane reading every item from LDS.
+        T mine = prev[lane];
+        asm volatile("" : "+v"(mine));  // loaded here, by every lane, not sunk into the branch below
         if (pending) {
-            const T* pp = s_piv[buf ^ 1];
+            const T* pp = prev;
             if (skip_prev <= 2) {
 #pragma unroll
                 for (int j0 = 2; j0 < PW; j0 += 16) {
+                    if (j0 <= rem) {
 #pragma unroll
-                    for (int j = j0; j < j0 + 16 && j < PW; ++j) v[j - 1] = __builtin_fma(l_prev, -pp[j], v[j]);
-                    asm volatile("" ::: "memory");  // sixteen items of the pivot row in registers at a time
+                        for (int j = j0; j < j0 + 16 && j < PW; ++j)
+                            v[j - 1] = __builtin_fma(l_prev, -lane_value(mine, j), v[j]);
+                    }
                 }
             } else {  // a zero pivot: the rest of its leaf (at most 16 columns) keeps its items
 #pragma unroll
@@ -277,9 +334,10 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         HS_PHASE(1);
 
         if (nwg > 1) {
+            // Wave 0 compares the workgroups' keys, then fetches the winner's row.
             if (wave == 0) {
                 unsigned long long gkv = 0ull;
-                int gkp = 0x7fffffff, gw = lane;
+                int gkp = 0x7fffffff, gl = 0;
                 bool ok = true;
                 if (lane < nwg) {
                     uint64_t spins = 0;
@@ -288,15 +346,15 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     }
                 }
                 bool all = __all(ok);
-                wave_best(gkv, gkp, gw);
+                wave_best(gkv, gkp, gl);
                 T item = T(0);
-                if (all) {
+                if (all && lane < rem) {
                     uint64_t spins = 0;
-                    while (!get_item<T>(ex->row[buf][gw], lane, tag, item)) {
+                    while (!get_item<T>(ex->row[buf][gl], lane, tag, item)) {
                         if (++spins > SPIN_LIMIT) { ok = false; break; }
                     }
-                    all = __all(ok);
                 }
+                all = __all(ok);
                 s_piv[buf][lane] = item;
                 if (lane == 0) {
                     s_p[buf] = gkp;
@@ -325,7 +383,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         // column's candidates to be out.
         if (has && pos > c) {
             const bool zero = pivot == T(0);
-            const int skip = zero ? leaf_end[c0 + c] - (int)c0 - c : 0;
+            const int skip = zero ? s_skip[c] : 0;
             const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * (T(1) / pivot) : x0 / pivot);
             x0 = 1 >= skip ? __builtin_fma(l, -piv[1], v[1]) : v[1];
             scratch[(int64_t)c * rows + phys] = l;
@@ -333,6 +391,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
             skip_prev = skip;
             pending = true;
         }
+        prev = piv;
         HS_PHASE(4);
     }
     if (timed)

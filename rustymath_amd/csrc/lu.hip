@@ -460,51 +460,75 @@ __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t ld
     }
 }
 
-// U12 = L11⁻¹ A12 for rows [c0, c0 + w), columns [cs, n): a workgroup takes 128 columns, stages
-// them and L11 through LDS with coalesced row reads, and each thread solves one column in registers,
-// each row taking its terms in order of the pivot column (factor.rs:668-689's leaf order).
-constexpr int TC = 128;
+// U12 = L11⁻¹ A12 for rows [c0, c0 + w), columns [cs, n), w <= 64: a workgroup takes 64 columns and keeps them
+// and L11 in LDS. In four blocks of 16 rows: the block's rows substitute among themselves (one lane per column, in
+// order of the pivot column), then the blocks below take the block's terms on MFMA (16x16x4 steps, each rounding
+// its four terms as fused multiply-adds in order). So every item takes its terms in order of the pivot column,
+// one fused multiply-add each, as rustymath's `forward` (core/src/linalg/factor.rs:668-689) gives them.
+constexpr int TC = 64;
 template <typename T>
-__global__ __launch_bounds__(TC) void trsm_block_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int w, int64_t cs,
-                                                        int64_t n) {
-    __shared__ T L[PW][PW + 1];
-    __shared__ T X[PW][TC + 1];
-    const int t = threadIdx.x;
+__global__ __launch_bounds__(256) void trsm_block_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int w, int64_t cs,
+                                                         int64_t n) {
+    __shared__ T Ls[PW][PW + 1];   // strictly lower part of L11, zeros elsewhere
+    __shared__ T Xs[PW][TC + 1];
+    typedef typename Mfma<T>::acc Acc;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t col0 = cs + (int64_t)blockIdx.x * TC;
-    // All loads in flight at once, from addresses clamped into the matrix; zeros then pad a narrow panel.
-    T lv[PW * PW / TC], xv[PW];
-    const int64_t col = col0 + t < n ? col0 + t : n - 1;
+    // All loads in flight at once, from addresses clamped into the matrix; zeros then pad.
+    T lv[PW * PW / 256], xv[PW * TC / 256];
 #pragma unroll
-    for (int q = 0; q < PW * PW / TC; ++q) {
-        const int idx = t + q * TC, r = idx / PW, c = idx % PW;
+    for (int q = 0; q < PW * PW / 256; ++q) {
+        const int idx = t + 256 * q, r = idx / PW, c = idx % PW;
         lv[q] = a[(c0 + (r < w ? r : w - 1)) * lda + c0 + (c < w ? c : w - 1)];
+        const int64_t col = col0 + c < n ? col0 + c : n - 1;
+        xv[q] = a[(c0 + (r < w ? r : w - 1)) * lda + col];
     }
-#pragma unroll
-    for (int r = 0; r < PW; ++r) xv[r] = a[(c0 + (r < w ? r : w - 1)) * lda + col];
     asm volatile("" ::: "memory");
 #pragma unroll
-    for (int q = 0; q < PW * PW / TC; ++q) {
-        const int idx = t + q * TC, r = idx / PW, c = idx % PW;
-        L[r][c] = (r < w && c < w) ? lv[q] : T(0);
+    for (int q = 0; q < PW * PW / 256; ++q) {
+        const int idx = t + 256 * q, r = idx / PW, c = idx % PW;
+        Ls[r][c] = (r < w && c < r) ? lv[q] : T(0);
+        Xs[r][c] = (r < w && col0 + c < n) ? xv[q] : T(0);
+    }
+    __syncthreads();
+    for (int kb = 0; kb < PW / 16 && 16 * kb < w; ++kb) {
+        const int k0 = 16 * kb;
+        if (wave == 0) {  // the block's rows among themselves; lane = column
+            T x[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) x[i] = Xs[k0 + i][lane];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+#pragma unroll
+                for (int r = q + 1; r < 16; ++r) x[r] = __builtin_fma(-Ls[k0 + r][k0 + q], x[q], x[r]);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Xs[k0 + i][lane] = x[i];
+        }
+        __syncthreads();
+        const int rb = wave, r0 = 16 * rb;  // the blocks below, one per wave, on MFMA
+        if (rb > kb && r0 < w) {
+#pragma unroll
+            for (int cb = 0; cb < TC / 16; ++cb) {
+                Acc acc;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = Xs[r0 + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)];
+#pragma unroll
+                for (int st = 0; st < 4; ++st) {
+                    const T av = -Ls[r0 + (lane & 15)][k0 + 4 * st + (lane >> 4)];
+                    const T bv = Xs[k0 + 4 * st + (lane >> 4)][16 * cb + (lane & 15)];
+                    acc = Mfma<T>::step(av, bv, acc);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Xs[r0 + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)] = acc[e];
+            }
+        }
+        __syncthreads();
     }
 #pragma unroll
-    for (int r = 0; r < PW; ++r) X[r][t] = (r < w && col0 + t < n) ? xv[r] : T(0);
-    __syncthreads();
-    T x[PW];
-#pragma unroll
-    for (int r = 0; r < PW; ++r) x[r] = X[r][t];
-    // Padding rows and columns are zero, so the full 64-row sweep leaves the real rows' bits unchanged.
-#pragma unroll
-    for (int q = 0; q < PW; ++q) {
-#pragma unroll
-        for (int r = q + 1; r < PW; ++r) x[r] = __builtin_fma(-L[r][q], x[q], x[r]);
+    for (int q = 0; q < PW * TC / 256; ++q) {
+        const int idx = t + 256 * q, r = idx / TC, c = idx % TC;
+        if (r < w && col0 + c < n) a[(c0 + r) * lda + col0 + c] = Xs[r][c];
     }
-#pragma unroll
-    for (int r = 0; r < PW; ++r) X[r][t] = x[r];
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < PW; ++r)
-        if (r < w && col0 + t < n) a[(c0 + r) * lda + col0 + t] = X[r][t];
 }
 
 template <typename T>
@@ -656,7 +680,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
     int rc = OK;
     auto trsm = [&](int64_t c0, int w, int64_t cs, int64_t ce) {
         if (ce <= cs) return OK;
-        trsm_block_kernel<T><<<(unsigned)((ce - cs + TC - 1) / TC), TC, 0, s>>>(a, lda, c0, w, cs, ce);
+        trsm_block_kernel<T><<<(unsigned)((ce - cs + TC - 1) / TC), 256, 0, s>>>(a, lda, c0, w, cs, ce);
         HS_HIP(hipGetLastError());
         return OK;
     };

@@ -25,6 +25,7 @@
 // factors, where perm is rustymath's `perm` (factor.rs:132-134). B is overwritten by X where the system is
 // regular. The result of a system does not depend on which workgroup ran it, when, or the tuning choices.
 #include "hs_kernels.h"
+#include "hs_pivot.h"
 #include <vector>
 #include <algorithm>
 #include <numeric>
@@ -49,32 +50,6 @@ struct Desc {
 
 template <typename T>
 struct Bits { typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type U; };
-
-// The first greatest (value, key): larger value, then smaller key.
-template <typename T>
-__device__ __forceinline__ void wave_best(T& v, int& r) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const T v2 = __shfl_xor(v, off);
-        const int r2 = __shfl_xor(r, off);
-        const bool take = (v2 > v) | ((v2 == v) & (r2 < r));
-        v = take ? v2 : v;
-        r = take ? r2 : r;
-    }
-}
-
-// Lane l's value of v (l uniform).
-template <typename T>
-__device__ __forceinline__ T readlane(T v, int l) {
-    if constexpr (sizeof(T) == 8) {
-        const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
-        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
-    } else {
-        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-    }
-}
 
 // One system's matrix as a buffer resource: loads at an index past its n² items return +0 and stores there are
 // dropped (the buffer unit's range check), so guarded loads need no branch and stay in flight together.
@@ -187,8 +162,8 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
     __shared__ short perm[BT];       // position -> row
     __shared__ short leaf_end[BT];   // end of the reference leaf of each column
     __shared__ short panel_end[BT];  // for a panel's first column, the panel's end
-    __shared__ T s_wval[2][WAVES];   // each wave's best candidate, double-buffered by column
-    __shared__ int s_wkey[2][WAVES];
+    __shared__ unsigned long long s_wkv[2][WAVES];   // each wave's best candidate, double-buffered by column
+    __shared__ int s_wkp[2][WAVES];
     // Work area: the panel's candidate rows [2][WAVES][BW], its rows of U [BW][BW] and unit-lower triangle
     // [BW][BW + 1]; or the substitutions' x [BT][MAXRHS].
     constexpr int PANEL_ITEMS = 2 * WAVES * BW + BW * BW + BW * (BW + 1);
@@ -271,34 +246,28 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                 for (int c = 0; c < w; ++c) {
                     const int buf = c & 1;
                     // First greatest magnitude at positions >= c, NaN ignored; all NaN -> position c (factor.rs:476-490).
-                    T val = T(-1);
-                    int key = 0x7fffffff;
-                    if (has && pos >= c) { const T x = fabs(v[0]); val = x != x ? T(-1) : x; key = pos; }
-                    T bv = val;
-                    int bk = key;
-                    wave_best(bv, bk);
-                    if (bk != 0x7fffffff && key == bk) {
+                    const bool cnd = has && pos >= c;
+                    unsigned long long kv = cnd ? mag_key(v[0]) : 0ull;
+                    int kp = cnd ? pos : 0x7fffffff, hl = 0;
+                    wave_best(kv, kp, hl);
+                    if (lane == hl) {
 #pragma unroll
                         for (int j = 0; j < BW; ++j) s_cand[buf][wave][j] = v[j];
                     }
-                    if (lane == 0) { s_wval[buf][wave] = bv; s_wkey[buf][wave] = bk; }
+                    if (lane == 0) { s_wkv[buf][wave] = kv; s_wkp[buf][wave] = kp; }
                     __syncthreads();
-                    T wv[WAVES];
-                    int wk[WAVES];
-#pragma unroll
-                    for (int u = 0; u < WAVES; ++u) { wv[u] = s_wval[buf][u]; wk[u] = s_wkey[buf][u]; }
-                    T wb = wv[0];
-                    int p = wk[0], win = 0;
-#pragma unroll
-                    for (int u = 1; u < WAVES; ++u) {
-                        const bool take = (wv[u] > wb) | ((wv[u] == wb) & (wk[u] < p));
-                        wb = take ? wv[u] : wb;
-                        p = take ? wk[u] : p;
-                        win = take ? u : win;
-                    }
+                    // The waves' bests: lane u of every wave reads wave u's, and a wave reduction picks the winner
+                    // (rather than every thread reading every wave's key from LDS).
+                    unsigned long long bkv = lane < WAVES ? s_wkv[buf][lane] : 0ull;
+                    int p = lane < WAVES ? s_wkp[buf][lane] : 0x7fffffff, win = 0;
+                    wave_best(bkv, p, win);
                     const T* piv = s_cand[buf][win];
+                    // The pivot row as scalar operands: lane j reads item j (every lane active), then reads go lane
+                    // to scalar.
+                    T mine = piv[lane < BW ? lane : 0];
+                    asm volatile("" : "+v"(mine));
                     if (tid < BW) s_u11[c][tid] = piv[tid];
-                    const T pivot = piv[0];
+                    const T pivot = lane_value(mine, 0);
                     if (has && pos == p) pos = c;
                     else if (has && pos == c) pos = p;
                     if (has && pos > c) {
@@ -307,7 +276,13 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                         if (pivot != T(0)) {
                             l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
 #pragma unroll
-                            for (int j = 1; j < BW; ++j) v[j - 1] = __builtin_fma(l, -piv[j], v[j]);
+                            for (int j0 = 1; j0 < BW; j0 += 16) {
+                                if (j0 < w - c) {  // items past the panel are never used
+#pragma unroll
+                                    for (int j = j0; j < j0 + 16 && j < BW; ++j)
+                                        v[j - 1] = __builtin_fma(l, -lane_value(mine, j), v[j]);
+                                }
+                            }
                         } else {
                             const int skip = leaf_end[c0 + c] - c0 - c;
 #pragma unroll
@@ -396,7 +371,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     for (int j = 0; j < SB; ++j) {
                         if (j < h) {
                             if (tr && lane == j) x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
-                            const T xj = readlane(x, j);
+                            const T xj = lane_value(x, j);
                             if (act && lane > j) x = __builtin_fma(xj, -mj[j], x);
                         }
                     }
@@ -439,7 +414,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                             const int L = SB - 1 - s;
                             if (s < h) {
                                 if (!tr && lane == L) x = x / dg;
-                                const T xl = readlane(x, L);
+                                const T xl = lane_value(x, L);
                                 if (act && lane < L) x = __builtin_fma(xl, -mj[L], x);
                             }
                         }
@@ -489,7 +464,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                                     for (int j = L + 1; j < SB; ++j) x = __builtin_fma(xs[j], -mj[j], x);
                                     x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
                                 }
-                                xs[L] = readlane(x, L);
+                                xs[L] = lane_value(x, L);
                             }
                         }
                         if (act) s_x[r][wave] = x;

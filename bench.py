@@ -14,6 +14,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'FP64-accurate solve GFLOP/s: low-prec LU+IR vs plain FP64 LU; batched solves/s'
 PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'hbm': 8.0e12}
+# Measured HBM bytes per launch of the FP32 trailing products in one mixed solve of order n: two rocprofv3 --pmc passes
+# (FETCH_SIZE, then WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --only-mixed`, each scaled by what the
+# same counter reads on a 1 GiB device copy (`bench.py --copy-probe 1024`: FETCH_SIZE counts half the bytes read,
+# WRITE_SIZE all the bytes written); the summary is profiles/gemm_f32_lu_update_n16384_traffic.md.
+TRAFFIC = {('gemm_f32', 16384): 139623424}
 
 
 def householder_matrix(n, seed, cond=1e6, reflectors=8):
@@ -132,6 +137,57 @@ def adjoint_bench(reps=3):
     return out
 
 
+def plain_fp64_bench(n=8192, k=16, reps=3):
+    """The plain FP64 workload on the GPU: matmul of two n×n float64 N(0, 1) matrices (2 n³ flops), and the LU solve
+    of an n×n float64 N(0, 1) system with k right-hand sides (2/3 n³ + 2 n² k flops: copy A, factor, solve), each the
+    best of `reps` runs timed with HIP events; with the product's error against the k·eps·|A||B| bound on a sample of
+    rows and the solve's normwise backward error ||B - A X||_F / (||A||_F ||X||_F)."""
+    import torch
+    from rustymath_amd import device as dv
+    g = torch.Generator(device='cuda'); g.manual_seed(0)
+    a = torch.randn(n, n, dtype=torch.float64, device='cuda', generator=g)
+    b = torch.randn(n, n, dtype=torch.float64, device='cuda', generator=g)
+    c = torch.empty(n, n, dtype=torch.float64, device='cuda')
+
+    def timed(fn):
+        best = math.inf
+        for r in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(); fn(); e1.record(); e1.synchronize()
+            if r: best = min(best, e0.elapsed_time(e1) / 1e3)
+        return best
+
+    t_mm = timed(lambda: dv.gemm(a, b, c))
+    rows = torch.arange(0, n, n // 8, device='cuda')
+    cols = torch.arange(0, n, n // 256, device='cuda')
+    ar, bc = a[rows].cpu().numpy(), b[:, cols].cpu().numpy()
+    exact = ar.astype('longdouble') @ bc.astype('longdouble')
+    bound = n * float(torch.finfo(torch.float64).eps) * (abs(ar) @ abs(bc))
+    mm_ratio = float((abs(c[rows][:, cols].cpu().numpy() - exact) / bound).max())
+    del b, c
+    g.manual_seed(1)
+    A = torch.randn(n, n, dtype=torch.float64, device='cuda', generator=g)
+    B = torch.randn(n, k, dtype=torch.float64, device='cuda', generator=g)
+    work, X = torch.empty_like(A), torch.empty_like(B)
+
+    def solve():
+        work.copy_(A)
+        f, perm = dv.getrf(work)
+        X.copy_(B)
+        dv.getrs(f, perm, X)
+    t_lu = timed(solve)
+    berr = float((B - A @ X).norm() / (A.norm() * X.norm()))
+    lu_flops = 2 / 3 * n ** 3 + 2 * n * n * k
+    return {'workload': f'matmul {n}x{n}x{n} float64 N(0,1) seed 0; LU solve {n}x{n} float64 N(0,1) with {k} right-hand '
+                        'sides, seed 1 (copy, factor, solve)',
+            'matmul_ms': round(t_mm * 1e3, 3), 'matmul_gflops': round(2 * n ** 3 / t_mm / 1e9, 1),
+            'matmul_frac_fp64_peak': round(2 * n ** 3 / t_mm / PEAK['fp64_mfma'], 4),
+            'matmul_max_error_over_bound': mm_ratio,
+            'lu_solve_ms': round(t_lu * 1e3, 3), 'lu_solve_gflops': round(lu_flops / t_lu / 1e9, 1),
+            'lu_solve_frac_fp64_peak': round(lu_flops / t_lu / PEAK['fp64_mfma'], 4),
+            'backward_error': berr, 'backward_error_bound': 10 * n * float(torch.finfo(torch.float64).eps)}
+
+
 def cpu_baseline(n=6144, runs=3, threads=16):
     """The oracle's LU solve (the C restatement of rustymath's blocked CPU kernels) on a bounded sample: an FP64 N(0, 1)
     system of order n on `threads` host threads, best of `runs`, in GFLOP/s (2/3 n³ + 2 n²)."""
@@ -164,7 +220,19 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--n', type=int, default=16384)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--only-mixed', action='store_true', help='the mixed-precision solve only (for profiling runs)')
+    ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
+                    help='only copy MIB MiB between two device buffers 5 times: the known-size kernel that scales the '
+                         'HBM byte counters of a profiling run')
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe << 18, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(5): dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_probe_bytes_per_launch': {'read': args.copy_probe << 20, 'written': args.copy_probe << 20}}))
+        return
     if args.gpus != 1:
         print(f'bench.py runs on one GPU: this build measures the single-GPU path only (asked for --gpus {args.gpus})',
               file=sys.stderr)
@@ -209,27 +277,36 @@ def main():
     t = sum(times) / len(times)
     fwd = ((x - x64).norm() / x64.norm()).item()
 
-    # The dominant kernel's roofline: the FP32 MFMA trailing product of the factorization, timed live with HIP events
-    # on its stream by the library over one more factorization (hs_profile_gemm).
-    roof = None
-    try:
-        gms, gl, gf = ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0)
-        lib.hs_profile_gemm(1, None, None, None)
-        dv.dsgesv(a, b, tol=tol); torch.cuda.synchronize()
-        lib.hs_profile_gemm(0, ctypes.byref(gms), ctypes.byref(gl), ctypes.byref(gf))
-        if gl.value:
-            per_launch = gf.value / gl.value
-            avg_s = gms.value / 1e3 / gl.value
-            achieved = per_launch / avg_s / 1e12
-            roof = {'bound': 'mfma', 'achieved': round(achieved, 2), 'peak': PEAK['fp32_mfma'] / 1e12, 'unit': 'TFLOP/s',
-                    'frac': round(achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': None,
-                    'kernel': 'gemm_mfma<float> (LU trailing update)', 'launches': gl.value,
-                    'flop_per_launch': round(per_launch), 'avg_launch_us': round(avg_s * 1e6, 2)}
-    except AttributeError:
-        pass
-
-    batched = batched_bench()
-    adj = adjoint_bench()
+    # The roofline of the MFMA kernel that carries the factorization's flops: the FP32 trailing product, timed live
+    # with HIP events on its stream by the library over one more solve (hs_profile_gemm), with its algorithmic flops and
+    # bytes per launch; `traffic` is the measured HBM bytes per launch (TRAFFIC). The panel kernel, which takes a
+    # comparable share of the step, is latency-bound (one pivot exchange per column): its per-column time comes from
+    # its own phase counters (hs_profile_panel, workgroup 0) over the same solve.
+    gms, gl, gf, gb = ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)
+    ticks = (ctypes.c_int64 * 8)()
+    lib.hs_profile_gemm(1, None, None, None, None)
+    lib.hs_profile_panel(1, None)
+    dv.dsgesv(a, b, tol=tol); torch.cuda.synchronize()
+    lib.hs_profile_gemm(0, ctypes.byref(gms), ctypes.byref(gl), ctypes.byref(gf), ctypes.byref(gb))
+    lib.hs_profile_panel(0, ticks)
+    per_launch = gf.value / gl.value
+    avg_s = gms.value / 1e3 / gl.value
+    achieved = per_launch / avg_s / 1e12
+    roof = {'bound': 'mfma', 'achieved': round(achieved, 2), 'peak': PEAK['fp32_mfma'] / 1e12, 'unit': 'TFLOP/s',
+            'frac': round(achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': TRAFFIC.get(('gemm_f32', n)),
+            'kernel': 'gemm_mfma<float> (LU trailing updates)', 'launches': gl.value,
+            'flop_per_launch': round(per_launch), 'algorithmic_bytes_per_launch': round(gb.value / gl.value),
+            'avg_launch_us': round(avg_s * 1e6, 2)}
+    panel_s = sum(ticks[:5]) / 1e8
+    panel = {'kernel': 'panel_kernel<float>', 'bound': 'latency (one pivot exchange per column)',
+             'column_loop_ms': round(panel_s * 1e3, 3), 'us_per_column': round(panel_s / n * 1e6, 3),
+             'phases_ms': {k: round(v / 1e5, 3) for k, v in zip(('reduce', 'publish_update', 'exchange', 'barrier',
+                                                                  'pivot_update'), ticks[:5])}}
+    batched = adj = plain = None
+    if not args.only_mixed:
+        batched = batched_bench()
+        adj = adjoint_bench()
+        plain = plain_fp64_bench()
 
     out = {
         'metric': METRIC, 'value': round(flops / t / 1e9, 2), 'unit': 'GFLOP/s', 'n_gpus': 1, 'steps': args.steps,
@@ -239,9 +316,9 @@ def main():
                                f's = logspace(0, -6)), cond 1e6, b N(0,1), seed 3', 'n': n, 'rhs': 1},
         'fp64_lu_solve_gflops': round(flops / t64 / 1e9, 2), 'speedup_vs_fp64_lu': round(t64 / t, 3),
         'refinement_solves': iters, 'backward_error': berr, 'backward_error_fp64_lu': r64, 'tolerance': tol,
-        'forward_error_vs_fp64_lu': fwd, 'roofline': roof, 'batched': batched, 'adjoint': adj,
+        'forward_error_vs_fp64_lu': fwd, 'roofline': roof, 'panel': panel, 'batched': batched, 'adjoint': adj, 'plain_fp64': plain,
     }
-    if not args.no_cpu:
+    if not args.no_cpu and not args.only_mixed:
         out['cpu_baseline'] = cpu_baseline()
         out['batched']['cpu_baseline'] = batched_cpu_baseline()
     print(json.dumps(out))

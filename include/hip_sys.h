@@ -118,8 +118,9 @@ int hs_dev_gesv_batched_free(int64_t plan);
  * on = 0 stops them; `out` (8 values, may be NULL) receives the sums. */
 int hs_profile_panel(int on, int64_t* out);
 /* Profiling of the MFMA product: on = 1 clears and starts bracketing every product launch with HIP events on its
- * stream, on = 0 stops; gives the launches' summed time (ms), their count and their algorithmic flops (2 m n k). */
-int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops);
+ * stream, on = 0 stops; gives the launches' summed time (ms), their count, their algorithmic flops (2 m n k) and
+ * algorithmic bytes (a and b read once, c written once and, where beta is not 0, read once). Any output may be NULL. */
+int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops, double* bytes);
 /* Profiling of the batched solves: on = 1 clears and starts per-phase timers in workgroup 0 (100 MHz ticks: setup,
  * panel columns, U rows and trailing update, regularity check and L, U), so that runs launch directly rather than
  * replay their graph; on = 0 stops. `out` (8 values, may be NULL) gets the sums, out[7] the systems timed. */

@@ -38,7 +38,7 @@ SIGNATURES = {
     'hs_dev_gesv_batched_run': (_i32, [_i64, _i32, _vp]),
     'hs_dev_gesv_batched_free': (_i32, [_i64]),
     'hs_profile_panel': (_i32, [_i32, _vp]),
-    'hs_profile_gemm': (_i32, [_i32, _vp, _vp, _vp]),
+    'hs_profile_gemm': (_i32, [_i32, _vp, _vp, _vp, _vp]),
     'hs_profile_batched': (_i32, [_i32, _vp]),
 }
 

@@ -390,6 +390,8 @@ int hs_dev_gesv_batched_free(int64_t plan) { return batched_free(plan); }
 
 int hs_profile_panel(int on, int64_t* out) { return panel_profile(on, (long long*)out); }
 int hs_profile_batched(int on, int64_t* out) { return batched_profile(on, (long long*)out); }
-int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops) { return gemm_profile(on, ms, launches, flops); }
+int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
+    return gemm_profile(on, ms, launches, flops, bytes);
+}
 
 }  // extern "C"

@@ -7,6 +7,7 @@
 // accumulator per result for the whole of k and feeds the MFMA unit in order of p, so it computes the
 // same chain of roundings; a k tail pads with +0 · (-0) = -0, which adds nothing to any value.
 #include "hs_kernels.h"
+#include <hip/hip_ext.h>
 #include <mutex>
 #include <vector>
 
@@ -17,26 +18,27 @@ namespace {
 struct GemmProfile {
     std::mutex mu;
     bool on = false;
-    struct Rec { hipEvent_t a, b; double flops; };
+    struct Rec { hipEvent_t a, b; double flops, bytes; };
     std::vector<Rec> recs;
 } g_prof;
 }  // namespace
 
-int gemm_profile(int on, double* ms, int64_t* launches, double* flops) {
+int gemm_profile(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
     std::lock_guard<std::mutex> lk(g_prof.mu);
-    double total = 0, fl = 0;
+    double total = 0, fl = 0, by = 0;
     int64_t count = 0;
     for (auto& r : g_prof.recs) {
         float e = 0;
         HS_HIP(hipEventSynchronize(r.b));
         HS_HIP(hipEventElapsedTime(&e, r.a, r.b));
-        total += e; fl += r.flops; ++count;
+        total += e; fl += r.flops; by += r.bytes; ++count;
         hipEventDestroy(r.a); hipEventDestroy(r.b);
     }
     g_prof.recs.clear();
     if (ms) *ms = total;
     if (launches) *launches = count;
     if (flops) *flops = fl;
+    if (bytes) *bytes = by;
     g_prof.on = on != 0;
     return OK;
 }
@@ -361,9 +363,20 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
     int tiles_m = (int)((m + BM - 1) / BM), tiles_n = (int)((n + BN - 1) / BN);
     int total = tiles_m * tiles_n;
     unsigned grid = (unsigned)(((total + 7) / 8) * 8);
-#define HS_LAUNCH(TA_, TB_) \
-    gemm_mfma<T, TA_, TB_><<<grid, THREADS, 0, s>>>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tiles_m, tiles_n)
-    GemmProfile::Rec rec{nullptr, nullptr, 2.0 * (double)m * (double)n * (double)k};
+    // While profiling, the launch carries its own start and stop events (hipExtLaunchKernelGGL records them at the
+    // dispatch's start and end), so the measured time is the kernel's alone.
+#define HS_LAUNCH(TA_, TB_)                                                                                        \
+    do {                                                                                                           \
+        if (profiled)                                                                                              \
+            hipExtLaunchKernelGGL(gemm_mfma<T, TA_, TB_>, dim3(grid), dim3(THREADS), 0, s, rec.a, rec.b, 0, m, n, k, \
+                                  alpha, a, lda, b, ldb, beta, c, ldc, tiles_m, tiles_n);                          \
+        else                                                                                                       \
+            gemm_mfma<T, TA_, TB_><<<grid, THREADS, 0, s>>>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tiles_m,   \
+                                                            tiles_n);                                              \
+    } while (0)
+    // Algorithmic bytes: a and b read once, c written once (and read once where beta is not 0).
+    GemmProfile::Rec rec{nullptr, nullptr, 2.0 * (double)m * (double)n * (double)k,
+                         (double)sizeof(T) * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n)};
     bool profiled = false;
     {
         std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -372,7 +385,6 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
     if (profiled) {
         HS_HIP(hipEventCreate(&rec.a));
         HS_HIP(hipEventCreate(&rec.b));
-        HS_HIP(hipEventRecord(rec.a, s));
     }
     if (!ta && !tb) HS_LAUNCH(false, false);
     else if (!ta && tb) HS_LAUNCH(false, true);
@@ -381,7 +393,6 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
 #undef HS_LAUNCH
     HS_HIP(hipGetLastError());
     if (profiled) {
-        HS_HIP(hipEventRecord(rec.b, s));
         std::lock_guard<std::mutex> lk(g_prof.mu);
         g_prof.recs.push_back(rec);
     }

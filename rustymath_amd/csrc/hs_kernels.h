@@ -53,8 +53,9 @@ int batched_free(long long handle);
 int batched_profile(int on, long long* out);
 
 // The MFMA product's live profile: on = 1 clears and starts, 0 stops; gives the summed event time (ms), the
-// number of launches and their algorithmic flops (2 m n k each) since the last call.
-int gemm_profile(int on, double* ms, int64_t* launches, double* flops);
+// number of launches, their algorithmic flops (2 m n k each) and bytes (a, b and c once each, c twice where it is
+// read) since the last call.
+int gemm_profile(int on, double* ms, int64_t* launches, double* flops, double* bytes);
 
 // The panel kernel's per-phase profile: on = 1 clears and enables, 0 disables; out[8] gets tick sums.
 int panel_profile(int on, long long* out);

@@ -137,6 +137,40 @@ def adjoint_bench(reps=3):
     return out
 
 
+def precision_variants(a, b, x64, n, reps=2):
+    """The mixed-precision workload's variants: FP32 factors with classical refinement and BF16 trailing products with
+    GMRES-based refinement, on the cond-1e6 matrix (a, b, its FP64 LU solution x64) and on an iid N(0, 1) matrix of the
+    same order (seed 4); each the best of `reps` solves, with refinement and GMRES step counts, the backward error and
+    the forward error against the plain FP64 LU solution."""
+    import torch
+    from rustymath_amd import device as dv
+    tol = 10 * n * torch.finfo(torch.float64).eps
+    flops = 2 / 3 * n ** 3 + 2 * n * n
+    g = torch.Generator(device='cuda'); g.manual_seed(4)
+    ag = torch.randn(n, n, dtype=torch.float64, device='cuda', generator=g)
+    bg = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+    fg, pg = dv.getrf(ag.clone())
+    xg = bg.reshape(n, 1).clone(); dv.getrs(fg, pg, xg); xg = xg.reshape(n)
+    del fg
+    out = {}
+    for name, (A, B, X) in (('cond1e6', (a, b, x64)), ('gaussian', (ag, bg, xg))):
+        for prec in ('fp32', 'bf16'):
+            best, res = math.inf, None
+            for _ in range(reps):
+                torch.cuda.synchronize(); t0 = time.perf_counter()
+                res = dv.dsgesv(A, B, tol=tol) if prec == 'fp32' else dv.dbgesv(A, B, tol=tol)
+                torch.cuda.synchronize(); best = min(best, time.perf_counter() - t0)
+            x = res[0]
+            line = {'ms': round(best * 1e3, 3), 'gflops': round(flops / best / 1e9, 1), 'refinement_steps': res[1],
+                    'backward_error': res[-1], 'forward_error_vs_fp64_lu': float((x - X).norm() / X.norm())}
+            if prec == 'bf16': line['gmres_steps'] = res[2]
+            out[f'{name}_{prec}'] = line
+    out['workload'] = (f'n={n}: the cond-1e6 matrix of the headline and an iid N(0,1) matrix (seed 4); fp32 = FP32 LU + '
+                       'classical FP64 refinement, bf16 = FP32 panels with BF16-input FP32-accumulate trailing products + '
+                       'FP64 GMRES-based refinement (restart 40); tolerance 10 n eps64')
+    return out
+
+
 def plain_fp64_bench(n=8192, k=16, reps=3):
     """The plain FP64 workload on the GPU: matmul of two n×n float64 N(0, 1) matrices (2 n³ flops), and the LU solve
     of an n×n float64 N(0, 1) system with k right-hand sides (2/3 n³ + 2 n² k flops: copy A, factor, solve), each the
@@ -302,8 +336,9 @@ def main():
              'column_loop_ms': round(panel_s * 1e3, 3), 'us_per_column': round(panel_s / n * 1e6, 3),
              'phases_ms': {k: round(v / 1e5, 3) for k, v in zip(('reduce', 'publish_update', 'exchange', 'barrier',
                                                                   'pivot_update'), ticks[:5])}}
-    batched = adj = plain = None
+    batched = adj = plain = variants = None
     if not args.only_mixed:
+        variants = precision_variants(a, b, x64, n)
         batched = batched_bench()
         adj = adjoint_bench()
         plain = plain_fp64_bench()
@@ -316,7 +351,8 @@ def main():
                                f's = logspace(0, -6)), cond 1e6, b N(0,1), seed 3', 'n': n, 'rhs': 1},
         'fp64_lu_solve_gflops': round(flops / t64 / 1e9, 2), 'speedup_vs_fp64_lu': round(t64 / t, 3),
         'refinement_solves': iters, 'backward_error': berr, 'backward_error_fp64_lu': r64, 'tolerance': tol,
-        'forward_error_vs_fp64_lu': fwd, 'roofline': roof, 'panel': panel, 'batched': batched, 'adjoint': adj, 'plain_fp64': plain,
+        'forward_error_vs_fp64_lu': fwd, 'roofline': roof, 'panel': panel, 'precision_variants': variants,
+        'batched': batched, 'adjoint': adj, 'plain_fp64': plain,
     }
     if not args.no_cpu and not args.only_mixed:
         out['cpu_baseline'] = cpu_baseline()

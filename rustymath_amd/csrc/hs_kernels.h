@@ -23,8 +23,11 @@ int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t ld
 // diagonal (unit diagonal implicit), U on and above; pivots[c] = the row swapped with row c (absolute,
 // 0-based, LAPACK ipiv style). The arithmetic is rustymath's lu_factor (core/src/linalg/factor.rs:617),
 // item for item. `info` (device, may be null) gets 0 or 1 + the first zero pivot.
+// bf16_trailing (FP32 only): the trailing products after each 256-column block take their operands rounded to
+// bfloat16 (gemm_bf16_update); panels, row moves and the rows of U stay FP32. Not the reference's arithmetic: the
+// low-precision factors of the BF16 mixed solve.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s);
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf16_trailing = false);
 
 // Solves with factors from getrf: x = U⁻¹ L⁻¹ P b in place on b (n×k, ld ldb, row-major), with the
 // reference's substitution orders (factor.rs:633-641, :668-748). `trans` solves Aᵀ x = b instead
@@ -36,6 +39,11 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
 // Mixed-precision solve: FP32 factors on MFMA, FP64 iterative refinement on the GPU (ir.hip).
 int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol,
             int* iters, double* berr, hipStream_t s);
+// Mixed-precision solve with BF16 trailing products: FP32 panels and BF16-input, FP32-accumulate products, then
+// FP64 GMRES-based refinement (GMRES on A d = r, left-preconditioned by the low-precision LU factors, restarted every
+// `restart` steps) until the backward error reaches tol (ir.hip). iters: refinement steps; inner: GMRES steps.
+int gesv_gmres_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, int restart,
+                  double tol, int* iters, int* inner, double* berr, hipStream_t s);
 
 // Batched solves (batched.hip): a plan for `count` systems with orders n[i] <= 512 and k[i] <= 8 right-hand
 // sides, A_i row-major at a + a_off[i], B_i row-major n×k at b + b_off[i] (overwritten by X_i where regular),

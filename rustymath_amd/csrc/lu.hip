@@ -558,7 +558,7 @@ int panel_profile(int on, long long* out) {
 // the trailing matrix takes one product with k up to 256, which reads and writes it a quarter as often as per-panel
 // updates would. Every item still takes its terms in order of the pivot column, one fused multiply-add each.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf16_trailing) {
     if (n == 0) return OK;
     if (n > (int64_t)MAX_WG * PT) { set_error("n above 16384: the panel kernel holds at most 32 x 512 rows"); return -1; }
     PanelProfile* st;
@@ -645,9 +645,17 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
                              T(1), a + c0 * lda + B1, lda, s);
             if (rc == OK) rc = trsm(c0, (int)(c1 - c0), B1, n);
         }
-        if (rc == OK)
+        if (rc == OK) {
+            if constexpr (std::is_same<T, float>::value) {
+                if (bf16_trailing) {
+                    rc = gemm_bf16_update(n - B1, n - B1, B1 - B0, a + B1 * lda + B0, lda, a + B0 * lda + B1, lda,
+                                          a + B1 * lda + B1, lda, s);
+                    continue;
+                }
+            }
             rc = gemm<T>(false, false, n - B1, n - B1, B1 - B0, T(-1), a + B1 * lda + B0, lda, a + B0 * lda + B1, lda, T(1),
                          a + B1 * lda + B1, lda, s);
+        }
     }
     if (rc == OK) {
         dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);
@@ -669,8 +677,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s) {
     return rc;
 }
 
-template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t);
-template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t);
+template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t, bool);
+template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t, bool);
 
 template <typename T>
 int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s) {

@@ -79,6 +79,38 @@ def dsgesv(a, b, max_iter=30, tol=None):
     return x, it.value, be.value
 
 
+def dbgesv(a, b, max_iter=30, restart=40, tol=None):
+    """Solves A x = b for the f64 matrix `a` (n×n, compact) and vector `b` with BF16 trailing products in the
+    factorization (FP32 panels, FP32 accumulation on MFMA) and FP64 GMRES-based refinement preconditioned by those
+    factors. Returns (x, refinement steps, GMRES steps, backward error); `tol` defaults to 10·n·eps64."""
+    import ctypes
+    _mat(a, 'a')
+    n = a.shape[0]
+    if a.dtype != torch.float64 or b.dtype != torch.float64: raise TypeError('dbgesv takes float64')
+    b = b.contiguous().reshape(n)
+    x = torch.empty(n, dtype=torch.float64, device=a.device)
+    if tol is None: tol = 10 * n * torch.finfo(torch.float64).eps
+    it, inner, be = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
+    _lib.check(_lib.load().hs_dev_dbgesv(n, a.data_ptr(), a.stride(0), b.data_ptr(), x.data_ptr(), max_iter, restart,
+                                         tol, ctypes.byref(it), ctypes.byref(inner), ctypes.byref(be), _stream()),
+               'dbgesv')
+    return x, it.value, inner.value, be.value
+
+
+def gemm_bf16_update(c, a, b):
+    """c -= bf16(a) bf16(b) in place for FP32 tensors (the BF16 factorization's trailing product): a and b rounded to
+    bfloat16 (nearest even), products summed in FP32 on MFMA."""
+    for t, name in ((a, 'a'), (b, 'b'), (c, 'c')):
+        _mat(t, name)
+        if t.dtype != torch.float32: raise TypeError('gemm_bf16_update takes float32')
+    m, k = a.shape
+    if b.shape[0] != k or c.shape != (m, b.shape[1]): raise ValueError('gemm_bf16_update: shapes do not match')
+    _lib.check(_lib.load().hs_dev_sgemm_bf16_update(m, b.shape[1], k, a.data_ptr(), a.stride(0), b.data_ptr(),
+                                                    b.stride(0), c.data_ptr(), c.stride(0), _stream()),
+               'gemm_bf16_update')
+    return c
+
+
 class BatchedSolve:
     """Many independent systems A_i X_i = B_i (orders n_i <= 512, k_i <= 8 right-hand sides) solved by one persistent
     launch, replayed from a hipGraph: the cells of a rustymath stack, each solved as `LuWork::solve` would

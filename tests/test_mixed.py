@@ -1,4 +1,5 @@
-"""GPU checks of the mixed-precision solve (hs_dev_dsgesv): FP32 LU on MFMA, FP64 iterative refinement on the GPU.
+"""GPU checks of the mixed-precision solves: hs_dev_dsgesv (FP32 LU on MFMA, FP64 iterative refinement on the GPU) and
+hs_dev_dbgesv (BF16 trailing products, FP64 GMRES-based refinement), and of the BF16 trailing product itself.
 
 There is no reference function to match (rustymath has no refinement), so the checks are the contract the solve
 states: the normwise backward error ||b − A x|| / (||A||_F ||x||) that it reports is the true one (recomputed
@@ -83,3 +84,47 @@ def test_low_precision_factors_are_the_oracle_fp32_lu(dv):
     wf, wperm, _ = oracle.lu(a32.cpu().numpy())
     assert (perm.cpu().numpy() == wperm).all()
     assert bits_equal(f.cpu().numpy(), wf.reshape(600, 600))
+
+
+@pytest.mark.parametrize('m,n,k', [(1, 1, 1), (130, 200, 70), (257, 129, 300), (512, 512, 256), (64, 1000, 33)])
+def test_bf16_update_is_the_rounded_product_summed_in_fp32(dv, m, n, k):
+    """c -= bf16(a) bf16(b): against the same product of the rounded operands in FP64 on the CPU, within FP32
+    accumulation's bound k·u32·(|a'||b'|) (a', b' the bfloat16 operands; u32 = 2^-24) plus the final rounding of c."""
+    import torch
+    g = torch.Generator(device='cuda'); g.manual_seed(m * 7 + n * 3 + k)
+    a = torch.randn(m, k, device='cuda', generator=g)
+    b = torch.randn(k, n, device='cuda', generator=g)
+    c = torch.randn(m, n, device='cuda', generator=g)
+    ar, br = a.bfloat16().double().cpu(), b.bfloat16().double().cpu()
+    want = c.double().cpu() - ar @ br
+    got = dv.gemm_bf16_update(c.clone(), a, b).double().cpu()
+    bound = (k + 2) * 2.0 ** -24 * ((ar.abs() @ br.abs()) + c.double().cpu().abs())
+    assert ((got - want).abs() <= bound).all(), float(((got - want).abs() / bound).max())
+
+
+@pytest.mark.parametrize('n', [2048])
+def test_bf16_gmres_refinement_reaches_fp64_accuracy(dv, n):
+    """The BF16 path on the cond-1e6 Householder matrix: classical refinement cannot converge from BF16 factors here
+    (cond · 2^-8 >> 1); GMRES-based refinement must reach 10·n·eps64, with the reported backward error the true one
+    and the solution as close to the plain FP64 LU solution as two backward-stable solves allow."""
+    import torch
+    from bench import householder_matrix
+    a, b = householder_matrix(n, seed=3)
+    tol = 10 * n * EPS
+    x, iters, inner, berr = dv.dbgesv(a, b, tol=tol)
+    assert berr <= tol and 1 <= iters <= 30 and inner >= iters, (iters, inner, berr)
+    assert abs(true_backward_error(a, x, b) - berr) <= 1e-3 * berr
+    x64 = fp64_solution(dv, a, b)
+    assert float((x - x64).norm() / x64.norm()) <= 2 * 1e6 * tol
+    torch.cuda.synchronize()
+
+
+def test_bf16_gmres_refinement_on_a_gaussian_system(dv):
+    import torch
+    n = 4096
+    g = torch.Generator(device='cuda'); g.manual_seed(1)
+    a = torch.randn(n, n, dtype=torch.float64, device='cuda', generator=g)
+    b = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+    x, iters, inner, berr = dv.dbgesv(a, b)
+    assert berr <= 10 * n * EPS, (iters, inner, berr)
+    assert abs(true_backward_error(a, x, b) - berr) <= 1e-3 * berr

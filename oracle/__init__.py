@@ -38,6 +38,7 @@ def lib():
                 f.argtypes = [vp, i64, vp, vp, vp]
                 f.restype = i32
             f = getattr(_lib, 'or_lu_solve_' + s); f.argtypes = [vp, vp, i64, vp, i64, vp]; f.restype = i32
+            f = getattr(_lib, 'or_lu_solve_t_' + s); f.argtypes = [vp, vp, i64, vp, i64, vp]; f.restype = i32
             f = getattr(_lib, 'or_lu_inverse_' + s); f.argtypes = [vp, vp, i64, vp]; f.restype = i32
             f = getattr(_lib, 'or_det_' + s); f.argtypes = [vp, vp, i64]; f.restype = t
             f = getattr(_lib, 'or_slogdet_' + s); f.argtypes = [vp, vp, i64, vp, vp]; f.restype = None
@@ -114,6 +115,19 @@ def lu_solve(factors, perm, b):
     perm = np.ascontiguousarray(perm, dtype=np.int64)
     x = np.zeros((n, k), factors.dtype)
     st = getattr(lib(), 'or_lu_solve_' + s)(_p(factors), _p(perm), n, _p(b2), k, _p(x))
+    if st: raise IndexError('row positions out of range')
+    return x[:, 0] if vec else x
+
+
+def lu_solve_t(factors, perm, b):
+    """Aᵀ x = b with the factors of A (the adjoint's solve, no counterpart in the reference): Uᵀ in increasing order with
+    `divide`'s diagonal scaling, Lᵀ in decreasing order, then the inverse row permutation."""
+    factors = np.ascontiguousarray(factors); s, n = _suffix(factors.dtype), factors.shape[0]
+    b = np.asarray(b, dtype=factors.dtype); vec = b.ndim == 1
+    b2 = np.ascontiguousarray(b.reshape(n, 1 if vec else b.shape[1])); k = b2.shape[1]
+    perm = np.ascontiguousarray(perm, dtype=np.int64)
+    x = np.zeros((n, k), factors.dtype)
+    st = getattr(lib(), 'or_lu_solve_t_' + s)(_p(factors), _p(perm), n, _p(b2), k, _p(x))
     if st: raise IndexError('row positions out of range')
     return x[:, 0] if vec else x
 

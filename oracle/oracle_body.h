@@ -454,6 +454,32 @@ int F(or_lu_solve_)(const T* factors, const int64_t* perm, int64_t n, const T* b
     return 0;
 }
 
+/* The transposed solve Aᵀ x = b with rustymath's LU factors (no counterpart in the reference, which has no
+ * gradients; it is the adjoint's solve): Aᵀ = Uᵀ Lᵀ P, so Uᵀ y = b in increasing order of the terms, each row then
+ * scaled by its diagonal as `divide` does (factor.rs:506); Lᵀ z = y in decreasing order; x[perm[i]] = z[i]. The
+ * terms are fused multiply-adds x_q · (−entry) + acc, as the substitutions above take them. */
+int F(or_lu_solve_t_)(const T* factors, const int64_t* perm, int64_t n, const T* b, int64_t k, T* x) {
+    for (int64_t i = 0; i < n; ++i) if (perm[i] < 0 || perm[i] >= n) return -2;
+    T* w = (T*)malloc(sizeof(T) * (size_t)(n * k > 0 ? n * k : 1));
+    memcpy(w, b, sizeof(T) * (size_t)(n * k));
+    for (int64_t c = 0; c < k; ++c) {
+        for (int64_t i = 0; i < n; ++i) {  /* Uᵀ: entry (i, q) = U(q, i) = factors[i n + q] */
+            T acc = w[i * k + c];
+            for (int64_t q = 0; q < i; ++q) acc = FMA(w[q * k + c], -factors[i * n + q], acc);
+            F(divide_)(&acc, 1, 1, factors[i * n + i]);
+            w[i * k + c] = acc;
+        }
+        for (int64_t i = n - 1; i >= 0; --i) {  /* Lᵀ: entry (i, q) = L(q, i) = factors[i n + q], unit diagonal */
+            T acc = w[i * k + c];
+            for (int64_t q = n - 1; q > i; --q) acc = FMA(w[q * k + c], -factors[i * n + q], acc);
+            w[i * k + c] = acc;
+        }
+    }
+    for (int64_t i = 0; i < n; ++i) memcpy(x + perm[i] * k, w + i * k, sizeof(T) * (size_t)k);
+    free(w);
+    return 0;
+}
+
 /* `lu_inverse`, factor.rs:645-664, rustymath's own kernels: out is row-major n×n. */
 int F(or_lu_inverse_)(const T* factors, const int64_t* perm, int64_t n, T* out) {
     int64_t ld = F(stride_)(n);

@@ -334,7 +334,7 @@ __global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ 
 
 template <typename T>
 int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
-         T beta, T* c, int64_t ldc, hipStream_t s) {
+         T beta, T* c, int64_t ldc, hipStream_t s, bool in_order) {
     if (m == 0 || n == 0) return OK;
     if (k == 0 || alpha == T(0)) {
         if (beta == T(1)) return OK;
@@ -343,7 +343,7 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
         HS_HIP(hipGetLastError());
         return OK;
     }
-    if (n == 1 || m == 1) {
+    if ((n == 1 || m == 1) && !in_order) {
         // As by_vector: for n = 1 the matrix is a and the vector b's column; for m = 1 the matrix is bᵀ.
         const T* mat; int64_t ld, rows, step, vstride, mlen; int mtrans; const T* vec;
         if (n == 1) {
@@ -400,8 +400,8 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
 }
 
 template int gemm<double>(bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*, int64_t,
-                          double, double*, int64_t, hipStream_t);
+                          double, double*, int64_t, hipStream_t, bool);
 template int gemm<float>(bool, bool, int64_t, int64_t, int64_t, float, const float*, int64_t, const float*, int64_t,
-                         float, float*, int64_t, hipStream_t);
+                         float, float*, int64_t, hipStream_t, bool);
 
 }  // namespace hs

@@ -9,10 +9,11 @@ namespace hs {
 // Tiled::gemm (core/src/linalg.rs:772, :799-807): c = alpha op(a) op(b) + beta c, with the reference's
 // routing: nothing for m or n = 0, a scaling for k = 0 or alpha = 0, `by_vector` for m or n = 1
 // (:1020-1073) and the MFMA kernel otherwise. Each result starts from 0 or beta c and takes alpha a b
-// in order of p, one fused multiply-add per step (:1353-1426).
+// in order of p, one fused multiply-add per step (:1353-1426). in_order keeps every shape on the MFMA kernel (no
+// `by_vector` orders for m or n = 1), for the library's own blocked substitutions.
 template <typename T>
 int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda,
-         const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s);
+         const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s, bool in_order = false);
 
 // BF16-input, FP32-accumulate product for the low-precision factorization: c (f32) = c - a b with a, b
 // rounded to bfloat16 once (MFMA 16x16x32 bf16).

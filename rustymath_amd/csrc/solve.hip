@@ -339,20 +339,24 @@ int trsv1_lu(int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
     return OK;
 }
 
-// One triangular pass of `mode` over all n rows of x (n×k, k <= MAXK) in place.
+// One triangular pass of `mode` over rows [lo, hi) of x (n×k, k <= MAXK) in place, taking only the terms of rows in
+// [lo, hi): the terms of rows outside it have been added already (by products, in the same order). For BWD_LEAVES,
+// [lo, hi) is a node of `backward`'s halving of [0, n), so the halving inside it is the reference's.
 template <typename T>
-int trsv_pass(int mode, int64_t n, int k, const T* f, int64_t ld, T* x, int64_t ldx, hipStream_t s) {
+int trsv_pass(int mode, int64_t lo, int64_t hi, int64_t n, int k, const T* f, int64_t ld, T* x, int64_t ldx,
+              hipStream_t s) {
     std::vector<std::pair<int, int>> blocks;  // in solve order
     const bool down = mode == FWD_UNIT || mode == FWD_DIAG;
     if (mode == BWD_LEAVES) {
         std::vector<std::pair<int, int>> lv;
-        backward_leaves(0, (int)n, lv);
+        backward_leaves((int)lo, (int)hi, lv);
         blocks.assign(lv.rbegin(), lv.rend());
     } else if (down) {
-        for (int r = 0; r < n; r += BR) blocks.push_back({r, (int)std::min<int64_t>(n, r + BR)});
+        for (int64_t r = lo; r < hi; r += BR) blocks.push_back({(int)r, (int)std::min<int64_t>(hi, r + BR)});
     } else {
-        int nb = (int)((n + BR - 1) / BR);
-        for (int q = nb - 1; q >= 0; --q) blocks.push_back({q * BR, (int)std::min<int64_t>(n, (int64_t)(q + 1) * BR)});
+        const int nb = (int)((hi - lo + BR - 1) / BR);
+        for (int q = nb - 1; q >= 0; --q)
+            blocks.push_back({(int)(lo + (int64_t)q * BR), (int)std::min<int64_t>(hi, lo + (int64_t)(q + 1) * BR)});
     }
     const int nb = (int)blocks.size();
     std::vector<int> r0(nb), r1(nb), of_row(n), seg_off(nb + 1), seg;
@@ -362,12 +366,12 @@ int trsv_pass(int mode, int64_t n, int k, const T* f, int64_t ld, T* x, int64_t 
         seg_off[b] = (int)seg.size() / 3;
         if (mode == BWD_LEAVES) {
             std::vector<std::pair<int, int>> rs;
-            backward_ranges(0, (int)n, r0[b], rs);
+            backward_ranges((int)lo, (int)hi, r0[b], rs);
             for (auto [a0, a1] : rs) { seg.push_back(a0); seg.push_back(a1); seg.push_back(1); }
         } else if (down) {
-            if (r0[b] > 0) { seg.push_back(0); seg.push_back(r0[b]); seg.push_back(1); }
+            if (r0[b] > lo) { seg.push_back((int)lo); seg.push_back(r0[b]); seg.push_back(1); }
         } else {
-            if (r1[b] < n) { seg.push_back(r1[b]); seg.push_back((int)n); seg.push_back(0); }
+            if (r1[b] < hi) { seg.push_back(r1[b]); seg.push_back((int)hi); seg.push_back(0); }
         }
     }
     seg_off[nb] = (int)seg.size() / 3;
@@ -406,6 +410,291 @@ int trsv_pass(int mode, int64_t n, int k, const T* f, int64_t ld, T* x, int64_t 
     return OK;
 }
 
+constexpr int BS = 64;  // rows of a block solve
+
+// The block's triangle [r0, r0 + m)², m <= 64, into LDS with coalesced loads: E[i][q] = entry (r0 + i, r0 + q), which
+// is f[i ld + q] for the transposed modes (rows of F) and f[q ld + i] otherwise; zero outside the block.
+template <typename T, int NT>
+__device__ __forceinline__ void stage_triangle(bool tmode, int64_t r0, int m, const T* __restrict__ f, int64_t ld,
+                                               T (*E)[BS + 1]) {
+    const int t = threadIdx.x;
+    T ev[BS * BS / NT];
+#pragma unroll
+    for (int e = 0; e < BS * BS / NT; ++e) {
+        const int idx = t + NT * e, i = tmode ? idx / BS : idx % BS, q = tmode ? idx % BS : idx / BS;
+        const int64_t ii = r0 + (i < m ? i : m - 1), qq = r0 + (q < m ? q : m - 1);
+        ev[e] = tmode ? f[ii * ld + qq] : f[qq * ld + ii];
+    }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int e = 0; e < BS * BS / NT; ++e) {
+        const int idx = t + NT * e, i = tmode ? idx / BS : idx % BS, q = tmode ? idx % BS : idx / BS;
+        E[i][q] = (i < m && q < m) ? ev[e] : T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T smallest_normal() {
+    return sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+}
+
+// A block of at most 64 rows [r0, r0 + m) solving among themselves in a right-looking sweep, k <= 64 columns of x:
+// lane i holds row r0 + i and its row of the triangle in registers; at step t row t is final, its value goes to every
+// lane by readlane, and the rows it feeds add its term. FWD_UNIT and FWD_DIAG step t = 0, 1, ... (each row's terms in
+// increasing order; FWD_DIAG scales a row by its diagonal as `divide` does, a reciprocal multiply or, for a subnormal
+// diagonal, a quotient, once its terms are in); BWD_UNIT steps from the bottom (terms in decreasing order). Eight
+// waves, two columns each at a time, so that two chains overlap in every wave.
+constexpr int SW_WAVES = 8, SW_NC = 2;
+template <typename T, int MODE>
+__global__ __launch_bounds__(64 * SW_WAVES) void sweep_kernel(int64_t r0, int m, int k, const T* __restrict__ f,
+                                                              int64_t ld, T* x, int64_t ldx) {
+    __shared__ T E[BS][BS + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    stage_triangle<T, 64 * SW_WAVES>(MODE != FWD_UNIT, r0, m, f, ld, E);
+    __syncthreads();
+    T e[BS];
+#pragma unroll
+    for (int q = 0; q < BS; ++q) e[q] = E[lane][q];
+    const T d = E[lane][lane < m ? lane : 0];
+    const int nrm = fabs(d) >= smallest_normal<T>();
+    const T ri = T(1) / d;
+    auto scale = [&](T v) { return v * ri; };
+    for (int c0 = wave * SW_NC; c0 < k; c0 += SW_WAVES * SW_NC) {
+        int me = lane;
+        asm volatile("" : "+v"(me));  // the step masks come from this copy, so they are not all kept across passes
+        T xv[SW_NC];
+#pragma unroll
+        for (int j = 0; j < SW_NC; ++j) {
+            const int c = c0 + j < k ? c0 + j : k - 1;
+            const T v = x[(r0 + (lane < m ? lane : m - 1)) * ldx + c];
+            xv[j] = lane < m ? v : T(0);
+        }
+#pragma unroll
+        for (int step = 0; step < BS; ++step) {
+            const int tt = MODE == BWD_UNIT ? BS - 1 - step : step;  // rows past m hold zeros and add nothing
+            T bc[SW_NC];
+#pragma unroll
+            for (int j = 0; j < SW_NC; ++j) {
+                T v = xv[j];
+                if (MODE == FWD_DIAG) v = __builtin_amdgcn_readlane(nrm, tt) ? scale(v) : v / d;
+                bc[j] = lane_value(v, tt);
+            }
+            if (MODE == BWD_UNIT ? me < tt : me > tt) {
+#pragma unroll
+                for (int j = 0; j < SW_NC; ++j) xv[j] = __builtin_fma(bc[j], -e[tt], xv[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SW_NC; ++j) {
+            T v = xv[j];
+            if (MODE == FWD_DIAG) v = nrm ? scale(v) : v / d;
+            if (lane < m && c0 + j < k) x[(r0 + lane) * ldx + c0 + j] = v;
+        }
+    }
+}
+
+// A node of `backward`'s halving with at most 64 rows (BWD_LEAVES). In a leaf (at most 32 rows) a row's terms come
+// in increasing order but the rows after it finish from the bottom up, so each row waits for all of them: the chains
+// are serial, one per column. Wave 0 runs them with lane c for column c (k <= 64) and the leaf's rows of x in
+// registers, the last row in the last slot, so that every row's terms are exactly the slots after it; the triangle's
+// items come from LDS (one address for all lanes) and the reciprocals of the diagonal are ready beforehand. The slots
+// before the leaf's first row compute rows that do not exist (the LDS before the triangle is a guard region for
+// them), are never stored and feed no real row, so the leaf is one straight run of code. A leaf with a subnormal
+// diagonal, whose rows take a quotient, runs a plain loop instead. Above 32 rows (factor.rs:693-707) the bottom leaf
+// goes first, then all four waves add the bottom rows' terms to the top rows in increasing order, then the top leaf.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void bwd_node_kernel(int64_t r0, int m, int k, const T* __restrict__ f, int64_t ld,
+                                                       T* x, int64_t ldx) {
+    constexpr int GUARD = BR * (BS + 1);  // covers rows and columns down to -31
+    __shared__ T Eg[GUARD + BS * (BS + 1)];
+    __shared__ T xb[BS][BS + 1];  // xb[row][column]
+    __shared__ T rdg[BR + BS];    // rdg[BR + i] = 1 / U(r0 + i, r0 + i)
+    T(*E)[BS + 1] = reinterpret_cast<T(*)[BS + 1]>(Eg + GUARD);  // E[i][q] = U(r0 + i, r0 + q) = f[(r0 + q) ld + r0 + i]
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    stage_triangle<T, 256>(false, r0, m, f, ld, E);
+    for (int idx = t; idx < BS * BS; idx += 256) {
+        const int r = idx / BS, cc = idx % BS;
+        xb[r][cc] = (r < m && cc < k) ? x[(r0 + r) * ldx + cc] : T(0);
+    }
+    __syncthreads();
+    if (t < BS) rdg[BR + t] = T(1) / E[t][t];
+    __syncthreads();
+    const T tiny = smallest_normal<T>();
+    const int split = m > BR ? (int)((2 * r0 + m) / 2 - r0) : 0;  // the node's middle, (r0 + r1) / 2, less r0
+    for (int phase = 0; phase < 2; ++phase) {  // the bottom leaf (or the only one), then the top leaf
+        const int lo = phase == 0 ? split : 0, hi = phase == 0 ? m : split;
+        if (phase == 1) {
+            if (split == 0) break;
+            T acc[BR / 4];  // rows wave, wave + 4, ... of the top half (split <= 32)
+#pragma unroll
+            for (int j = 0; j < BR / 4; ++j) acc[j] = xb[wave + 4 * j][lane];
+            for (int q = split; q < m; ++q) {
+                const T xq = xb[q][lane];
+#pragma unroll
+                for (int j = 0; j < BR / 4; ++j) acc[j] = __builtin_fma(xq, -E[wave + 4 * j][q], acc[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < BR / 4; ++j)
+                if (wave + 4 * j < split) xb[wave + 4 * j][lane] = acc[j];
+            __syncthreads();
+        }
+        if (wave == 0) {
+            const int dl = lo + (lane < hi - lo ? lane : 0);
+            if (__all(fabs(E[dl][dl]) >= tiny)) {
+                const int off = BR - (hi - lo), base = lo - off;  // slot u holds row base + u
+                T xr[BR];
+#pragma unroll
+                for (int u = 0; u < BR; ++u) xr[u] = xb[u >= off ? base + u : lo][lane];
+                // One address register for the triangle and one for the reciprocals, the slots as fixed offsets.
+                int eb = GUARD + base * (BS + 1) + base, rb = BR + base;
+                asm volatile("" : "+v"(eb), "+v"(rb));
+                // Each row's items are loaded while the row below it runs its chain.
+                T ec[BR], en[BR], rc = rdg[rb + BR - 1], rn = rc;
+#pragma unroll
+                for (int u = BR - 1; u >= 0; --u) {
+                    if (u > 0) {
+#pragma unroll
+                        for (int q = u; q < BR; ++q) en[q] = Eg[eb + (u - 1) * (BS + 1) + q];
+                        rn = rdg[rb + u - 1];
+                    }
+                    __builtin_amdgcn_sched_barrier(0);  // keep those loads ahead of this row's chain
+                    T acc = xr[u];
+#pragma unroll
+                    for (int q = u + 1; q < BR; ++q) acc = __builtin_fma(xr[q], -ec[q], acc);
+                    xr[u] = acc * rc;
+#pragma unroll
+                    for (int q = 0; q < BR; ++q) ec[q] = en[q];
+                    rc = rn;
+                }
+#pragma unroll
+                for (int u = 0; u < BR; ++u)
+                    if (u >= off) xb[base + u][lane] = xr[u];
+            } else {
+                for (int row = hi - 1; row >= lo; --row) {
+                    T acc = xb[row][lane];
+                    for (int q = row + 1; q < hi; ++q) acc = __builtin_fma(xb[q][lane], -E[row][q], acc);
+                    const T dg = E[row][row];
+                    xb[row][lane] = fabs(dg) >= tiny ? acc * rdg[BR + row] : acc / dg;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = t; idx < m * BS; idx += 256) {
+        const int r = idx / BS, cc = idx % BS;
+        if (cc < k) x[(r0 + r) * ldx + cc] = xb[r][cc];
+    }
+}
+
+// c -= A b for the substitutions' products: b and c have at most 16 columns, so one wave owns a strip of 16 rows of c
+// (one MFMA accumulator) and takes the whole of k in order, 4 terms per 16x16x4 step, eight steps' operands in flight.
+// A(i, p) is a[p lda + i] (ta) or a[i lda + p]. Padding: rows and columns past the ends read +0 (results dropped),
+// terms past k read (+0)(-0) = -0, which changes no sum.
+template <typename T, bool TA>
+__global__ __launch_bounds__(256) void gemm_skinny_kernel(int64_t m, int nc, int64_t k, const T* __restrict__ a,
+                                                          int64_t lda, const T* __restrict__ b, int64_t ldb, T* c,
+                                                          int64_t ldc) {
+    typedef typename Mfma<T>::acc Acc;
+    const int lane = threadIdx.x & 63;
+    const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (i0 >= m) return;
+    const int li = lane & 15, lk = lane >> 4;
+    Acc acc;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t r = i0 + Mfma<T>::row(lane, e);
+        const T v = c[(r < m ? r : m - 1) * ldc + (li < nc ? li : nc - 1)];
+        acc[e] = (r < m && li < nc) ? v : T(0);
+    }
+    const int64_t ai = i0 + li < m ? i0 + li : m - 1;
+    const int bc = li < nc ? li : nc - 1;
+    const bool aok = i0 + li < m, bok = li < nc;
+    constexpr int D = 8;
+    for (int64_t p0 = 0; p0 < k; p0 += 4 * D) {
+        T av[D], bv[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int64_t p = p0 + 4 * d + lk, pc = p < k ? p : k - 1;
+            av[d] = TA ? a[pc * lda + ai] : a[ai * lda + pc];
+            bv[d] = b[pc * ldb + bc];
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int64_t p = p0 + 4 * d + lk;
+            const T x = (aok && p < k) ? -av[d] : T(0);
+            const T y = p < k ? (bok ? bv[d] : T(0)) : T(-0.0);
+            acc = Mfma<T>::step(x, y, acc);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t r = i0 + Mfma<T>::row(lane, e);
+        if (r < m && li < nc) c[r * ldc + li] = acc[e];
+    }
+}
+
+template <typename T>
+int gemm_skinny(bool ta, int64_t m, int64_t nc, int64_t k, const T* a, int64_t lda, const T* b, int64_t ldb, T* c,
+                int64_t ldc, hipStream_t s) {
+    if (m == 0 || nc == 0 || k == 0) return OK;
+    const unsigned grid = (unsigned)((m + 63) / 64);
+    if (ta) gemm_skinny_kernel<T, true><<<grid, 256, 0, s>>>(m, (int)nc, k, a, lda, b, ldb, c, ldc);
+    else gemm_skinny_kernel<T, false><<<grid, 256, 0, s>>>(m, (int)nc, k, a, lda, b, ldb, c, ldc);
+    HS_HIP(hipGetLastError());
+    return OK;
+}
+
+// Substitutions over many rows as blocks of rows that solve among themselves (block_solve_kernel) and MFMA products
+// that add a finished block's terms to the rows still to solve, in order of the terms (in-order products: every item
+// takes one fused multiply-add per term, in the order the substitution would).
+template <typename T>
+int block_solve(int mode, int64_t r0, int64_t m, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
+    for (int64_t c0 = 0; c0 < k; c0 += BS) {  // at most 64 columns a launch
+        const int kc = (int)std::min<int64_t>(BS, k - c0);
+        if (mode == BWD_LEAVES) bwd_node_kernel<T><<<1, 256, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
+        else if (mode == FWD_UNIT) sweep_kernel<T, FWD_UNIT><<<1, 64 * SW_WAVES, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
+        else if (mode == FWD_DIAG) sweep_kernel<T, FWD_DIAG><<<1, 64 * SW_WAVES, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
+        else sweep_kernel<T, BWD_UNIT><<<1, 64 * SW_WAVES, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
+        HS_HIP(hipGetLastError());
+    }
+    return OK;
+}
+
+template <typename T>
+int fwd_blocked(int mode, int64_t n, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
+    for (int64_t lo = 0; lo < n; lo += BS) {
+        const int64_t hi = std::min<int64_t>(n, lo + BS);
+        const int rc = block_solve<T>(mode, lo, hi - lo, k, lu, lda, w, s);
+        if (rc) return rc;
+        if (hi < n) {
+            // Rows [hi, n) take rows [lo, hi): entry (i, q) is L(i, q) = F[q][i] (FWD_UNIT) or Uᵀ(i, q) = F[i][q].
+            const bool ta = mode == FWD_UNIT;
+            const T* a = ta ? lu + lo * lda + hi : lu + hi * lda + lo;
+            const int rc = k <= 16 ? gemm_skinny<T>(ta, n - hi, k, hi - lo, a, lda, w + lo * k, k, w + hi * k, k, s)
+                                   : gemm<T>(ta, false, n - hi, k, hi - lo, T(-1), a, lda, w + lo * k, k, T(1), w + hi * k,
+                                             k, s, true);
+            if (rc) return rc;
+        }
+    }
+    return OK;
+}
+
+// `backward`'s halving (factor.rs:693-707): the bottom half first, then the top half takes the bottom half's terms in
+// increasing order, then the top half; nodes of at most 64 rows go to one block solve.
+template <typename T>
+int bwd_halving(int64_t r0, int64_t r1, int64_t n, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
+    if (r1 - r0 <= BS) return block_solve<T>(BWD_LEAVES, r0, r1 - r0, k, lu, lda, w, s);
+    const int64_t mid = (r0 + r1) / 2;
+    int rc = bwd_halving<T>(mid, r1, n, k, lu, lda, w, s);
+    // Rows [r0, mid) take rows [mid, r1): entry (i, q) is U(i, q) = F[q][i].
+    if (rc == OK)
+        rc = k <= 16 ? gemm_skinny<T>(true, mid - r0, k, r1 - mid, lu + mid * lda + r0, lda, w + mid * k, k, w + r0 * k, k, s)
+                     : gemm<T>(true, false, mid - r0, k, r1 - mid, T(-1), lu + mid * lda + r0, lda, w + mid * k, k, T(1),
+                               w + r0 * k, k, s, true);
+    if (rc == OK) rc = bwd_halving<T>(r0, mid, n, k, lu, lda, w, s);
+    return rc;
+}
+
 }  // namespace
 
 template <typename T>
@@ -418,21 +707,21 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
     if (!trans) {
         gather_rows<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, (int)k, b, ldb, perm, w, k);
         HS_HIP(hipGetLastError());
-        if (k == 1) rc = trsv1_lu<T>(n, lu, lda, w, s);
-        for (int64_t c0 = 0; c0 < k && rc == OK && k > 1; c0 += MAXK) {
-            int kc = (int)std::min<int64_t>(MAXK, k - c0);
-            rc = trsv_pass<T>(FWD_UNIT, n, kc, lu, lda, w + c0, k, s);
-            if (rc == OK) rc = trsv_pass<T>(k == 1 ? BWD_SINGLE : BWD_LEAVES, n, kc, lu, lda, w + c0, k, s);
+        if (k == 1) {
+            rc = trsv1_lu<T>(n, lu, lda, w, s);
+        } else {
+            rc = fwd_blocked<T>(FWD_UNIT, n, k, lu, lda, w, s);
+            if (rc == OK) rc = bwd_halving<T>(0, n, n, k, lu, lda, w, s);
         }
         if (rc == OK) {
             HS_HIP(hipMemcpy2DAsync(b, ldb * sizeof(T), w, k * sizeof(T), k * sizeof(T), n, hipMemcpyDeviceToDevice, s));
         }
     } else {
         HS_HIP(hipMemcpy2DAsync(w, k * sizeof(T), b, ldb * sizeof(T), k * sizeof(T), n, hipMemcpyDeviceToDevice, s));
+        rc = fwd_blocked<T>(FWD_DIAG, n, k, lu, lda, w, s);
         for (int64_t c0 = 0; c0 < k && rc == OK; c0 += MAXK) {
-            int kc = (int)std::min<int64_t>(MAXK, k - c0);
-            rc = trsv_pass<T>(FWD_DIAG, n, kc, lu, lda, w + c0, k, s);
-            if (rc == OK) rc = trsv_pass<T>(BWD_UNIT, n, kc, lu, lda, w + c0, k, s);
+            const int kc = (int)std::min<int64_t>(MAXK, k - c0);
+            rc = trsv_pass<T>(BWD_UNIT, 0, n, n, kc, lu, lda, w + c0, k, s);
         }
         if (rc == OK) {
             scatter_rows<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, (int)k, w, k, perm, b, ldb);

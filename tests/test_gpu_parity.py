@@ -182,3 +182,22 @@ def test_singular_matrix_is_not_solved_by_lu(la):
     "factor.rs:221-226: a negligible pivot sends rustymath to least squares, which this library leaves out loudly."
     with pytest.raises(NotImplementedError):
         la.solve(np.array([[1., 2], [2, 4]]), np.array([1., 2]))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [1, 5, 100, 513, 1030])
+@pytest.mark.parametrize('k', [1, 3, 16])
+def test_solves_and_transposed_solves_match_the_oracle_at_block_sizes(dtype, n, k):
+    """Blocked substitutions: 512-row block passes joined by in-order MFMA products, for A x = b (`lu_solve`'s orders,
+    the halving of `backward` across nodes) and Aᵀ x = b (the adjoint's orders), against the oracle bit for bit."""
+    import torch
+    from rustymath_amd import device as dv
+    rng = np.random.default_rng(n * 31 + k)
+    a = rng.standard_normal((n, n)).astype(dtype)
+    b = rng.standard_normal((n, k)).astype(dtype)
+    f, perm = dv.getrf(torch.from_numpy(a).cuda())
+    wf, wperm, _ = oracle.lu(a)
+    x = dv.getrs(f, perm, torch.from_numpy(b).cuda()).cpu().numpy()
+    assert bits_equal(x, oracle.lu_solve(wf, wperm, b))
+    y = dv.getrs(f, perm, torch.from_numpy(b).cuda(), trans=True).cpu().numpy()
+    assert bits_equal(y, oracle.lu_solve_t(wf, wperm, b))

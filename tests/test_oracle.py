@@ -262,3 +262,26 @@ def test_backward_error_of_a_plain_solve():
     a = np.random.default_rng(1).standard_normal((300, 300)); b = np.random.default_rng(1).standard_normal(300)
     x = oracle.solve(a, b)
     assert backward_error(a, x, b) <= 10 * 300 * EPS[np.float64]
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_transposed_solve(dtype):
+    """The adjoint's transposed solve (no reference counterpart): Aᵀ x = b within the backward-stable bound, LAPACK's
+    solution of Aᵀ x = b within the forward bound cond(A)·10·n·eps, and exact on a scaled permutation (powers of two),
+    where every term is exact."""
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 7, 33, 130):
+        a = rng.standard_normal((n, n)).astype(dtype)
+        b = rng.standard_normal((n, 3)).astype(dtype)
+        f, perm, _ = oracle.lu(a)
+        x = oracle.lu_solve_t(f, perm, b).astype(np.float64)
+        r = np.linalg.norm(a.T.astype(np.float64) @ x - b) / (np.linalg.norm(a) * np.linalg.norm(x))
+        assert r <= 10 * n * np.finfo(dtype).eps
+        want = np.linalg.solve(a.T.astype(np.float64), b.astype(np.float64))
+        fwd = np.linalg.norm(x - want) / np.linalg.norm(want)
+        assert fwd <= 10 * n * np.finfo(dtype).eps * np.linalg.cond(a.astype(np.float64))
+    p = np.eye(6, dtype=dtype)[[3, 0, 5, 1, 2, 4]] * np.array([2, 4, 8, 0.5, 1, 16], dtype=dtype)[:, None]
+    b = rng.standard_normal((6, 2)).astype(dtype)
+    f, perm, _ = oracle.lu(p)
+    x = oracle.lu_solve_t(f, perm, b)
+    assert np.array_equal(x, np.linalg.solve(p.T.astype(np.float64), b.astype(np.float64)).astype(dtype))

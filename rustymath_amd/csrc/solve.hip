@@ -7,23 +7,23 @@
 //   U x = y, k columns   `backward` (:693): halving above NS = 32 rows, so a row takes the bottom half of
 //                        each ancestor in increasing order, outermost ancestor first, then the rows of its
 //                        own leaf after it in increasing order, then a reciprocal multiply (`divide` :506).
-// One kernel serves all of them. A workgroup owns a block of rows (a reference leaf where the order needs
-// it) and all k columns; blocks are numbered in solve order, so a block only ever waits for blocks with
-// smaller numbers, which were dispatched before it. Each row keeps one running sum per column and adds
-// the solved rows of other blocks in the order above, one fused multiply-add each, as soon as their
-// block publishes them; then the block substitutes within itself and publishes its rows.
+// The transposed solve takes Uᵀ in increasing order with `divide`'s scaling, then Lᵀ in decreasing order.
+//
+// One column (k = 1): one launch per pass, a one-wave block per 32 rows that adds earlier blocks' rows as
+// they are published (trsv1_kernel). Several columns: 64-row blocks that solve among themselves in one
+// workgroup (sweep_kernel for the increasing and the Lᵀ orders, bwd_node_kernel for a node of the halving),
+// joined by in-order MFMA products that add a finished block's terms to the rows still to solve
+// (gemm_skinny_kernel for k <= 16, the tiled product otherwise).
 #include "hs_kernels.h"
+#include <algorithm>
 #include <atomic>
-#include <vector>
 
 namespace hs {
 
 namespace {
 
-constexpr int BR = 32;         // rows per block (NS of factor.rs:82; leaves have at most this many)
-constexpr int MAXK = 16;       // right-hand sides per launch
+constexpr int BR = 32;  // NS of factor.rs:82: the halving's leaves have at most this many rows
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
-
 
 enum Mode : int {
     FWD_UNIT = 0,    // L (unit lower of F): increasing order, no division
@@ -33,131 +33,6 @@ enum Mode : int {
     BWD_UNIT = 4,    // Lᵀ (upper, unit): decreasing order, no division
 };
 
-template <typename T> struct Bits;
-template <> struct Bits<double> { typedef unsigned long long u; };
-template <> struct Bits<float> { typedef unsigned int u; };
-// Device-coherent (sc1) loads and stores of the rows that blocks publish to each other.
-template <typename T>
-__device__ __forceinline__ void put(T* p, T v) {
-    typedef typename Bits<T>::u U;
-    __hip_atomic_store(reinterpret_cast<U*>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T get(const T* p) {
-    typedef typename Bits<T>::u U;
-    U u = __hip_atomic_load(reinterpret_cast<U*>(const_cast<T*>(p)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __builtin_bit_cast(T, u);
-}
-
-// Segments: seg[seg_off[b] .. seg_off[b+1]) are (first row, end row, direction) of the other blocks'
-// rows that block b's rows add, in order; direction 1 = increasing rows, 0 = decreasing.
-// The blocks are launch indices 0, 8, 16, ... (one XCD under round-robin dispatch; the others return at
-// once): a block publishes its solved rows to x and raises its flag with device-coherent (sc1) stores, and
-// the blocks after it read both with sc1 loads. Each producer block's rows of x and the 32x32 tile of
-// matrix items that the consumer needs come in through LDS with coalesced loads.
-template <typename T>
-__global__ __launch_bounds__(BR * MAXK) void trsv_blocks(int mode, int64_t n, int k, const T* __restrict__ f, int64_t ld,
-                                                         T* x, int64_t ldx, const int* __restrict__ blk_r0,
-                                                         const int* __restrict__ blk_r1, const int* __restrict__ blk_of_row,
-                                                         const int* __restrict__ seg_off, const int* __restrict__ seg,
-                                                         unsigned long long* flags, int* status) {
-    if (blockIdx.x & 7) return;
-    const int b = blockIdx.x >> 3;
-    const int r0 = blk_r0[b], r1 = blk_r1[b], rows = r1 - r0;
-    const int tid = threadIdx.x, li = tid % BR, col = tid / BR;
-    const bool live = li < rows && col < k;
-    const int64_t i = r0 + li;
-    const bool tmode = mode == FWD_DIAG || mode == BWD_UNIT;  // entry(i, p) = f[i*ld + p]
-    __shared__ T xs[BR][MAXK];
-    __shared__ T es[BR][BR + 1];  // es[row of this block][producer row]
-    __shared__ int s_abort;
-    if (tid == 0) s_abort = 0;
-
-    T acc = live ? x[i * ldx + col] : T(0);
-    __syncthreads();
-
-    for (int s = seg_off[b]; s < seg_off[b + 1]; ++s) {
-        const int a0 = seg[3 * s], a1 = seg[3 * s + 1], inc = seg[3 * s + 2];
-        int p = inc ? a0 : a1 - 1;
-        while (inc ? p < a1 : p >= a0) {
-            const int pb = blk_of_row[p];
-            const int q0 = blk_r0[pb], q1 = blk_r1[pb];
-            const int lo = q0 > a0 ? q0 : a0, hi = q1 < a1 ? q1 : a1, cnt = hi - lo;
-            if (tid == 0) {
-                uint64_t spins = 0;
-                while (__hip_atomic_load(&flags[pb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull) {
-                    if (++spins > SPIN_LIMIT) { s_abort = 1; atomicExch(status, ERR_LIMIT); break; }
-                }
-            }
-            __syncthreads();
-            if (s_abort) return;
-            // Stage the producer's solved rows and this block's matrix items against them.
-            for (int idx = tid; idx < cnt * k; idx += blockDim.x) {
-                const int rr = idx / k, cc = idx % k;
-                xs[rr][cc] = get(&x[(int64_t)(lo + rr) * ldx + cc]);
-            }
-            for (int idx = tid; idx < rows * cnt; idx += blockDim.x) {
-                int rr, qq;
-                if (tmode) { rr = idx / cnt; qq = idx % cnt; }   // row-contiguous items
-                else { qq = idx / rows; rr = idx % rows; }       // column-contiguous items
-                es[rr][qq] = tmode ? f[(int64_t)(r0 + rr) * ld + lo + qq] : f[(int64_t)(lo + qq) * ld + r0 + rr];
-            }
-            __syncthreads();
-            if (live) {
-                if (inc)
-                    for (int q = 0; q < cnt; ++q) acc = __builtin_fma(xs[q][col], -es[li][q], acc);
-                else
-                    for (int q = cnt - 1; q >= 0; --q) acc = __builtin_fma(xs[q][col], -es[li][q], acc);
-            }
-            __syncthreads();
-            p = inc ? hi : lo - 1;
-        }
-    }
-
-    // Substitution within the block, with its own triangle staged in LDS.
-    for (int idx = tid; idx < rows * rows; idx += blockDim.x) {
-        int rr, qq;
-        if (tmode) { rr = idx / rows; qq = idx % rows; }
-        else { qq = idx / rows; rr = idx % rows; }
-        es[rr][qq] = tmode ? f[(int64_t)(r0 + rr) * ld + r0 + qq] : f[(int64_t)(r0 + qq) * ld + r0 + rr];
-    }
-    __syncthreads();
-    const bool down = mode == FWD_UNIT || mode == FWD_DIAG;
-    const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
-    if (mode == BWD_LEAVES) {
-        // Row r1-1 first; each row adds the rows after it in increasing order, then divides.
-        for (int t = rows - 1; t >= 0; --t) {
-            if (live && li == t) {
-                for (int q = t + 1; q < rows; ++q) acc = __builtin_fma(xs[q][col], -es[li][q], acc);
-                const T d = es[li][li];
-                acc = fabs(d) >= tiny ? acc * (T(1) / d) : acc / d;
-                xs[t][col] = acc;
-            }
-            __syncthreads();
-        }
-    } else {
-        for (int st = 0; st < rows; ++st) {
-            const int t = down ? st : rows - 1 - st;
-            if (live && li == t) {
-                const T d = es[li][li];
-                if (mode == BWD_SINGLE) acc = acc / d;
-                else if (mode == FWD_DIAG) acc = fabs(d) >= tiny ? acc * (T(1) / d) : acc / d;
-                xs[t][col] = acc;
-            }
-            __syncthreads();
-            const bool after = down ? li > t : li < t;
-            if (live && after) acc = __builtin_fma(xs[t][col], -es[li][t], acc);
-        }
-    }
-    if (live) put(&x[i * ldx + col], acc);
-    __syncthreads();  // every store of this block has completed (s_waitcnt before the barrier)
-    if (tid == 0) __hip_atomic_store(&flags[b], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void fill_u64(unsigned long long* p, int64_t n, unsigned long long v) {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx < n) p[idx] = v;
-}
 
 template <typename T>
 __global__ void gather_rows(int64_t n, int k, const T* __restrict__ b, int64_t ldb, const int64_t* __restrict__ perm, T* x, int64_t ldx) {
@@ -173,19 +48,6 @@ __global__ void scatter_rows(int64_t n, int k, const T* __restrict__ w, int64_t 
     if (idx >= n * k) return;
     int64_t r = idx / k, c = idx % k;
     x[perm[r] * ldx + c] = w[r * ldw + c];
-}
-
-// The bottom-half ranges of `backward`'s halving (factor.rs:693-707) that the rows of the leaf
-// [l0, l1) add before their own leaf, outermost first.
-void backward_ranges(int r0, int r1, int l0, std::vector<std::pair<int, int>>& out) {
-    if (r1 - r0 <= BR) return;
-    int mid = (r0 + r1) / 2;
-    if (l0 < mid) { out.push_back({mid, r1}); backward_ranges(r0, mid, l0, out); }
-    else backward_ranges(mid, r1, l0, out);
-}
-void backward_leaves(int r0, int r1, std::vector<std::pair<int, int>>& out) {
-    if (r1 - r0 > BR) { int mid = (r0 + r1) / 2; backward_leaves(r0, mid, out); backward_leaves(mid, r1, out); return; }
-    out.push_back({r0, r1});
 }
 
 // One right-hand side (the refinement's corrections, single-column lu_solve): one wave per 32-row block, in
@@ -233,30 +95,37 @@ __device__ __forceinline__ T lane_value(T v, int l) {
 
 constexpr int B1 = 32;
 
+// One column, one pass: a one-wave block per 32 rows (lane i holds row r0 + i), blocks numbered in solve order. A block
+// adds each earlier block's rows as soon as they are published (tagged words, so a stale word never matches), its
+// tiles of the matrix loaded one producer ahead, then substitutes within itself and publishes. Orders: FWD_UNIT (L,
+// increasing), BWD_SINGLE (U, decreasing, then a quotient: `single`, factor.rs:725), and for the transposed solve
+// FWD_DIAG (Uᵀ, increasing, then `divide`'s reciprocal multiply) and BWD_UNIT (Lᵀ, decreasing). The transposed modes
+// read rows of F where the others read columns.
 template <typename T>
 __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T* __restrict__ f, int64_t ld, T* x,
                                                    unsigned long long* pub, unsigned long long epoch, int* status) {
     const int nb = (int)((n + B1 - 1) / B1), b = blockIdx.x, lane = threadIdx.x;
-    const bool fwd = mode == FWD_UNIT;
+    const bool fwd = mode == FWD_UNIT || mode == FWD_DIAG, rowwise = mode == FWD_DIAG || mode == BWD_UNIT;
     const int q = fwd ? b : nb - 1 - b;  // this block's rows [q B1, ...)
     const int64_t r0 = (int64_t)q * B1;
     const int h = (int)(n - r0 < B1 ? n - r0 : B1);
     const bool act = lane < h;
     const int64_t i = r0 + (act ? lane : 0);
     T acc = act ? x[i] : T(0);
-    // Tile of producer block pq: e[j] = F(i, pq B1 + j) (column-major factors), padded with zeros.
+    // Tile of producer block pq: e[j] = entry (i, pq B1 + j), which is F[pq B1 + j][i] or, row-wise, F[i][pq B1 + j];
+    // padded with zeros.
     auto load_tile = [&](int pq, T* e) {
         const int64_t p0 = (int64_t)pq * B1;
 #pragma unroll
         for (int j = 0; j < B1; ++j) {
             const int64_t pc = p0 + j < n ? p0 + j : n - 1;
-            const T v = f[pc * ld + i];
+            const T v = rowwise ? f[i * ld + pc] : f[pc * ld + i];
             e[j] = p0 + j < n ? v : T(0);
         }
     };
     const int np = fwd ? q : nb - 1 - q;
     T e[B1], en[B1], d[B1];
-    load_tile(q, d);  // the diagonal tile d[j] = F(i, r0 + j), off the critical path
+    load_tile(q, d);  // the diagonal tile, off the critical path
     if (np > 0) load_tile(fwd ? 0 : nb - 1, e);
     for (int t = 0; t < np; ++t) {
         const int pq = fwd ? t : nb - 1 - t;
@@ -285,22 +154,25 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
         for (int j = 0; j < B1; ++j) e[j] = en[j];
     }
     // Within the block.
+    T dg = T(1);
+#pragma unroll
+    for (int j = 0; j < B1; ++j) if (j == lane) dg = d[j];
     if (fwd) {
+        const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+        const T rdg = T(1) / dg;
 #pragma unroll
         for (int j = 0; j < B1; ++j) {
             if (j < h) {
+                if (mode == FWD_DIAG && lane == j) acc = fabs(dg) >= tiny ? acc * rdg : acc / dg;
                 const T xj = lane_value(acc, j);
                 if (lane > j) acc = __builtin_fma(xj, -d[j], acc);
             }
         }
     } else {
-        T dg = T(1);
-#pragma unroll
-        for (int j = 0; j < B1; ++j) if (j == lane) dg = d[j];
 #pragma unroll
         for (int j = B1 - 1; j >= 0; --j) {
             if (j < h) {
-                if (lane == j) acc = acc / dg;
+                if (mode == BWD_SINGLE && lane == j) acc = acc / dg;
                 const T xj = lane_value(acc, j);
                 if (lane < j) acc = __builtin_fma(xj, -d[j], acc);
             }
@@ -316,9 +188,10 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
 // match (the published words are also cleared before each pass).
 std::atomic<unsigned long long> g_epoch1{1};
 
-// L then U (lu_solve's order for one column) in place on x (n items, contiguous), with one status check.
+// L then U (lu_solve's order for one column) in place on x (n items, contiguous), or Uᵀ then Lᵀ (trans), with one
+// status check.
 template <typename T>
-int trsv1_lu(int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
+int trsv1_lu(bool trans, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
     const int nb = (int)((n + B1 - 1) / B1);
     const size_t words = (size_t)n * (sizeof(T) == 8 ? 2 : 1);
     char* buf = (char*)pool_get(words * 8 + 64);
@@ -327,80 +200,9 @@ int trsv1_lu(int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
     int* status = (int*)(buf + words * 8);
     HS_HIP(hipMemsetAsync(buf, 0, words * 8 + sizeof(int), s));
     const unsigned long long e1 = (g_epoch1.fetch_add(2) % 0xfffffff0ull) | 1ull;  // odd, and e1 + 1 < 2^32
-    trsv1_kernel<T><<<nb, 64, 0, s>>>(FWD_UNIT, n, f, ld, x, pub, e1, status);
+    trsv1_kernel<T><<<nb, 64, 0, s>>>(trans ? FWD_DIAG : FWD_UNIT, n, f, ld, x, pub, e1, status);
     HS_HIP(hipGetLastError());
-    trsv1_kernel<T><<<nb, 64, 0, s>>>(BWD_SINGLE, n, f, ld, x, pub, e1 + 1, status);
-    HS_HIP(hipGetLastError());
-    int st = 0;
-    HS_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
-    HS_HIP(hipStreamSynchronize(s));
-    pool_put(buf);
-    if (st) { set_error("triangular solve wait ran out"); return ERR_LIMIT; }
-    return OK;
-}
-
-// One triangular pass of `mode` over rows [lo, hi) of x (n×k, k <= MAXK) in place, taking only the terms of rows in
-// [lo, hi): the terms of rows outside it have been added already (by products, in the same order). For BWD_LEAVES,
-// [lo, hi) is a node of `backward`'s halving of [0, n), so the halving inside it is the reference's.
-template <typename T>
-int trsv_pass(int mode, int64_t lo, int64_t hi, int64_t n, int k, const T* f, int64_t ld, T* x, int64_t ldx,
-              hipStream_t s) {
-    std::vector<std::pair<int, int>> blocks;  // in solve order
-    const bool down = mode == FWD_UNIT || mode == FWD_DIAG;
-    if (mode == BWD_LEAVES) {
-        std::vector<std::pair<int, int>> lv;
-        backward_leaves((int)lo, (int)hi, lv);
-        blocks.assign(lv.rbegin(), lv.rend());
-    } else if (down) {
-        for (int64_t r = lo; r < hi; r += BR) blocks.push_back({(int)r, (int)std::min<int64_t>(hi, r + BR)});
-    } else {
-        const int nb = (int)((hi - lo + BR - 1) / BR);
-        for (int q = nb - 1; q >= 0; --q)
-            blocks.push_back({(int)(lo + (int64_t)q * BR), (int)std::min<int64_t>(hi, lo + (int64_t)(q + 1) * BR)});
-    }
-    const int nb = (int)blocks.size();
-    std::vector<int> r0(nb), r1(nb), of_row(n), seg_off(nb + 1), seg;
-    for (int b = 0; b < nb; ++b) {
-        r0[b] = blocks[b].first; r1[b] = blocks[b].second;
-        for (int r = r0[b]; r < r1[b]; ++r) of_row[r] = b;
-        seg_off[b] = (int)seg.size() / 3;
-        if (mode == BWD_LEAVES) {
-            std::vector<std::pair<int, int>> rs;
-            backward_ranges((int)lo, (int)hi, r0[b], rs);
-            for (auto [a0, a1] : rs) { seg.push_back(a0); seg.push_back(a1); seg.push_back(1); }
-        } else if (down) {
-            if (r0[b] > lo) { seg.push_back((int)lo); seg.push_back(r0[b]); seg.push_back(1); }
-        } else {
-            if (r1[b] < hi) { seg.push_back(r1[b]); seg.push_back((int)hi); seg.push_back(0); }
-        }
-    }
-    seg_off[nb] = (int)seg.size() / 3;
-    if (seg.empty()) { seg.push_back(0); seg.push_back(0); seg.push_back(1); }
-
-    size_t ibytes = sizeof(int) * (2 * (size_t)nb + (size_t)n + nb + 1 + seg.size());
-    char* buf = (char*)pool_get(ibytes + sizeof(unsigned long long) * nb + sizeof(int) + 64);
-    if (!buf) return hip_status(hipErrorOutOfMemory, "pool_get");
-    std::vector<int> host;
-    host.reserve(ibytes / sizeof(int));
-    host.insert(host.end(), r0.begin(), r0.end());
-    host.insert(host.end(), r1.begin(), r1.end());
-    host.insert(host.end(), of_row.begin(), of_row.end());
-    host.insert(host.end(), seg_off.begin(), seg_off.end());
-    host.insert(host.end(), seg.begin(), seg.end());
-    int* d = (int*)buf;
-    HS_HIP(hipMemcpyAsync(d, host.data(), ibytes, hipMemcpyHostToDevice, s));
-    size_t foff = (ibytes + 63) / 64 * 64;
-    unsigned long long* flags = (unsigned long long*)(buf + foff);
-    int* status = (int*)(flags + nb);
-    fill_u64<<<(nb + 255) / 256, 256, 0, s>>>(flags, nb, 0ull);
-    HS_HIP(hipGetLastError());
-    HS_HIP(hipMemsetAsync(status, 0, sizeof(int), s));
-    const int* d_r0 = d;
-    const int* d_r1 = d + nb;
-    const int* d_of = d + 2 * nb;
-    const int* d_soff = d_of + n;
-    const int* d_seg = d_soff + nb + 1;
-    trsv_blocks<T><<<8 * nb, BR * MAXK, 0, s>>>(mode, n, k, f, ld, x, ldx, d_r0, d_r1, d_of, d_soff, d_seg, flags, status);
+    trsv1_kernel<T><<<nb, 64, 0, s>>>(trans ? BWD_UNIT : BWD_SINGLE, n, f, ld, x, pub, e1 + 1, status);
     HS_HIP(hipGetLastError());
     int st = 0;
     HS_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -587,10 +389,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 }
 
 // c -= A b for the substitutions' products: b and c have at most 16 columns, so one wave owns a strip of 16 rows of c
-// (one MFMA accumulator) and takes the whole of k in order, 4 terms per 16x16x4 step, eight steps' operands in flight.
-// A(i, p) is a[p lda + i] (ta) or a[i lda + p]. Padding: rows and columns past the ends read +0 (results dropped),
-// terms past k read (+0)(-0) = -0, which changes no sum.
-template <typename T, bool TA>
+// (one MFMA accumulator) and takes the whole of k in order, 4 terms per 16x16x4 step: increasing p, or decreasing p
+// for REV. The terms come in stages of 4 D, the next stage's operands in flight while the current one runs; whole
+// stages step plain pointers, the last partial stage clamps and pads. A(i, p) is a[p lda + i] (ta) or a[i lda + p].
+// Rows and columns past the ends read the last row or column (their results are dropped); terms past k read
+// (+0)(-0) = -0, which changes no sum.
+template <typename T, bool TA, bool REV, int D>
 __global__ __launch_bounds__(256) void gemm_skinny_kernel(int64_t m, int nc, int64_t k, const T* __restrict__ a,
                                                           int64_t lda, const T* __restrict__ b, int64_t ldb, T* c,
                                                           int64_t ldc) {
@@ -603,27 +407,67 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(int64_t m, int nc, int
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int64_t r = i0 + Mfma<T>::row(lane, e);
-        const T v = c[(r < m ? r : m - 1) * ldc + (li < nc ? li : nc - 1)];
-        acc[e] = (r < m && li < nc) ? v : T(0);
+        acc[e] = c[(r < m ? r : m - 1) * ldc + (li < nc ? li : nc - 1)];
     }
     const int64_t ai = i0 + li < m ? i0 + li : m - 1;
     const int bc = li < nc ? li : nc - 1;
-    const bool aok = i0 + li < m, bok = li < nc;
-    constexpr int D = 8;
-    for (int64_t p0 = 0; p0 < k; p0 += 4 * D) {
+    // Operand indices of this lane's term t (p = t, or k - 1 - t for REV), and their steps per MFMA.
+    const int64_t p_first = REV ? k - 1 - lk : lk;
+    const int64_t ia0 = TA ? p_first * lda + ai : ai * lda + p_first, ib0 = p_first * ldb + bc;
+    const int64_t sa = (REV ? -4 : 4) * (TA ? lda : 1), sb = (REV ? -4 : 4) * ldb;
+    const int64_t full = k / (4 * D) * (4 * D);  // terms in whole stages
+    auto load = [&](int64_t t0, T(&av)[D], T(&bv)[D]) {  // a whole stage from term t0
+        int64_t ia = ia0 + (t0 / 4) * sa, ib = ib0 + (t0 / 4) * sb;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            av[d] = TA ? a[ia] : a[ia + d * sa];  // rows of A: one add a step; columns: fixed offsets
+            bv[d] = b[ib];
+            if (TA) ia += sa;
+            ib += sb;
+            // One running index per operand, rather than a register pair for every step's address.
+            asm volatile("" : "+v"(ia), "+v"(ib));
+        }
+    };
+    auto run = [&](const T(&av)[D], const T(&bv)[D]) {
+        // The stage's operands pass through an empty asm here, so that their negations are not moved up to the
+        // loads (where they would wait for the loads just issued).
+        T x[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            x[d] = av[d];
+            asm volatile("" : "+v"(x[d]));
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc = Mfma<T>::step(-x[d], bv[d], acc);
+    };
+    if (full > 0) {
+        // Pairs of stages in one straight block; every prefetch is issued (past the last whole stage it reloads that
+        // stage), so that each wait is for the older stage only (2 D loads a stage stay under the 63 that a wait
+        // can count).
+        const int64_t last = full - 4 * D;
+        T a0[D], b0[D], a1[D], b1[D];
+        load(0, a0, b0);
+        int64_t t0 = 0;
+        for (; t0 + 8 * D <= full; t0 += 8 * D) {
+            load(t0 + 4 * D, a1, b1);
+            run(a0, b0);
+            load(t0 + 8 * D < last ? t0 + 8 * D : last, a0, b0);
+            run(a1, b1);
+        }
+        if (t0 < full) run(a0, b0);
+    }
+    if (full < k) {  // the partial stage
         T av[D], bv[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int64_t p = p0 + 4 * d + lk, pc = p < k ? p : k - 1;
-            av[d] = TA ? a[pc * lda + ai] : a[ai * lda + pc];
-            bv[d] = b[pc * ldb + bc];
+            const int64_t t = full + 4 * d + lk, tc = t < k ? t : k - 1, p = REV ? k - 1 - tc : tc;
+            av[d] = TA ? a[p * lda + ai] : a[ai * lda + p];
+            bv[d] = b[p * ldb + bc];
         }
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int64_t p = p0 + 4 * d + lk;
-            const T x = (aok && p < k) ? -av[d] : T(0);
-            const T y = p < k ? (bok ? bv[d] : T(0)) : T(-0.0);
-            acc = Mfma<T>::step(x, y, acc);
+            const bool in = full + 4 * d + lk < k;
+            acc = Mfma<T>::step(in ? -av[d] : T(0), in ? bv[d] : T(-0.0), acc);
         }
     }
 #pragma unroll
@@ -633,13 +477,21 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(int64_t m, int nc, int
     }
 }
 
+// c (m x nc, nc <= 16) -= A b in order of the terms (decreasing p when rev), a 4-wave workgroup per 64 rows of c.
 template <typename T>
-int gemm_skinny(bool ta, int64_t m, int64_t nc, int64_t k, const T* a, int64_t lda, const T* b, int64_t ldb, T* c,
-                int64_t ldc, hipStream_t s) {
+int gemm_skinny(bool ta, bool rev, int64_t m, int64_t nc, int64_t k, const T* a, int64_t lda, const T* b, int64_t ldb,
+                T* c, int64_t ldc, hipStream_t s) {
     if (m == 0 || nc == 0 || k == 0) return OK;
     const unsigned grid = (unsigned)((m + 63) / 64);
-    if (ta) gemm_skinny_kernel<T, true><<<grid, 256, 0, s>>>(m, (int)nc, k, a, lda, b, ldb, c, ldc);
-    else gemm_skinny_kernel<T, false><<<grid, 256, 0, s>>>(m, (int)nc, k, a, lda, b, ldb, c, ldc);
+#define HS_SKINNY(TA, REV, D) gemm_skinny_kernel<T, TA, REV, D><<<grid, 256, 0, s>>>(m, (int)nc, k, a, lda, b, ldb, c, ldc)
+    if (k <= 64) {
+        if (ta) rev ? HS_SKINNY(true, true, 16) : HS_SKINNY(true, false, 16);
+        else rev ? HS_SKINNY(false, true, 16) : HS_SKINNY(false, false, 16);
+    } else {
+        if (ta) rev ? HS_SKINNY(true, true, 24) : HS_SKINNY(true, false, 24);
+        else rev ? HS_SKINNY(false, true, 24) : HS_SKINNY(false, false, 24);
+    }
+#undef HS_SKINNY
     HS_HIP(hipGetLastError());
     return OK;
 }
@@ -670,11 +522,27 @@ int fwd_blocked(int mode, int64_t n, int64_t k, const T* lu, int64_t lda, T* w, 
             // Rows [hi, n) take rows [lo, hi): entry (i, q) is L(i, q) = F[q][i] (FWD_UNIT) or Uᵀ(i, q) = F[i][q].
             const bool ta = mode == FWD_UNIT;
             const T* a = ta ? lu + lo * lda + hi : lu + hi * lda + lo;
-            const int rc = k <= 16 ? gemm_skinny<T>(ta, n - hi, k, hi - lo, a, lda, w + lo * k, k, w + hi * k, k, s)
+            const int rc = k <= 16 ? gemm_skinny<T>(ta, false, n - hi, k, hi - lo, a, lda, w + lo * k, k, w + hi * k, k, s)
                                    : gemm<T>(ta, false, n - hi, k, hi - lo, T(-1), a, lda, w + lo * k, k, T(1), w + hi * k,
                                              k, s, true);
             if (rc) return rc;
         }
+    }
+    return OK;
+}
+
+// Lᵀ x = z from the bottom (terms in decreasing order): 64-row blocks from the end up, each solving among itself, then
+// the rows above it take its rows' terms, last row first.
+template <typename T>
+int bwd_unit_blocked(int64_t n, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
+    for (int64_t hi = n; hi > 0; hi -= BS) {
+        const int64_t lo = std::max<int64_t>(0, hi - BS);
+        int rc = block_solve<T>(BWD_UNIT, lo, hi - lo, k, lu, lda, w, s);
+        // Rows [0, lo) take rows [lo, hi): entry (i, q) is Lᵀ(i, lo + q) = F[i][lo + q].
+        for (int64_t c0 = 0; c0 < k && lo > 0 && rc == OK; c0 += 16)
+            rc = gemm_skinny<T>(false, true, lo, std::min<int64_t>(16, k - c0), hi - lo, lu + lo, lda, w + lo * k + c0, k,
+                                w + c0, k, s);
+        if (rc) return rc;
     }
     return OK;
 }
@@ -688,7 +556,7 @@ int bwd_halving(int64_t r0, int64_t r1, int64_t n, int64_t k, const T* lu, int64
     int rc = bwd_halving<T>(mid, r1, n, k, lu, lda, w, s);
     // Rows [r0, mid) take rows [mid, r1): entry (i, q) is U(i, q) = F[q][i].
     if (rc == OK)
-        rc = k <= 16 ? gemm_skinny<T>(true, mid - r0, k, r1 - mid, lu + mid * lda + r0, lda, w + mid * k, k, w + r0 * k, k, s)
+        rc = k <= 16 ? gemm_skinny<T>(true, false, mid - r0, k, r1 - mid, lu + mid * lda + r0, lda, w + mid * k, k, w + r0 * k, k, s)
                      : gemm<T>(true, false, mid - r0, k, r1 - mid, T(-1), lu + mid * lda + r0, lda, w + mid * k, k, T(1),
                                w + r0 * k, k, s, true);
     if (rc == OK) rc = bwd_halving<T>(r0, mid, n, k, lu, lda, w, s);
@@ -708,7 +576,7 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
         gather_rows<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, (int)k, b, ldb, perm, w, k);
         HS_HIP(hipGetLastError());
         if (k == 1) {
-            rc = trsv1_lu<T>(n, lu, lda, w, s);
+            rc = trsv1_lu<T>(false, n, lu, lda, w, s);
         } else {
             rc = fwd_blocked<T>(FWD_UNIT, n, k, lu, lda, w, s);
             if (rc == OK) rc = bwd_halving<T>(0, n, n, k, lu, lda, w, s);
@@ -718,10 +586,11 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
         }
     } else {
         HS_HIP(hipMemcpy2DAsync(w, k * sizeof(T), b, ldb * sizeof(T), k * sizeof(T), n, hipMemcpyDeviceToDevice, s));
-        rc = fwd_blocked<T>(FWD_DIAG, n, k, lu, lda, w, s);
-        for (int64_t c0 = 0; c0 < k && rc == OK; c0 += MAXK) {
-            const int kc = (int)std::min<int64_t>(MAXK, k - c0);
-            rc = trsv_pass<T>(BWD_UNIT, 0, n, n, kc, lu, lda, w + c0, k, s);
+        if (k == 1) {
+            rc = trsv1_lu<T>(true, n, lu, lda, w, s);
+        } else {
+            rc = fwd_blocked<T>(FWD_DIAG, n, k, lu, lda, w, s);
+            if (rc == OK) rc = bwd_unit_blocked<T>(n, k, lu, lda, w, s);
         }
         if (rc == OK) {
             scatter_rows<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, (int)k, w, k, perm, b, ldb);

@@ -185,11 +185,34 @@ def test_singular_matrix_is_not_solved_by_lu(la):
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
-@pytest.mark.parametrize('n', [1, 5, 100, 513, 1030])
 @pytest.mark.parametrize('k', [1, 3, 16])
+def test_solves_with_subnormal_diagonals_match_the_oracle(dtype, k):
+    """A matrix scaled into the subnormal range: U's diagonal items are subnormal, so every row of U takes a quotient
+    rather than a reciprocal multiply (`divide`, factor.rs:506), in the block solves of both orders."""
+    import torch
+    from rustymath_amd import device as dv
+    n = 150
+    rng = np.random.default_rng(5 + k)
+    scale = 2.0 ** (-1060 if dtype == np.float64 else -140)
+    a = (rng.standard_normal((n, n)) * scale).astype(dtype)
+    b = (rng.standard_normal((n, k)) * scale).astype(dtype)
+    f, perm = dv.getrf(torch.from_numpy(a).cuda())
+    wf, wperm, _ = oracle.lu(a)
+    assert np.abs(np.diag(wf)).max() < np.finfo(dtype).tiny
+    assert np.isfinite(oracle.lu_solve(wf, wperm, b)).all() and np.isfinite(oracle.lu_solve_t(wf, wperm, b)).all()
+    x = dv.getrs(f, perm, torch.from_numpy(b).cuda()).cpu().numpy()
+    assert bits_equal(x, oracle.lu_solve(wf, wperm, b))
+    y = dv.getrs(f, perm, torch.from_numpy(b).cuda(), trans=True).cpu().numpy()
+    assert bits_equal(y, oracle.lu_solve_t(wf, wperm, b))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [1, 5, 100, 513, 1030])
+@pytest.mark.parametrize('k', [1, 3, 16, 21, 70])
 def test_solves_and_transposed_solves_match_the_oracle_at_block_sizes(dtype, n, k):
-    """Blocked substitutions: 512-row block passes joined by in-order MFMA products, for A x = b (`lu_solve`'s orders,
-    the halving of `backward` across nodes) and Aᵀ x = b (the adjoint's orders), against the oracle bit for bit."""
+    """Blocked substitutions: 64-row block solves joined by in-order MFMA products, for A x = b (`lu_solve`'s orders,
+    the halving of `backward` across nodes) and Aᵀ x = b (the adjoint's orders), against the oracle bit for bit; k past
+    16 takes the products in column groups (or the tiled product), past 64 the block solves in column chunks."""
     import torch
     from rustymath_amd import device as dv
     rng = np.random.default_rng(n * 31 + k)

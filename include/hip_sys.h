@@ -59,6 +59,18 @@ int hs_slu_factor(const float* a, int64_t n, float* factors, int64_t* perm, int6
 int hs_dlu_solve(const double* factors, const int64_t* perm, int64_t n, const double* b, int64_t k, double* x);
 int hs_slu_solve(const float* factors, const int64_t* perm, int64_t n, const float* b, int64_t k, float* x);
 
+/* The inverse (row-major n×n) from factors and perm as hs_*lu_factor gives them; an out-of-range row position is
+ * argument -2. Replaces `Factor::matrix_lu_inverse_into` / `lu_inverse` (factor.rs:91, :645-664) behind `Lu::inverse`
+ * (:150-154), in the orders of rustymath's own kernels. */
+int hs_dlu_inverse(const double* factors, const int64_t* perm, int64_t n, double* out);
+int hs_slu_inverse(const float* factors, const int64_t* perm, int64_t n, float* out);
+
+/* `LuWork::inverse_into` for a square a (factor.rs:215-219), which `pinv` takes for a regular matrix (:211-213):
+ * out = a⁻¹ and *det = its determinant where every pivot exceeds eps·n·max|a| (*regular = 1); *regular = 0 and out
+ * untouched where the reference would take the SVD's pseudoinverse, which this library leaves to the caller. */
+int hs_dinverse(const double* a, int64_t n, double* out, double* det, int* regular);
+int hs_sinverse(const float* a, int64_t n, float* out, float* det, int* regular);
+
 /* `LuWork::solve` for a square a (factor.rs:207-226): x = a⁻¹ b through LU where every pivot exceeds
  * eps·n·max|a| (*regular = 1); *regular = 0 and x untouched where the reference would take least squares,
  * which this library leaves to the caller. */
@@ -81,6 +93,11 @@ int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_
                   int64_t ldb, void* stream);
 int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t ldf, const int64_t* perm, float* b,
                   int64_t ldb, void* stream);
+/* The inverse from device factors (ld ldf) and perm into out (row-major, ld ldo), as hs_dlu_inverse computes it. */
+int hs_dev_dgetri(int64_t n, const double* factors, int64_t ldf, const int64_t* perm, double* out, int64_t ldo,
+                  void* stream);
+int hs_dev_sgetri(int64_t n, const float* factors, int64_t ldf, const int64_t* perm, float* out, int64_t ldo,
+                  void* stream);
 
 /* Mixed precision (no counterpart in the reference, which has no iterative refinement): solves A x = b for
  * the n×n f64 matrix a (row-major, lda = n) and one right-hand side b (device, n f64) by factoring a copy of A

@@ -24,6 +24,10 @@ SIGNATURES = {
     'hs_slu_factor': (_i32, [_vp, _i64, _vp, _vp, _vp]),
     'hs_dlu_solve': (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
     'hs_slu_solve': (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
+    'hs_dlu_inverse': (_i32, [_vp, _vp, _i64, _vp]),
+    'hs_slu_inverse': (_i32, [_vp, _vp, _i64, _vp]),
+    'hs_dinverse': (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    'hs_sinverse': (_i32, [_vp, _i64, _vp, _vp, _vp]),
     'hs_dsolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),
     'hs_ssolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),
     'hs_dev_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
@@ -32,6 +36,8 @@ SIGNATURES = {
     'hs_dev_sgetrf': (_i32, [_i64, _vp, _i64, _vp, _vp]),
     'hs_dev_dgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_sgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    'hs_dev_dgetri': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    'hs_dev_sgetri': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_dsgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i32, _d, _vp, _vp, _vp]),
     'hs_dev_sgemm_bf16_update': (_i32, [_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
     'hs_dev_dbgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i32, _i32, _d, _vp, _vp, _vp, _vp]),

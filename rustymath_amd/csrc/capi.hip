@@ -221,6 +221,24 @@ int host_lu_solve(const T* factors, const int64_t* perm, int64_t n, const T* b, 
     return rc;
 }
 
+// Copies the row-major n×n a to the device, factors it in place and decides `regular` (`LuWork::regular`,
+// factor.rs:221-226: every pivot exceeds eps·n·max|a|, NaN counting as not negligible).
+template <typename T>
+int host_factor_regular(const T* a, int64_t n, T* d, int64_t* dp, unsigned long long* dm, int& reg, hipStream_t s) {
+    reg = 0;
+    if (hipMemcpyAsync(d, a, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess) return hip_status(hipErrorUnknown, "H2D");
+    hipMemsetAsync(dm, 0, 64, s);
+    maxabs_kernel<T><<<256, 256, 0, s>>>(n * n, d, dm);
+    int rc = dev_getrf_perm<T>(n, d, n, dp, nullptr, nullptr, s);
+    if (rc) return rc;
+    int* dreg = (int*)(dm + 1);
+    fill_int<<<1, 1, 0, s>>>(dreg, 1);
+    regular_kernel<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, d, n, dm, dreg);
+    if (hipMemcpyAsync(&reg, dreg, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return hip_status(hipErrorUnknown, "regular");
+    return OK;
+}
+
 template <typename T>
 int host_solve(const T* a, int64_t n, const T* b, int64_t k, T* x, int* regular) {
     if (n < 0) return -2;
@@ -232,28 +250,85 @@ int host_solve(const T* a, int64_t n, const T* b, int64_t k, T* x, int* regular)
     T* db = (T*)pool_get(sizeof(T) * (size_t)(n * (k ? k : 1)));
     int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)n);
     unsigned long long* dm = (unsigned long long*)pool_get(64);
-    int rc = OK, reg = 1;
+    int rc = OK, reg = 0;
     if (!d || !db || !dp || !dm) rc = hip_status(hipErrorOutOfMemory, "pool_get");
-    if (!rc && (hipMemcpyAsync(d, a, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-                (k && hipMemcpyAsync(db, b, sizeof(T) * n * k, hipMemcpyHostToDevice, s) != hipSuccess)))
+    if (!rc && k && hipMemcpyAsync(db, b, sizeof(T) * n * k, hipMemcpyHostToDevice, s) != hipSuccess)
         rc = hip_status(hipErrorUnknown, "H2D");
-    if (!rc) {
-        hipMemsetAsync(dm, 0, 64, s);
-        maxabs_kernel<T><<<256, 256, 0, s>>>(n * n, d, dm);
-        rc = dev_getrf_perm<T>(n, d, n, dp, nullptr, nullptr, s);
-    }
-    if (!rc) {
-        int* dreg = (int*)(dm + 1);
-        fill_int<<<1, 1, 0, s>>>(dreg, 1);
-        regular_kernel<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, d, n, dm, dreg);
-        if (hipMemcpyAsync(&reg, dreg, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-            rc = hip_status(hipErrorUnknown, "regular");
-    }
+    if (!rc) rc = host_factor_regular<T>(a, n, d, dp, dm, reg, s);
     if (!rc && reg && k) rc = getrs<T>(false, n, k, d, n, dp, db, k, s);
     if (!rc && reg && k && hipMemcpyAsync(x, db, sizeof(T) * n * k, hipMemcpyDeviceToHost, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "D2H");
     hipStreamSynchronize(s);
     pool_put(d); pool_put(db); pool_put(dp); pool_put(dm);
     if (regular) *regular = rc ? 0 : reg;
+    return rc;
+}
+
+// The determinant from factors and perm (`Lu::determinant`, factor.rs:174): U's diagonal multiplied in order from 1,
+// negated for an odd permutation (`odd`, :186-194, counting cycles at their smallest positions).
+template <typename T>
+T determinant_of(const T* diag, const int64_t* perm, int64_t n) {
+    T d = T(1);
+    for (int64_t j = 0; j < n; ++j) d = d * diag[j];
+    int64_t cycles = 0;
+    for (int64_t st = 0; st < n; ++st) {
+        int64_t i = perm[st];
+        while (i > st) i = perm[i];
+        cycles += i == st;
+    }
+    return (n - cycles) % 2 ? -d : d;
+}
+
+// `LuWork::inverse_into` (factor.rs:215-219): where a is regular, out = a⁻¹ (row-major n×n) and *det its determinant.
+template <typename T>
+int host_inverse(const T* a, int64_t n, T* out, T* det, int* regular) {
+    if (n < 0) return -2;
+    if (regular) *regular = 0;
+    if (n == 0) return OK;
+    hipStream_t s = thread_stream();
+    T* d = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    T* dx = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)n);
+    unsigned long long* dm = (unsigned long long*)pool_get(64);
+    int rc = OK, reg = 0;
+    if (!d || !dx || !dp || !dm) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc) rc = host_factor_regular<T>(a, n, d, dp, dm, reg, s);
+    if (!rc && reg) rc = getri<T>(n, d, n, dp, dx, n, s);
+    if (!rc && reg) {
+        std::vector<T> diag((size_t)n);
+        std::vector<int64_t> perm((size_t)n);
+        if (hipMemcpyAsync(out, dx, sizeof(T) * n * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpy2DAsync(diag.data(), sizeof(T), d, sizeof(T) * (n + 1), sizeof(T), n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(perm.data(), dp, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = hip_status(hipErrorUnknown, "D2H");
+        else if (det)
+            *det = determinant_of<T>(diag.data(), perm.data(), n);
+    }
+    hipStreamSynchronize(s);
+    pool_put(d); pool_put(dx); pool_put(dp); pool_put(dm);
+    if (regular) *regular = rc ? 0 : reg;
+    return rc;
+}
+
+// `Lu::inverse` (factor.rs:150-154) from host factors (column-major n×n) and perm.
+template <typename T>
+int host_lu_inverse(const T* factors, const int64_t* perm, int64_t n, T* out) {
+    if (n < 0) return -3;
+    for (int64_t i = 0; i < n; ++i) if (perm[i] < 0 || perm[i] >= n) return -2;
+    if (n == 0) return OK;
+    hipStream_t s = thread_stream();
+    T* df = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    T* dx = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)n);
+    int rc = OK;
+    if (!df || !dx || !dp) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && (hipMemcpyAsync(df, factors, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(dp, perm, sizeof(int64_t) * n, hipMemcpyHostToDevice, s) != hipSuccess))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) rc = getri<T>(n, df, n, dp, dx, n, s);
+    if (!rc && hipMemcpyAsync(out, dx, sizeof(T) * n * n, hipMemcpyDeviceToHost, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "D2H");
+    hipStreamSynchronize(s);
+    pool_put(df); pool_put(dx); pool_put(dp);
     return rc;
 }
 
@@ -309,6 +384,18 @@ int hs_dlu_solve(const double* factors, const int64_t* perm, int64_t n, const do
 int hs_slu_solve(const float* factors, const int64_t* perm, int64_t n, const float* b, int64_t k, float* x) {
     return host_lu_solve<float>(factors, perm, n, b, k, x);
 }
+int hs_dlu_inverse(const double* factors, const int64_t* perm, int64_t n, double* out) {
+    return host_lu_inverse<double>(factors, perm, n, out);
+}
+int hs_slu_inverse(const float* factors, const int64_t* perm, int64_t n, float* out) {
+    return host_lu_inverse<float>(factors, perm, n, out);
+}
+int hs_dinverse(const double* a, int64_t n, double* out, double* det, int* regular) {
+    return host_inverse<double>(a, n, out, det, regular);
+}
+int hs_sinverse(const float* a, int64_t n, float* out, float* det, int* regular) {
+    return host_inverse<float>(a, n, out, det, regular);
+}
 int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x, int* regular) {
     return host_solve<double>(a, n, b, k, x, regular);
 }
@@ -349,6 +436,20 @@ int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t
     if (k < 0) return -3;
     hipStream_t s = (hipStream_t)stream;
     return getrs<float>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
+}
+int hs_dev_dgetri(int64_t n, const double* factors, int64_t ldf, const int64_t* perm, double* out, int64_t ldo,
+                  void* stream) {
+    if (n < 0) return -1;
+    if (ldf < n) return -3;
+    if (ldo < n) return -6;
+    return getri<double>(n, factors, ldf, perm, out, ldo, (hipStream_t)stream);
+}
+int hs_dev_sgetri(int64_t n, const float* factors, int64_t ldf, const int64_t* perm, float* out, int64_t ldo,
+                  void* stream) {
+    if (n < 0) return -1;
+    if (ldf < n) return -3;
+    if (ldo < n) return -6;
+    return getri<float>(n, factors, ldf, perm, out, ldo, (hipStream_t)stream);
 }
 
 int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol, int* iters,

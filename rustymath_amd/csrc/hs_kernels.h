@@ -37,6 +37,11 @@ template <typename T>
 int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int64_t* pivots, T* b, int64_t ldb,
           hipStream_t s);
 
+// The inverse from factors and perm as getrf leaves them (`lu_inverse`, factor.rs:645-664): out (n×n, row-major, ld
+// ldo) = U⁻¹ L⁻¹ with its columns permuted back, in the reference's orders of operations.
+template <typename T>
+int getri(int64_t n, const T* lu, int64_t lda, const int64_t* perm, T* out, int64_t ldo, hipStream_t s);
+
 // Mixed-precision solve: FP32 factors on MFMA, FP64 iterative refinement on the GPU (ir.hip).
 int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol,
             int* iters, double* berr, hipStream_t s);

@@ -248,9 +248,11 @@ __device__ __forceinline__ T smallest_normal() {
 // waves, two columns each at a time, so that two chains overlap in every wave.
 constexpr int SW_WAVES = 8, SW_NC = 2;
 template <typename T, int MODE>
-__global__ __launch_bounds__(64 * SW_WAVES) void sweep_kernel(int64_t r0, int m, int k, const T* __restrict__ f,
+__global__ __launch_bounds__(64 * SW_WAVES) void sweep_kernel(int64_t r0, int m, int64_t ktot, const T* __restrict__ f,
                                                               int64_t ld, T* x, int64_t ldx) {
     __shared__ T E[BS][BS + 1];
+    x += (int64_t)blockIdx.x * BS;  // this workgroup's 64 columns
+    const int k = (int)std::min<int64_t>(BS, ktot - (int64_t)blockIdx.x * BS);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     stage_triangle<T, 64 * SW_WAVES>(MODE != FWD_UNIT, r0, m, f, ld, E);
     __syncthreads();
@@ -305,8 +307,10 @@ __global__ __launch_bounds__(64 * SW_WAVES) void sweep_kernel(int64_t r0, int m,
 // diagonal, whose rows take a quotient, runs a plain loop instead. Above 32 rows (factor.rs:693-707) the bottom leaf
 // goes first, then all four waves add the bottom rows' terms to the top rows in increasing order, then the top leaf.
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void bwd_node_kernel(int64_t r0, int m, int k, const T* __restrict__ f, int64_t ld,
-                                                       T* x, int64_t ldx) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void bwd_node_kernel(
+    int64_t r0, int m, int64_t ktot, const T* __restrict__ f, int64_t ld, T* x, int64_t ldx) {
+    x += (int64_t)blockIdx.x * BS;  // this workgroup's 64 columns
+    const int k = (int)std::min<int64_t>(BS, ktot - (int64_t)blockIdx.x * BS);
     constexpr int GUARD = BR * (BS + 1);  // covers rows and columns down to -31
     __shared__ T Eg[GUARD + BS * (BS + 1)];
     __shared__ T xb[BS][BS + 1];  // xb[row][column]
@@ -500,15 +504,13 @@ int gemm_skinny(bool ta, bool rev, int64_t m, int64_t nc, int64_t k, const T* a,
 // that add a finished block's terms to the rows still to solve, in order of the terms (in-order products: every item
 // takes one fused multiply-add per term, in the order the substitution would).
 template <typename T>
-int block_solve(int mode, int64_t r0, int64_t m, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
-    for (int64_t c0 = 0; c0 < k; c0 += BS) {  // at most 64 columns a launch
-        const int kc = (int)std::min<int64_t>(BS, k - c0);
-        if (mode == BWD_LEAVES) bwd_node_kernel<T><<<1, 256, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
-        else if (mode == FWD_UNIT) sweep_kernel<T, FWD_UNIT><<<1, 64 * SW_WAVES, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
-        else if (mode == FWD_DIAG) sweep_kernel<T, FWD_DIAG><<<1, 64 * SW_WAVES, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
-        else sweep_kernel<T, BWD_UNIT><<<1, 64 * SW_WAVES, 0, s>>>(r0, (int)m, kc, lu, lda, w + c0, k);
-        HS_HIP(hipGetLastError());
-    }
+int block_solve(int mode, int64_t r0, int64_t m, int64_t k, const T* lu, int64_t lda, T* w, int64_t ldw, hipStream_t s) {
+    const unsigned grid = (unsigned)((k + BS - 1) / BS);  // a workgroup per 64 columns
+    if (mode == BWD_LEAVES) bwd_node_kernel<T><<<grid, 256, 0, s>>>(r0, (int)m, k, lu, lda, w, ldw);
+    else if (mode == FWD_UNIT) sweep_kernel<T, FWD_UNIT><<<grid, 64 * SW_WAVES, 0, s>>>(r0, (int)m, k, lu, lda, w, ldw);
+    else if (mode == FWD_DIAG) sweep_kernel<T, FWD_DIAG><<<grid, 64 * SW_WAVES, 0, s>>>(r0, (int)m, k, lu, lda, w, ldw);
+    else sweep_kernel<T, BWD_UNIT><<<grid, 64 * SW_WAVES, 0, s>>>(r0, (int)m, k, lu, lda, w, ldw);
+    HS_HIP(hipGetLastError());
     return OK;
 }
 
@@ -516,7 +518,7 @@ template <typename T>
 int fwd_blocked(int mode, int64_t n, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
     for (int64_t lo = 0; lo < n; lo += BS) {
         const int64_t hi = std::min<int64_t>(n, lo + BS);
-        const int rc = block_solve<T>(mode, lo, hi - lo, k, lu, lda, w, s);
+        const int rc = block_solve<T>(mode, lo, hi - lo, k, lu, lda, w, k, s);
         if (rc) return rc;
         if (hi < n) {
             // Rows [hi, n) take rows [lo, hi): entry (i, q) is L(i, q) = F[q][i] (FWD_UNIT) or Uᵀ(i, q) = F[i][q].
@@ -537,7 +539,7 @@ template <typename T>
 int bwd_unit_blocked(int64_t n, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
     for (int64_t hi = n; hi > 0; hi -= BS) {
         const int64_t lo = std::max<int64_t>(0, hi - BS);
-        int rc = block_solve<T>(BWD_UNIT, lo, hi - lo, k, lu, lda, w, s);
+        int rc = block_solve<T>(BWD_UNIT, lo, hi - lo, k, lu, lda, w, k, s);
         // Rows [0, lo) take rows [lo, hi): entry (i, q) is Lᵀ(i, lo + q) = F[i][lo + q].
         for (int64_t c0 = 0; c0 < k && lo > 0 && rc == OK; c0 += 16)
             rc = gemm_skinny<T>(false, true, lo, std::min<int64_t>(16, k - c0), hi - lo, lu + lo, lda, w + lo * k + c0, k,
@@ -551,7 +553,7 @@ int bwd_unit_blocked(int64_t n, int64_t k, const T* lu, int64_t lda, T* w, hipSt
 // increasing order, then the top half; nodes of at most 64 rows go to one block solve.
 template <typename T>
 int bwd_halving(int64_t r0, int64_t r1, int64_t n, int64_t k, const T* lu, int64_t lda, T* w, hipStream_t s) {
-    if (r1 - r0 <= BS) return block_solve<T>(BWD_LEAVES, r0, r1 - r0, k, lu, lda, w, s);
+    if (r1 - r0 <= BS) return block_solve<T>(BWD_LEAVES, r0, r1 - r0, k, lu, lda, w, k, s);
     const int64_t mid = (r0 + r1) / 2;
     int rc = bwd_halving<T>(mid, r1, n, k, lu, lda, w, s);
     // Rows [r0, mid) take rows [mid, r1): entry (i, q) is U(i, q) = F[q][i].
@@ -602,6 +604,53 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
     return rc;
 }
 
+namespace {
+template <typename T>
+__global__ void fill_identity(int64_t n, T* w) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n * n) w[idx] = idx / n == idx % n ? T(1) : T(0);
+}
+template <typename T>
+__global__ void scatter_columns(int64_t n, const T* __restrict__ w, const int64_t* __restrict__ perm, T* out,
+                                int64_t ldo) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n * n) out[(idx / n) * ldo + perm[idx % n]] = w[idx];
+}
+}  // namespace
+
+// `lu_inverse` with rustymath's own kernels (factor.rs:645-664): from the identity, L⁻¹ by `forward`'s block mode
+// (32-row blocks; a block's rows take only the columns up to the block's end, the rest stay zero, and a product
+// carries each block to every row below it, routed as `gemm` routes it), then U⁻¹ by `backward`'s halving over all n
+// columns, then out[r][perm[c]] = x[r][c]. out is row-major n×n with leading dimension ldo.
+template <typename T>
+int getri(int64_t n, const T* lu, int64_t lda, const int64_t* perm, T* out, int64_t ldo, hipStream_t s) {
+    if (n == 0) return OK;
+    T* w = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    if (!w) return hip_status(hipErrorOutOfMemory, "pool_get");
+    const unsigned grid = (unsigned)((n * n + 255) / 256);
+    fill_identity<T><<<grid, 256, 0, s>>>(n, w);
+    HS_HIP(hipGetLastError());
+    int rc = OK;
+    for (int64_t i0 = 0; i0 < n && rc == OK; i0 += BR) {
+        const int64_t i1 = std::min<int64_t>(n, i0 + BR);
+        rc = block_solve<T>(FWD_UNIT, i0, i1 - i0, i1, lu, lda, w, n, s);
+        // Rows [i1, n) take rows [i0, i1) over columns [0, i1): entry (i, q) is L(i1 + i, i0 + q) = F[i0 + q][i1 + i].
+        if (rc == OK && i1 < n)
+            rc = gemm<T>(true, false, n - i1, i1, i1 - i0, T(-1), lu + i0 * lda + i1, lda, w + i0 * n, n, T(1), w + i1 * n,
+                         n, s);
+    }
+    if (rc == OK) rc = bwd_halving<T>(0, n, n, n, lu, lda, w, s);
+    if (rc == OK) {
+        scatter_columns<T><<<grid, 256, 0, s>>>(n, w, perm, out, ldo);
+        HS_HIP(hipGetLastError());
+    }
+    HS_HIP(hipStreamSynchronize(s));
+    pool_put(w);
+    return rc;
+}
+
+template int getri<double>(int64_t, const double*, int64_t, const int64_t*, double*, int64_t, hipStream_t);
+template int getri<float>(int64_t, const float*, int64_t, const int64_t*, float*, int64_t, hipStream_t);
 template int getrs<double>(bool, int64_t, int64_t, const double*, int64_t, const int64_t*, double*, int64_t, hipStream_t);
 template int getrs<float>(bool, int64_t, int64_t, const float*, int64_t, const int64_t*, float*, int64_t, hipStream_t);
 

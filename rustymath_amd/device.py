@@ -63,6 +63,17 @@ def getrs(factors, perm, b, trans=False):
     return b
 
 
+def getri(factors, perm):
+    "The inverse (n×n, row-major) from factors and perm as getrf gives them (`lu_inverse`, factor.rs:645-664)."
+    _mat(factors, 'factors')
+    n = factors.shape[0]
+    out = torch.empty((n, n), dtype=factors.dtype, device=factors.device)
+    f = getattr(_lib.load(), f'hs_dev_{_sfx(factors.dtype)}getri')
+    _lib.check(f(n, factors.data_ptr(), factors.stride(0), perm.data_ptr(), out.data_ptr(), out.stride(0), _stream()),
+               'getri')
+    return out
+
+
 def dsgesv(a, b, max_iter=30, tol=None):
     """Solves A x = b for the f64 matrix `a` (n×n, compact) and vector `b` by an FP32 LU on MFMA and FP64 iterative
     refinement on the GPU. Returns (x, corrections, backward error); `tol` defaults to 10·n·eps64."""

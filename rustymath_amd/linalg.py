@@ -96,6 +96,10 @@ class Lu(NamedTuple):
         "Solve for `b` using these factors, without factoring again or applying a cutoff"
         return lu_solve(self.factors, self.p, b)
 
+    def inverse(self):
+        "The inverse from these factors, without factoring again"
+        return lu_inverse(self.factors, self.p)
+
     def determinant(self):
         "The determinant from these factors"
         return lu_det(self.factors, self.p)
@@ -105,12 +109,11 @@ class Lu(NamedTuple):
         return lu_slogdet(self.factors, self.p)
 
 
-def lu(a):
-    """The packed LU factors of each square matrix, with partial pivoting, and the positions of its rows
-    (src/linalg/stacked.rs:174-187). Row `i` of `l @ u` is row `p[i]` of the matrix."""
+def _factor(a, name):
+    "(factors at the compute width, perm, store dtype) of each square matrix of a stack."
     a = np.asarray(a)
     compute, store = _compute_dtype(a)
-    if a.ndim < 2 or a.shape[-1] != a.shape[-2]: raise ValueError(f'lu needs square matrices, not {a.shape[-2:]}')
+    if a.ndim < 2 or a.shape[-1] != a.shape[-2]: raise ValueError(f'{name} needs square matrices, not {a.shape[-2:]}')
     n = a.shape[-1]
     frame = a.shape[:-2]
     cells = np.ascontiguousarray(a.reshape((int(np.prod(frame, dtype=np.int64)), n, n)).astype(compute, copy=False))
@@ -118,8 +121,15 @@ def lu(a):
     perm = np.empty((cells.shape[0], n), np.int64)
     f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_factor')
     for i in range(cells.shape[0]):
-        _lib.check(f(_ptr(cells[i]), n, _ptr(factors[i]), _ptr(perm[i]), None), 'lu')
-    return Lu(factors.reshape(frame + (n, n)).astype(store, copy=False), perm.reshape(frame + (n,)))
+        _lib.check(f(_ptr(cells[i]), n, _ptr(factors[i]), _ptr(perm[i]), None), name)
+    return factors.reshape(frame + (n, n)), perm.reshape(frame + (n,)), store
+
+
+def lu(a):
+    """The packed LU factors of each square matrix, with partial pivoting, and the positions of its rows
+    (src/linalg/stacked.rs:174-187). Row `i` of `l @ u` is row `p[i]` of the matrix."""
+    factors, perm, store = _factor(a, 'lu')
+    return Lu(factors.astype(store, copy=False), perm)
 
 
 def _positions(p, n):
@@ -190,6 +200,70 @@ def solve(a, b):
     return out.astype(store, copy=False)
 
 
+def lu_inverse(factors, p):
+    """The inverse of each matrix from its packed LU factors and row positions, without factoring again
+    (src/linalg/stacked.rs:207-214 over core/src/linalg/factor.rs:645-664)."""
+    factors = np.asarray(factors)
+    compute, store = _compute_dtype(factors)
+    n = factors.shape[-1]
+    perm = _positions(p, n)
+    frame = factors.shape[:-2]
+    count = int(np.prod(frame, dtype=np.int64))
+    fc = np.ascontiguousarray(factors.reshape((count, n, n)).astype(compute, copy=False))
+    pc = np.ascontiguousarray(np.broadcast_to(perm, frame + (n,)).reshape((count, n)))
+    out = np.empty_like(fc)
+    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_inverse')
+    for i in range(count):
+        _lib.check(f(_ptr(fc[i]), _ptr(pc[i]), n, _ptr(out[i])), 'lu_inverse')
+    return out.reshape(frame + (n, n)).astype(store, copy=False)
+
+
+def _square_cells(a, name):
+    a = np.asarray(a)
+    compute, store = _compute_dtype(a)
+    if a.ndim < 2: raise ValueError(f'{name} needs matrices')
+    m, n = a.shape[-2:]
+    frame = a.shape[:-2]
+    cells = np.ascontiguousarray(a.reshape((int(np.prod(frame, dtype=np.int64)), m, n)).astype(compute, copy=False))
+    return cells, frame, compute, store
+
+
+def pinv(a):
+    """The inverse of each regular square matrix, through LU with no cutoff (src/linalg/stacked.rs:125-133 over
+    `LuWork::pinv_into`, core/src/linalg/factor.rs:211-219). A matrix that is not square, or has a negligible pivot,
+    takes the SVD's pseudoinverse in rustymath, which is outside this library."""
+    cells, frame, compute, store = _square_cells(a, 'pinv')
+    m, n = cells.shape[-2:]
+    if m != n: raise NotImplementedError('the pseudoinverse of a non-square matrix is outside this library')
+    out = np.empty_like(cells)
+    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'inverse')
+    regular = ctypes.c_int(0)
+    for i in range(cells.shape[0]):
+        _lib.check(f(_ptr(cells[i]), n, _ptr(out[i]), None, ctypes.byref(regular)), 'pinv')
+        if n and not regular.value:
+            raise NotImplementedError('a negligible LU pivot: rustymath takes the SVD pseudoinverse of this matrix, '
+                                      'which is outside this library')
+    return out.reshape(frame + (n, n)).astype(store, copy=False)
+
+
+def det(a):
+    """The determinant of each square matrix, from its LU factors at the compute width, then stored
+    (src/linalg/stacked.rs:136-144, factor.rs:174)."""
+    factors, perm, store = _factor(a, 'det')
+    d = np.asarray(lu_det(factors, perm)).astype(store, copy=False)
+    return d[()] if d.ndim == 0 else d
+
+
+def slogdet(a):
+    """The sign of each square matrix's determinant and the natural logarithm of its magnitude, from its LU factors
+    at the compute width (src/linalg/stacked.rs:160-171, factor.rs:176-182)."""
+    factors, perm, store = _factor(a, 'slogdet')
+    sign, log = lu_slogdet(factors, perm)
+    sign = np.asarray(sign).astype(store, copy=False)
+    log = np.asarray(log).astype(store, copy=False)
+    return (sign[()] if sign.ndim == 0 else sign), (log[()] if log.ndim == 0 else log)
+
+
 def lu_lower(factors):
     "The unit lower triangle of each matrix's packed LU factors (core/src/linalg/factor.rs:158-165)."
     f = np.swapaxes(np.asarray(factors), -1, -2)
@@ -219,10 +293,11 @@ def lu_det(factors, p):
     perm = _positions(p, n)
     fc = factors.reshape(int(np.prod(factors.shape[:-2], dtype=np.int64)), n, n); pc = perm.reshape(fc.shape[0], n)
     out = np.empty(fc.shape[0], factors.dtype)
-    for i in range(fc.shape[0]):
-        d = factors.dtype.type(1)
-        for j in range(n): d = d * fc[i, j, j]
-        out[i] = -d if _odd(pc[i]) else d
+    with np.errstate(over='ignore', invalid='ignore'):  # the fold overflows to ±inf as the reference's does
+        for i in range(fc.shape[0]):
+            d = factors.dtype.type(1)
+            for j in range(n): d = d * fc[i, j, j]
+            out[i] = -d if _odd(pc[i]) else d
     out = out.reshape(factors.shape[:-2])
     return out[()] if out.ndim == 0 else out
 

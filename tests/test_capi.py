@@ -32,6 +32,13 @@ def test_bad_arguments_are_rejected_before_any_device_work():
     assert lib.hs_dgemm(111, 111, 4, 4, 4, 1.0, None, 3, None, 4, 0.0, None, 4) == -8
     assert lib.hs_dgemm(111, 111, 0, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4) == 0
     assert lib.hs_dlu_factor(None, -1, None, None, None) == -2
+    bad = (ctypes.c_int64 * 3)(0, 3, 1)
+    assert lib.hs_dlu_inverse(None, None, -1, None) == -3
+    assert lib.hs_dlu_inverse(None, bad, 3, None) == -2
+    assert lib.hs_dinverse(None, -1, None, None, None) == -2
+    assert lib.hs_dev_dgetri(-1, None, 1, None, None, 1, None) == -1
+    assert lib.hs_dev_dgetri(4, None, 3, None, None, 4, None) == -3
+    assert lib.hs_dev_sgetri(4, None, 4, None, None, 2, None) == -6
 
 
 def test_linalg_validates_shapes_on_the_host():
@@ -40,6 +47,9 @@ def test_linalg_validates_shapes_on_the_host():
     with pytest.raises(ValueError): la.matmul(np.ones((2, 3)), np.ones((4, 2)))
     with pytest.raises(ValueError): la.lu(np.ones((2, 3)))
     with pytest.raises(IndexError): la.lu_solve(np.eye(3), np.array([-1, 0, 1]), np.ones(3))
+    with pytest.raises(IndexError): la.Lu(np.eye(3), np.array([-1, 0, 1])).inverse()  # tests/test_linalg.py:32
+    with pytest.raises(NotImplementedError): la.pinv(np.ones((2, 3)))
+    with pytest.raises(ValueError): la.det(np.ones((2, 3)))
     with pytest.raises(TypeError): la.matmul(np.ones((2, 2), complex), np.ones((2, 2)))
 
 

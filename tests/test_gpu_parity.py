@@ -224,3 +224,74 @@ def test_solves_and_transposed_solves_match_the_oracle_at_block_sizes(dtype, n, 
     assert bits_equal(x, oracle.lu_solve(wf, wperm, b))
     y = dv.getrs(f, perm, torch.from_numpy(b).cuda(), trans=True).cpu().numpy()
     assert bits_equal(y, oracle.lu_solve_t(wf, wperm, b))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [1, 2, 5, 31, 32, 33, 64, 65, 97, 130, 257, 530])
+def test_inverse_matches_the_oracle_bit_for_bit(la, dtype, n):
+    """`Lu::inverse` (factor.rs:645-664): L⁻¹ by `forward`'s 32-row block mode (a last block of one row takes its
+    product through `by_vector`, as at n = 33 and 65), U⁻¹ by `backward`'s halving over all columns, then the columns
+    permuted back; the same bits as the oracle, and a small residual (tests/linalg.rs:389-390)."""
+    a = random(n, n, 3 * n + 1, dtype)
+    f = la.lu(a)
+    want = oracle.lu_inverse(f.factors, f.p)
+    got = f.inverse()
+    assert got.dtype == dtype and bits_equal(got, want), f'max diff {np.abs(got - want).max()}'
+    aw, inv = a.astype(np.float64), got.astype(np.float64)
+    bound = 4 * EPS[dtype] * n
+    assert distance(aw @ inv, np.eye(n), max(norm(aw) * norm(inv), 1.0)) < bound
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_device_inverse_and_pinv_agree_with_lu_inverse(la, dtype):
+    """The device entry point gives the host one's bits; `pinv` of a regular matrix is the LU inverse
+    (tests/linalg.rs:546-547) and `det` is the LU determinant; a singular matrix has no LU inverse."""
+    import torch
+    from rustymath_amd import device as dv
+    n = 300
+    a = random(n, n, 77, dtype) + np.eye(n, dtype=dtype)
+    f = la.lu(a)
+    inv = f.inverse()
+    fd, perm = dv.getrf(torch.from_numpy(a).cuda())
+    assert bits_equal(dv.getri(fd, perm).cpu().numpy(), inv)
+    assert bits_equal(la.pinv(a), inv)
+    assert la.det(a) == f.determinant()
+    stack = np.stack([a, 2 * a])
+    assert bits_equal(la.pinv(stack)[1], la.lu(2 * a).inverse())
+    with pytest.raises(NotImplementedError): la.pinv(np.array([[1., 2], [2, 4]], dtype))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_inverse_scales_exactly(la, dtype):
+    "tests/linalg.rs:477-483: lu(a s).inverse() s == lu(a).inverse() bit for bit, for powers of two s."
+    n = 70
+    fi = np.finfo(dtype)
+    a = random(n, n, 23, dtype)
+    f = la.lu(a).inverse()
+    for s in [dtype(2.0 ** -20), dtype(2.0 ** 20), dtype(fi.tiny / fi.eps)]:
+        assert bits_equal(la.lu(a * s).inverse() * s, f)
+
+
+def test_inverse_and_determinant_known_answers(la):
+    """nbs/linalg.ipynb: inverse [[5, -2], [-2, 1]] of [[1, 2], [2, 5]] with determinant 1; tests/linalg.rs:553-558:
+    diag(1, 2, 3, 4) with two rows swapped has determinant -24; a zero column gives 0 and a log of -inf;
+    tests/test_linalg.py:6-32's stack of 64 fixtures against NumPy, float16 computing in float32."""
+    a = np.array([[1., 2], [2, 5]])
+    assert bits_equal(la.lu(a).inverse(), np.array([[5., -2], [-2, 1]])) and la.det(a) == 1.0
+    d = np.diag([1., 2, 3, 4])[[1, 0, 2, 3]]
+    sign, log = la.slogdet(d)
+    assert la.det(d) == -24.0 and sign == -1.0 and abs(log - math.log(24.0)) < 1e-14
+    assert la.det(np.array([[1., 0], [2, 0]])) == 0.0
+    assert la.slogdet(np.array([[1., 2], [2, 4]])) == (0.0, -np.inf)
+    base = np.array([[0., 2, 1], [3, 1, -1], [1, 0, 4]])
+    wide = np.stack([base + k / 8 * np.eye(3) for k in range(64)])
+    for dt in (np.float16, np.float32, np.float64):
+        f = la.lu(wide.astype(dt))
+        assert np.allclose(f.inverse(), np.linalg.inv(wide), rtol=0.003, atol=0.003)
+        assert np.allclose(f.determinant(), np.linalg.det(wide), rtol=0.003, atol=0.003)
+        assert np.allclose(la.pinv(wide.astype(dt)), np.linalg.inv(wide), rtol=0.003, atol=0.003)
+        sign, log = la.slogdet(wide.astype(dt))
+        want = np.linalg.slogdet(wide)
+        assert np.allclose(sign, want.sign, rtol=0.003, atol=0.003) and np.allclose(log, want.logabsdet, rtol=0.003, atol=0.003)
+    h = la.det(np.eye(3, dtype=np.float16) * 2)
+    assert np.asarray(h).dtype == np.float16 and h == 8

@@ -77,6 +77,16 @@ int hs_sinverse(const float* a, int64_t n, float* out, float* det, int* regular)
 int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x, int* regular);
 int hs_ssolve(const float* a, int64_t n, const float* b, int64_t k, float* x, int* regular);
 
+/* Stacks of `count` cells of one order n (row-major cells back to back): `Stacked::solve` (src/linalg/stacked.rs:88-101)
+ * gives x (count×n×k) and regular (count) as hs_*solve does for each cell (x holds b where regular = 0), and
+ * `Stacked::lu` (:174-187) gives factors (count×n×n, each column-major as hs_*lu_factor gives them) and perm
+ * (count×n). Orders up to 512 with at most 8 right-hand sides run as one batched launch (as the batched plan below,
+ * bit for bit each cell's single-matrix result); larger cells run one by one. */
+int hs_dsolve_stack(int64_t count, int64_t n, int64_t k, const double* a, const double* b, double* x, int* regular);
+int hs_ssolve_stack(int64_t count, int64_t n, int64_t k, const float* a, const float* b, float* x, int* regular);
+int hs_dlu_stack(int64_t count, int64_t n, const double* a, double* factors, int64_t* perm);
+int hs_slu_stack(int64_t count, int64_t n, const float* a, float* factors, int64_t* perm);
+
 /* Device-resident forms (no counterpart in the reference, whose API takes host slices): pointers are
  * device memory, `stream` a hipStream_t (NULL = the null stream). */
 int hs_dev_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,

@@ -333,6 +333,81 @@ int host_lu_inverse(const T* factors, const int64_t* perm, int64_t n, T* out) {
 }
 
 template <typename T>
+__global__ void batched_to_reference(int64_t count, int64_t n, const T* __restrict__ a, const int64_t* __restrict__ perm,
+                                     T* f) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count * n * n) return;
+    const int64_t cell = idx / (n * n), rem = idx % (n * n), c = rem / n, r = rem % n;
+    f[idx] = a[cell * n * n + perm[cell * n + r] * n + c];  // column c, row r of the cell's LU
+}
+
+// A stack of `count` cells of order n (and k right-hand sides): `Stacked::solve` (src/linalg/stacked.rs:88-101) when
+// x is given, `Stacked::lu` (:174-187) when factors are. Orders up to 512 with at most 8 right-hand sides go to the
+// batched kernel in one launch (each cell as `LuWork::solve` or `lu_factor`, bit for bit); larger cells go one by
+// one through the single-matrix path.
+template <typename T>
+int host_stack(int64_t count, int64_t n, int64_t k, const T* a, const T* b, T* x, int* regular, T* factors,
+               int64_t* perm) {
+    if (count < 0) return -1;
+    if (n < 0) return -2;
+    if (k < 0) return -3;
+    const bool solve = x != nullptr;
+    if (count == 0) return OK;
+    if (n == 0) {  // the reference's `regular` declines an empty matrix (factor.rs:222)
+        if (regular) for (int64_t i = 0; i < count; ++i) regular[i] = 0;
+        return OK;
+    }
+    if (n > 512 || k > 8) {
+        for (int64_t i = 0; i < count; ++i) {
+            int reg = 0;
+            const int rc = solve ? host_solve<T>(a + i * n * n, n, b + i * n * k, k, x + i * n * k, &reg)
+                                 : host_lu_factor<T>(a + i * n * n, n, factors + i * n * n, perm + i * n, nullptr);
+            if (rc) return rc;
+            if (solve && !reg && k) std::memcpy(x + i * n * k, b + i * n * k, sizeof(T) * (size_t)(n * k));
+            if (solve && regular) regular[i] = reg;
+        }
+        return OK;
+    }
+    const int64_t kk = solve ? k : 0;
+    hipStream_t s = thread_stream();
+    T* da = (T*)pool_get(sizeof(T) * (size_t)(count * n * n));
+    T* db = (T*)pool_get(sizeof(T) * (size_t)std::max<int64_t>(count * n * kk, 1));
+    T* df = solve ? nullptr : (T*)pool_get(sizeof(T) * (size_t)(count * n * n));
+    int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)(count * n));
+    int* dr = (int*)pool_get(sizeof(int) * (size_t)count);
+    int rc = OK;
+    if (!da || !db || (!solve && !df) || !dp || !dr) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && (hipMemcpyAsync(da, a, sizeof(T) * count * n * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+                (kk && hipMemcpyAsync(db, b, sizeof(T) * count * n * kk, hipMemcpyHostToDevice, s) != hipSuccess)))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) {
+        std::vector<int64_t> ns((size_t)count, n), ks((size_t)count, kk), ao((size_t)count), bo((size_t)count), po((size_t)count);
+        for (int64_t i = 0; i < count; ++i) { ao[i] = i * n * n; bo[i] = i * n * kk; po[i] = i * n; }
+        long long h = 0;
+        rc = batched_plan(count, ns.data(), ks.data(), ao.data(), bo.data(), po.data(), sizeof(T) == 8, da, db, dp, dr, 0, 32, 1,
+                          &h);
+        if (!rc) rc = batched_run(h, false, s);
+        if (h) { const int rf = batched_free(h); if (!rc) rc = rf; }
+    }
+    if (!rc && solve) {
+        if ((kk && hipMemcpyAsync(x, db, sizeof(T) * count * n * kk, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+            (regular && hipMemcpyAsync(regular, dr, sizeof(int) * count, hipMemcpyDeviceToHost, s) != hipSuccess))
+            rc = hip_status(hipErrorUnknown, "D2H");
+    }
+    if (!rc && !solve) {
+        batched_to_reference<T><<<(unsigned)((count * n * n + 255) / 256), 256, 0, s>>>(count, n, da, dp, df);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(factors, df, sizeof(T) * count * n * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(perm, dp, sizeof(int64_t) * count * n, hipMemcpyDeviceToHost, s) != hipSuccess)
+            rc = hip_status(hipErrorUnknown, "D2H");
+    }
+    hipStreamSynchronize(s);
+    pool_put(da); pool_put(db); pool_put(dp); pool_put(dr);
+    if (df) pool_put(df);
+    return rc;
+}
+
+template <typename T>
 int dev_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
              T beta, T* C, int64_t ldc, void* stream) {
     bool ta, tb;
@@ -395,6 +470,18 @@ int hs_dinverse(const double* a, int64_t n, double* out, double* det, int* regul
 }
 int hs_sinverse(const float* a, int64_t n, float* out, float* det, int* regular) {
     return host_inverse<float>(a, n, out, det, regular);
+}
+int hs_dsolve_stack(int64_t count, int64_t n, int64_t k, const double* a, const double* b, double* x, int* regular) {
+    return host_stack<double>(count, n, k, a, b, x, regular, nullptr, nullptr);
+}
+int hs_ssolve_stack(int64_t count, int64_t n, int64_t k, const float* a, const float* b, float* x, int* regular) {
+    return host_stack<float>(count, n, k, a, b, x, regular, nullptr, nullptr);
+}
+int hs_dlu_stack(int64_t count, int64_t n, const double* a, double* factors, int64_t* perm) {
+    return host_stack<double>(count, n, 0, a, nullptr, nullptr, nullptr, factors, perm);
+}
+int hs_slu_stack(int64_t count, int64_t n, const float* a, float* factors, int64_t* perm) {
+    return host_stack<float>(count, n, 0, a, nullptr, nullptr, nullptr, factors, perm);
 }
 int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x, int* regular) {
     return host_solve<double>(a, n, b, k, x, regular);

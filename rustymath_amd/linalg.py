@@ -119,9 +119,13 @@ def _factor(a, name):
     cells = np.ascontiguousarray(a.reshape((int(np.prod(frame, dtype=np.int64)), n, n)).astype(compute, copy=False))
     factors = np.empty_like(cells)
     perm = np.empty((cells.shape[0], n), np.int64)
-    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_factor')
-    for i in range(cells.shape[0]):
-        _lib.check(f(_ptr(cells[i]), n, _ptr(factors[i]), _ptr(perm[i]), None), name)
+    if cells.shape[0] > 1:  # a stack: one batched launch (src/linalg/stacked.rs:174-187)
+        f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_stack')
+        _lib.check(f(cells.shape[0], n, _ptr(cells), _ptr(factors), _ptr(perm)), name)
+    else:
+        f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_factor')
+        for i in range(cells.shape[0]):
+            _lib.check(f(_ptr(cells[i]), n, _ptr(factors[i]), _ptr(perm[i]), None), name)
     return factors.reshape(frame + (n, n)), perm.reshape(frame + (n,)), store
 
 
@@ -188,13 +192,22 @@ def solve(a, b):
     ac = _cells(a.reshape(fa + (n, n)).astype(compute, copy=False), frame, (n, n))
     bc = _cells(b2.reshape(fb + (n, k)).astype(compute, copy=False), frame, (n, k))
     out = np.zeros_like(bc)
-    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'solve')
-    regular = ctypes.c_int(0)
-    for i in range(ac.shape[0]):
-        _lib.check(f(_ptr(ac[i]), n, _ptr(bc[i]), k, _ptr(out[i]), ctypes.byref(regular)), 'solve')
-        if not regular.value:
-            raise NotImplementedError('a negligible LU pivot: rustymath solves this matrix by least squares, '
-                                      'which is outside this library')
+    if ac.shape[0] > 1:  # a stack: one batched launch where the cells are small enough (stacked.rs:88-101)
+        ac, bc = np.ascontiguousarray(ac), np.ascontiguousarray(bc)
+        regular = np.zeros(ac.shape[0], np.int32)
+        f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'solve_stack')
+        _lib.check(f(ac.shape[0], n, k, _ptr(ac), _ptr(bc), _ptr(out), _ptr(regular)), 'solve')
+        ok = bool(regular.all()) or n == 0
+    else:
+        f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'solve')
+        reg = ctypes.c_int(0)
+        ok = True
+        for i in range(ac.shape[0]):
+            _lib.check(f(_ptr(ac[i]), n, _ptr(bc[i]), k, _ptr(out[i]), ctypes.byref(reg)), 'solve')
+            ok = ok and (bool(reg.value) or n == 0)
+    if not ok:
+        raise NotImplementedError('a negligible LU pivot: rustymath solves this matrix by least squares, '
+                                  'which is outside this library')
     out = out.reshape(tuple(frame) + (n, k))
     if vector: out = out[..., 0]
     return out.astype(store, copy=False)

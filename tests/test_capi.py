@@ -36,6 +36,11 @@ def test_bad_arguments_are_rejected_before_any_device_work():
     assert lib.hs_dlu_inverse(None, None, -1, None) == -3
     assert lib.hs_dlu_inverse(None, bad, 3, None) == -2
     assert lib.hs_dinverse(None, -1, None, None, None) == -2
+    assert lib.hs_dsolve_stack(-1, 2, 1, None, None, None, None) == -1
+    assert lib.hs_dsolve_stack(2, -2, 1, None, None, None, None) == -2
+    assert lib.hs_ssolve_stack(2, 2, -1, None, None, None, None) == -3
+    assert lib.hs_dlu_stack(2, -1, None, None, None) == -2
+    assert lib.hs_dlu_stack(0, 5, None, None, None) == 0
     assert lib.hs_dev_dgetri(-1, None, 1, None, None, 1, None) == -1
     assert lib.hs_dev_dgetri(4, None, 3, None, None, 4, None) == -3
     assert lib.hs_dev_sgetri(4, None, 4, None, None, 2, None) == -6

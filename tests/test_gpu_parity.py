@@ -295,3 +295,28 @@ def test_inverse_and_determinant_known_answers(la):
         assert np.allclose(sign, want.sign, rtol=0.003, atol=0.003) and np.allclose(log, want.logabsdet, rtol=0.003, atol=0.003)
     h = la.det(np.eye(3, dtype=np.float16) * 2)
     assert np.asarray(h).dtype == np.float16 and h == 8
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n,k', [(1, 1), (17, 3), (40, 8), (100, 1), (512, 2), (530, 2), (33, 9)])
+def test_stacks_match_each_cell(la, dtype, n, k):
+    """`Stacked::solve` and `Stacked::lu` (src/linalg/stacked.rs:88-101, :174-187): a stack goes to the batched kernel
+    in one launch where its cells allow (n <= 512, k <= 8) and cell by cell otherwise; either way every cell is the
+    oracle's `solve` and `lu_factor` of that cell, bit for bit."""
+    rng = np.random.default_rng(n * 10 + k)
+    count = 5
+    a = rng.standard_normal((count, n, n)).astype(dtype) + 2 * np.eye(n, dtype=dtype)
+    b = rng.standard_normal((count, n, k)).astype(dtype)
+    x = la.solve(a, b)
+    f = la.lu(a)
+    for i in range(count):
+        wf, wperm, _ = oracle.lu(a[i])
+        assert (f.p[i] == wperm).all() and bits_equal(f.factors[i], wf), f'cell {i} factors'
+        assert bits_equal(x[i], oracle.solve(a[i], b[i])), f'cell {i} solution'
+    assert bits_equal(f.solve(b), x)
+
+
+def test_a_stack_with_a_negligible_pivot_is_left_to_least_squares(la):
+    a = np.stack([np.eye(4), np.array([[1., 2, 0, 0], [2, 4, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]])])
+    with pytest.raises(NotImplementedError): la.solve(a, np.ones((2, 4, 1)))
+    assert la.solve(np.zeros((3, 0, 0)), np.zeros((3, 0, 2))).shape == (3, 0, 2)

@@ -77,6 +77,12 @@ int hs_sinverse(const float* a, int64_t n, float* out, float* det, int* regular)
 int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x, int* regular);
 int hs_ssolve(const float* a, int64_t n, const float* b, int64_t k, float* x, int* regular);
 
+/* The Cholesky factor L (row-major n×n, zeros above the diagonal) with L Lᵀ = a, reading only a's lower triangle;
+ * *ok = 0 and l untouched where a is not positive definite (a diagonal item not > 0), where the reference gives None.
+ * Replaces `Factor::matrix_cholesky` / `cholesky_factor` (factor.rs:93, :267, :777-797), in its order of operations. */
+int hs_dcholesky(const double* a, int64_t n, double* l, int* ok);
+int hs_scholesky(const float* a, int64_t n, float* l, int* ok);
+
 /* Stacks of `count` cells of one order n (row-major cells back to back): `Stacked::solve` (src/linalg/stacked.rs:88-101)
  * gives x (count×n×k) and regular (count) as hs_*solve does for each cell (x holds b where regular = 0), and
  * `Stacked::lu` (:174-187) gives factors (count×n×n, each column-major as hs_*lu_factor gives them) and perm
@@ -108,6 +114,10 @@ int hs_dev_dgetri(int64_t n, const double* factors, int64_t ldf, const int64_t* 
                   void* stream);
 int hs_dev_sgetri(int64_t n, const float* factors, int64_t ldf, const int64_t* perm, float* out, int64_t ldo,
                   void* stream);
+
+/* The Cholesky factor in place on device memory (ld lda); *ok (host) as hs_dcholesky gives it. Synchronises `stream`. */
+int hs_dev_dpotrf(int64_t n, double* a, int64_t lda, int* ok, void* stream);
+int hs_dev_spotrf(int64_t n, float* a, int64_t lda, int* ok, void* stream);
 
 /* Mixed precision (no counterpart in the reference, which has no iterative refinement): solves A x = b for
  * the n×n f64 matrix a (row-major, lda = n) and one right-hand side b (device, n f64) by factoring a copy of A

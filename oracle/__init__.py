@@ -43,6 +43,8 @@ def lib():
             f = getattr(_lib, 'or_det_' + s); f.argtypes = [vp, vp, i64]; f.restype = t
             f = getattr(_lib, 'or_slogdet_' + s); f.argtypes = [vp, vp, i64, vp, vp]; f.restype = None
             f = getattr(_lib, 'or_solve_' + s); f.argtypes = [vp, vp, i64, i64, vp]; f.restype = i32
+            for name in ('or_cholesky_', 'or_cholesky_flat_'):
+                f = getattr(_lib, name + s); f.argtypes = [vp, i64, vp]; f.restype = i32
         _lib.or_set_threads.argtypes = [i32]
         _lib.or_get_threads.restype = i32
     return _lib
@@ -138,6 +140,15 @@ def lu_inverse(factors, perm):
     out = np.zeros((n, n), factors.dtype)
     getattr(lib(), 'or_lu_inverse_' + s)(_p(factors), _p(np.ascontiguousarray(perm, dtype=np.int64)), n, _p(out))
     return out
+
+
+def cholesky(a, flat=False):
+    """`cholesky_factor` (factor.rs:777-797): L (row-major, zeros above the diagonal) with L Lᵀ = a from a's lower
+    triangle, or None where a is not positive definite. `flat` gives the left-looking statement of the same order."""
+    a = np.ascontiguousarray(a); s, n = _suffix(a.dtype), a.shape[0]
+    out = np.zeros((n, n), a.dtype)
+    ok = getattr(lib(), ('or_cholesky_flat_' if flat else 'or_cholesky_') + s)(_p(a), n, _p(out))
+    return out if ok == 1 else None
 
 
 def det(factors, perm):

@@ -110,6 +110,7 @@ static void micro_s(int64_t kc, const float* ap, const float* bp, float* c, int6
 #define FMA fma
 #define ABS fabs
 #define LOG log
+#define SQRT sqrt
 #define TMIN DBL_MIN
 #define EPS DBL_EPSILON
 #define MRT 6
@@ -121,6 +122,7 @@ static void micro_s(int64_t kc, const float* ap, const float* bp, float* c, int6
 #undef FMA
 #undef ABS
 #undef LOG
+#undef SQRT
 #undef TMIN
 #undef EPS
 #undef MRT
@@ -132,6 +134,7 @@ static void micro_s(int64_t kc, const float* ap, const float* bp, float* c, int6
 #define FMA fmaf
 #define ABS fabsf
 #define LOG logf
+#define SQRT sqrtf
 #define TMIN FLT_MIN
 #define EPS FLT_EPSILON
 #define MRT 6

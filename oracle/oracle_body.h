@@ -493,6 +493,92 @@ int F(or_lu_inverse_)(const T* factors, const int64_t* perm, int64_t n, T* out) 
     return 0;
 }
 
+/* `cholesky_block`, factor.rs:29: columns per block of `cholesky_factor`. */
+static int64_t F(cholesky_block_)(int64_t n) { int64_t w = n / 8; return w < 32 ? 32 : (w > 128 ? 128 : w); }
+
+/* `cholesky_panel`, factor.rs:751-775: columns c0..c1 of the `rows`-row block that `panel` holds column by column,
+ * `ld` apart, halving the columns down to LEAF = 16 as `lu_panel` does. 0 where a diagonal item is not > 0. */
+static int F(cholesky_panel_)(T* panel, int64_t rows, int64_t ld, int64_t c0, int64_t c1) {
+    if (c1 - c0 > 16) {
+        int64_t mid = (c0 + c1) / 2;
+        if (!F(cholesky_panel_)(panel, rows, ld, c0, mid)) return 0;
+        /* l = Matrix::rows(&left[c0 * ld + mid..], ld); right[mid..] -= l.adjoint() l, [c1 - mid, mid - c0, rows - mid] */
+        F(view_) l = F(rows_)(panel + c0 * ld + mid, ld, mid * ld - (c0 * ld + mid));
+        F(gemm_)(panel + mid * ld + mid, ld, F(transposed_)(l), l, c1 - mid, mid - c0, rows - mid, (T)-1, (T)1);
+        return F(cholesky_panel_)(panel, rows, ld, mid, c1);
+    }
+    for (int64_t c = c0; c < c1; ++c) {
+        T d = panel[c * ld + c];
+        if (!(d > (T)0)) return 0;
+        d = SQRT(d);
+        panel[c * ld + c] = d;
+        T* lc = panel + c * ld + c + 1;
+        F(divide_)(lc, rows - c - 1, 1, d);
+        /* each later column j of the leaf loses L[j][c] times column c, from row j down (`axpy`, :452) */
+        for (int64_t j = 0; j < c1 - c - 1; ++j) {
+            T* column = panel + (c + 1 + j) * ld;
+            T a = -lc[j];
+            for (int64_t r = c + 1 + j; r < rows; ++r) column[r] = F(mla_)(lc[r - c - 1], a, column[r]);
+        }
+    }
+    return 1;
+}
+
+/* `cholesky_factor`, factor.rs:777-797: L (row-major n×n, zeros above the diagonal) with L Lᵀ = a, from a's lower
+ * triangle, by blocks of `cholesky_block(n)` columns: each block's columns factor in a transposed copy (columns
+ * `stride(rows)` apart), then one product (`gram`, the lower part of -x xᵀ, linalg.rs:777-779, which never takes
+ * `by_vector`) updates the rest. Gives 1, or 0 (l untouched) where a is not positive definite. */
+int F(or_cholesky_)(const T* a, int64_t n, T* out) {
+    if (n < 0) return -2;
+    int64_t ld = F(stride_)(n);
+    T* l = (T*)calloc((size_t)(n * ld > 0 ? n * ld : 1), sizeof(T));
+    for (int64_t i = 0; i < n; ++i) memcpy(l + i * ld, a + i * n, sizeof(T) * (size_t)n);
+    int64_t width = F(cholesky_block_)(n);
+    int ok = 1;
+    for (int64_t k0 = 0; k0 < n && ok; k0 += width) {
+        int64_t k1 = k0 + width < n ? k0 + width : n;
+        int64_t nb = k1 - k0, rows = n - k0, w = n - k1, pl = F(stride_)(rows);
+        T* panel = (T*)calloc((size_t)(nb * pl), sizeof(T));
+        for (int64_t c = 0; c < nb; ++c) for (int64_t r = 0; r < rows; ++r) panel[c * pl + r] = l[(k0 + r) * ld + k0 + c];
+        ok = F(cholesky_panel_)(panel, rows, pl, 0, nb);
+        if (ok) {
+            for (int64_t r = 0; r < rows; ++r) for (int64_t c = 0; c < nb; ++c) l[(k0 + r) * ld + k0 + c] = panel[c * pl + r];
+            if (w > 0) {
+                F(view_) x = F(transposed_)(F(rows_)(panel + nb, pl, nb * pl - nb));
+                F(product_)(l + k1 * ld + k1, ld, x, (T)-1, F(transposed_)(x), w, nb, w, 0);
+            }
+        }
+        free(panel);
+    }
+    if (ok) for (int64_t i = 0; i < n; ++i) for (int64_t j = 0; j < n; ++j) out[i * n + j] = j <= i ? l[i * ld + j] : (T)0;
+    free(l);
+    return ok;
+}
+
+/* The factorization that `cholesky_factor`'s blocks and halving compute, stated flat and left-looking: item (i, j) of
+ * L, i >= j, is a[i][j] less L[i][p] L[j][p] for p = 0, 1, ..., j - 1 in that order, one fused multiply-add each,
+ * then the square root (i = j, which must be > 0) or `divide` by L[j][j]. Same arguments as or_cholesky_. */
+int F(or_cholesky_flat_)(const T* a, int64_t n, T* out) {
+    if (n < 0) return -2;
+    T* l = (T*)calloc((size_t)(n * n > 0 ? n * n : 1), sizeof(T));
+    for (int64_t j = 0; j < n; ++j) {
+        for (int64_t i = j; i < n; ++i) {
+            T acc = a[i * n + j];
+            for (int64_t p = 0; p < j; ++p) acc = F(mla_)(l[i * n + p], -l[j * n + p], acc);
+            if (i == j) {
+                if (!(acc > (T)0)) { free(l); return 0; }
+                l[j * n + j] = SQRT(acc);
+            } else {
+                l[i * n + j] = acc;
+                F(divide_)(&l[i * n + j], 1, 1, l[j * n + j]);
+            }
+        }
+    }
+    memcpy(out, l, sizeof(T) * (size_t)(n * n));
+    free(l);
+    return 1;
+}
+
 /* `odd`, factor.rs:186-194. */
 static int F(odd_)(const int64_t* perm, int64_t n) {
     int64_t cycles = 0;

@@ -28,6 +28,10 @@ SIGNATURES = {
     'hs_slu_inverse': (_i32, [_vp, _vp, _i64, _vp]),
     'hs_dinverse': (_i32, [_vp, _i64, _vp, _vp, _vp]),
     'hs_sinverse': (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    'hs_dcholesky': (_i32, [_vp, _i64, _vp, _vp]),
+    'hs_scholesky': (_i32, [_vp, _i64, _vp, _vp]),
+    'hs_dev_dpotrf': (_i32, [_i64, _vp, _i64, _vp, _vp]),
+    'hs_dev_spotrf': (_i32, [_i64, _vp, _i64, _vp, _vp]),
     'hs_dsolve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     'hs_ssolve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     'hs_dlu_stack': (_i32, [_i64, _i64, _vp, _vp, _vp]),

@@ -332,6 +332,27 @@ int host_lu_inverse(const T* factors, const int64_t* perm, int64_t n, T* out) {
     return rc;
 }
 
+// `cholesky` (factor.rs:267, `cholesky_factor` :777-797) from a host matrix: l = L (row-major, zeros above) and
+// *ok = 1, or *ok = 0 and l untouched where a is not positive definite.
+template <typename T>
+int host_cholesky(const T* a, int64_t n, T* l, int* ok) {
+    if (n < 0) return -2;
+    if (ok) *ok = 0;
+    int okv = 1;
+    if (n == 0) { if (ok) *ok = 1; return OK; }
+    hipStream_t s = thread_stream();
+    T* d = (T*)pool_get(sizeof(T) * (size_t)(n * n));
+    int rc = OK;
+    if (!d) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && hipMemcpyAsync(d, a, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) rc = potrf<T>(n, d, n, &okv, s);
+    if (!rc && okv && hipMemcpyAsync(l, d, sizeof(T) * n * n, hipMemcpyDeviceToHost, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "D2H");
+    hipStreamSynchronize(s);
+    if (d) pool_put(d);
+    if (ok) *ok = rc ? 0 : okv;
+    return rc;
+}
+
 template <typename T>
 __global__ void batched_to_reference(int64_t count, int64_t n, const T* __restrict__ a, const int64_t* __restrict__ perm,
                                      T* f) {
@@ -470,6 +491,20 @@ int hs_dinverse(const double* a, int64_t n, double* out, double* det, int* regul
 }
 int hs_sinverse(const float* a, int64_t n, float* out, float* det, int* regular) {
     return host_inverse<float>(a, n, out, det, regular);
+}
+int hs_dcholesky(const double* a, int64_t n, double* l, int* ok) { return host_cholesky<double>(a, n, l, ok); }
+int hs_scholesky(const float* a, int64_t n, float* l, int* ok) { return host_cholesky<float>(a, n, l, ok); }
+int hs_dev_dpotrf(int64_t n, double* a, int64_t lda, int* ok, void* stream) {
+    if (n < 0) return -1;
+    if (lda < n) return -3;
+    if (!ok) return -4;
+    return potrf<double>(n, a, lda, ok, (hipStream_t)stream);
+}
+int hs_dev_spotrf(int64_t n, float* a, int64_t lda, int* ok, void* stream) {
+    if (n < 0) return -1;
+    if (lda < n) return -3;
+    if (!ok) return -4;
+    return potrf<float>(n, a, lda, ok, (hipStream_t)stream);
 }
 int hs_dsolve_stack(int64_t count, int64_t n, int64_t k, const double* a, const double* b, double* x, int* regular) {
     return host_stack<double>(count, n, k, a, b, x, regular, nullptr, nullptr);

@@ -42,6 +42,12 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
 template <typename T>
 int getri(int64_t n, const T* lu, int64_t lda, const int64_t* perm, T* out, int64_t ldo, hipStream_t s);
 
+// The Cholesky factor in place (`cholesky_factor`, factor.rs:777-797): a (row-major n×n, ld lda, lower triangle read)
+// becomes L with zeros above the diagonal, in the reference's order of operations; *ok = 0 (a's contents then
+// undefined) where a diagonal item is not > 0, as the reference gives None. Synchronises `s`.
+template <typename T>
+int potrf(int64_t n, T* a, int64_t lda, int* ok, hipStream_t s);
+
 // Mixed-precision solve: FP32 factors on MFMA, FP64 iterative refinement on the GPU (ir.hip).
 int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol,
             int* iters, double* berr, hipStream_t s);

@@ -74,6 +74,17 @@ def getri(factors, perm):
     return out
 
 
+def potrf(a):
+    """The Cholesky factor in place on the row-major `a` (lower triangle read, L left with zeros above) with rustymath's
+    order of operations (factor.rs:777-797); returns (a, ok), ok False where `a` is not positive definite."""
+    import ctypes
+    _mat(a, 'a')
+    ok = ctypes.c_int(0)
+    f = getattr(_lib.load(), f'hs_dev_{_sfx(a.dtype)}potrf')
+    _lib.check(f(a.shape[0], a.data_ptr(), a.stride(0), ctypes.byref(ok), _stream()), 'potrf')
+    return a, bool(ok.value)
+
+
 def dsgesv(a, b, max_iter=30, tol=None):
     """Solves A x = b for the f64 matrix `a` (n×n, compact) and vector `b` by an FP32 LU on MFMA and FP64 iterative
     refinement on the GPU. Returns (x, corrections, backward error); `tol` defaults to 10·n·eps64."""

@@ -277,6 +277,22 @@ def slogdet(a):
     return (sign[()] if sign.ndim == 0 else sign), (log[()] if log.ndim == 0 else log)
 
 
+def cholesky(a):
+    """The lower triangular L with `L @ L.T == a` for each symmetric positive definite matrix, reading only its lower
+    triangle (src/linalg/stacked.rs:260-271 over core/src/linalg/factor.rs:777-797). A matrix that isn't positive
+    definite gives NaN, with no warning."""
+    cells, frame, compute, store = _square_cells(a, 'cholesky')
+    m, n = cells.shape[-2:]
+    if m != n: raise ValueError(f'cholesky needs square matrices, not {(m, n)}')
+    out = np.empty_like(cells)
+    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'cholesky')
+    ok = ctypes.c_int(0)
+    for i in range(cells.shape[0]):
+        _lib.check(f(_ptr(cells[i]), n, _ptr(out[i]), ctypes.byref(ok)), 'cholesky')
+        if not ok.value: out[i] = np.nan
+    return out.reshape(frame + (n, n)).astype(store, copy=False)
+
+
 def lu_lower(factors):
     "The unit lower triangle of each matrix's packed LU factors (core/src/linalg/factor.rs:158-165)."
     f = np.swapaxes(np.asarray(factors), -1, -2)

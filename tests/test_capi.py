@@ -41,6 +41,9 @@ def test_bad_arguments_are_rejected_before_any_device_work():
     assert lib.hs_ssolve_stack(2, 2, -1, None, None, None, None) == -3
     assert lib.hs_dlu_stack(2, -1, None, None, None) == -2
     assert lib.hs_dlu_stack(0, 5, None, None, None) == 0
+    assert lib.hs_dcholesky(None, -1, None, None) == -2
+    assert lib.hs_dev_dpotrf(-1, None, 1, None, None) == -1
+    assert lib.hs_dev_spotrf(4, None, 3, None, None) == -3
     assert lib.hs_dev_dgetri(-1, None, 1, None, None, 1, None) == -1
     assert lib.hs_dev_dgetri(4, None, 3, None, None, 4, None) == -3
     assert lib.hs_dev_sgetri(4, None, 4, None, None, 2, None) == -6
@@ -55,6 +58,7 @@ def test_linalg_validates_shapes_on_the_host():
     with pytest.raises(IndexError): la.Lu(np.eye(3), np.array([-1, 0, 1])).inverse()  # tests/test_linalg.py:32
     with pytest.raises(NotImplementedError): la.pinv(np.ones((2, 3)))
     with pytest.raises(ValueError): la.det(np.ones((2, 3)))
+    with pytest.raises(ValueError): la.cholesky(np.ones((2, 3)))
     with pytest.raises(TypeError): la.matmul(np.ones((2, 2), complex), np.ones((2, 2)))
 
 

@@ -320,3 +320,47 @@ def test_a_stack_with_a_negligible_pivot_is_left_to_least_squares(la):
     a = np.stack([np.eye(4), np.array([[1., 2, 0, 0], [2, 4, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]])])
     with pytest.raises(NotImplementedError): la.solve(a, np.ones((2, 4, 1)))
     assert la.solve(np.zeros((3, 0, 0)), np.zeros((3, 0, 2))).shape == (3, 0, 2)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [1, 2, 5, 31, 32, 33, 63, 64, 65, 71, 100, 129, 300, 530])
+def test_cholesky_matches_the_oracle_bit_for_bit(la, dtype, n):
+    """`cholesky_factor` (factor.rs:777-797): every item of L in the reference's order (its terms in order of p, then
+    the square root or `divide`), whatever the blocking: the oracle's bits at sizes either side of both blockings, zeros
+    above the diagonal, and only the lower triangle read (tests/linalg.rs:391-396)."""
+    from test_oracle import spd_of
+    spd = spd_of(n, dtype)
+    want = oracle.cholesky(spd)
+    junk = spd.copy(); junk[np.triu_indices(n, 1)] = 7.0
+    got = la.cholesky(junk)
+    assert got.dtype == dtype and bits_equal(got, want), f'max diff {np.abs(got - want).max()}'
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_cholesky_failures_scaling_and_device_form(la, dtype):
+    """A matrix that is not positive definite gives NaN (tests/test_linalg.py:180-181, :228-233, cell by cell in a
+    stack); scaling by an even power of two scales L exactly (tests/linalg.rs:484-486); the device entry point gives
+    the same bits; NumPy agrees to rounding (tests/test_linalg.py:109-110)."""
+    import torch
+    from rustymath_amd import device as dv
+    from test_oracle import spd_of
+    assert np.isnan(la.cholesky(-np.eye(2, dtype=dtype))).all()
+    a = np.broadcast_to(np.eye(3, dtype=dtype), (2, 8, 3, 3)).copy()
+    a[0, -1, 0, 0] = -1
+    l = la.cholesky(a)
+    assert np.isnan(l[0, -1]).all() and np.isfinite(l[1]).all() and np.isfinite(l[0, :-1]).all()
+    n = 150
+    spd = spd_of(n, dtype)
+    l = la.cholesky(spd)
+    fi = np.finfo(dtype)
+    for s in [dtype(fi.tiny / fi.eps), dtype(fi.eps / fi.tiny)]:
+        c = s if np.sqrt(s) * np.sqrt(s) == s else s + s
+        assert bits_equal(la.cholesky(spd * c), l * np.sqrt(c))
+    t, ok = dv.potrf(torch.from_numpy(spd.copy()).cuda())
+    assert ok and bits_equal(t.cpu().numpy(), l)
+    _, ok = dv.potrf(torch.from_numpy(-spd).cuda())
+    assert not ok
+    sw = spd.astype(np.float64)
+    assert np.allclose(l, np.linalg.cholesky(sw), rtol=100 * fi.eps * n, atol=1e-30)
+    h = la.cholesky(np.array([[4., 2], [2, 5]], np.float16))
+    assert h.dtype == np.float16 and (h == np.array([[2, 0], [1, 2]], np.float16)).all()

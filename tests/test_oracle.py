@@ -285,3 +285,49 @@ def test_transposed_solve(dtype):
     f, perm, _ = oracle.lu(p)
     x = oracle.lu_solve_t(f, perm, b)
     assert np.array_equal(x, np.linalg.solve(p.T.astype(np.float64), b.astype(np.float64)).astype(dtype))
+
+
+def spd_of(n, dtype):
+    "tests/linalg.rs:391-392: A Aᵀ + n I for A = random(n, n, n + 1), formed in float64 and narrowed."
+    aw = random(n, n, n + 1, dtype).astype(np.float64)
+    return (aw @ aw.T + n * np.eye(n)).astype(dtype)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [0, 1, 2, 5, 31, 32, 33, 71, 300])
+def test_cholesky(dtype, n):
+    """tests/linalg.rs:391-399: L is lower triangular with L Lᵀ near A A^T + n I, and a negative definite matrix has
+    no factor; the blocks and the halving (block width 32 here, 37 at n = 300) equal the flat left-looking statement
+    of the same order bit for bit; NumPy's (LAPACK's) factor agrees to rounding (tests/test_linalg.py:109-110)."""
+    spd = spd_of(n, dtype)
+    l = oracle.cholesky(spd)
+    assert l is not None and (np.triu(l, 1) == 0).all()
+    lw, sw = l.astype(np.float64), spd.astype(np.float64)
+    bound = 4 * np.finfo(dtype).eps * max(n, 1)
+    assert distance(lw @ lw.T, sw, max(norm(sw), 1.0)) < bound
+    assert bits_equal(l, oracle.cholesky(spd, flat=True))
+    if n: 
+        assert oracle.cholesky(-spd) is None
+        assert np.allclose(l, np.linalg.cholesky(sw), rtol=100 * np.finfo(dtype).eps * n, atol=1e-30)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_cholesky_scales_exactly(dtype):
+    "tests/linalg.rs:484-486: cholesky(A c) == cholesky(A) sqrt(c) bit for bit for an even power of two c."
+    n = 70
+    fi = np.finfo(dtype)
+    tiny = dtype(fi.tiny / fi.eps)
+    quartic = dtype(2.0 ** (int(math.log2(float(fi.max)) / 4) + 6))
+    spd = spd_of(n, dtype)
+    l = oracle.cholesky(spd)
+    for s in [tiny, dtype(1) / tiny, quartic]:
+        c = s if np.sqrt(s) * np.sqrt(s) == s else s + s
+        assert bits_equal(oracle.cholesky(spd * c), l * np.sqrt(c))
+
+
+def test_cholesky_known_answers():
+    """A hand-worked factor: [[4, 2], [2, 5]] = [[2, 0], [1, 2]] [[2, 1], [0, 2]]; only the lower triangle is read;
+    a zero or NaN diagonal has no factor (tests/test_linalg.py:180-181 gives NaN for -I)."""
+    assert bits_equal(oracle.cholesky(np.array([[4., 99], [2, 5]])), np.array([[2., 0], [1, 2]]))
+    assert oracle.cholesky(-np.eye(2)) is None and oracle.cholesky(np.zeros((3, 3))) is None
+    assert oracle.cholesky(np.array([[np.nan]])) is None

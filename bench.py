@@ -335,7 +335,7 @@ def main():
     panel = {'kernel': 'panel_kernel<float>', 'bound': 'latency (one pivot exchange per column)',
              'column_loop_ms': round(panel_s * 1e3, 3), 'us_per_column': round(panel_s / n * 1e6, 3),
              'phases_ms': {k: round(v / 1e5, 3) for k, v in zip(('reduce', 'publish_update', 'exchange', 'barrier',
-                                                                  'pivot_update'), ticks[:5])}}
+                                                                  'pivot_update', 'row_loads', 'row_stores'), ticks[:7])}}
     batched = adj = plain = variants = None
     if not args.only_mixed:
         variants = precision_variants(a, b, x64, n)

@@ -24,6 +24,7 @@ namespace {
 
 constexpr int PW = 64;            // widest panel
 constexpr int PT = 512;           // threads (rows) of a panel workgroup
+constexpr int SG = 16;            // columns a panel workgroup stages through LDS at a time, loading and storing its rows
 constexpr int MAX_WG = 32;        // panel workgroups: one per CU of one XCD, one flag per lane of a wave
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
 
@@ -151,11 +152,30 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     // per-lane condition would be issued and waited for one at a time). Items past the panel's width only ever meet
     // other such items (an item in column j takes the pivot row's item in column j), are never written, and so may
     // hold any value.
+    // The rows come in through LDS, 16 columns at a time, so that the global loads run along rows (a wave reads four
+    // rows' 16 items per load) rather than one item of 64 rows per load.
     T v[PW];
-    {
-        const T* src = a + (c0 + (has ? phys : 0)) * lda + c0;
+    __shared__ T s_stage[PT][SG + 1];
+    const int wg_rows = (int)(rows - (int64_t)g * PT < PT ? rows - (int64_t)g * PT : PT);
 #pragma unroll
-        for (int j = 0; j < PW; ++j) v[j] = src[j < w ? j : w - 1];
+    for (int ch = 0; ch < PW / SG; ++ch) {
+        T tmp[SG];
+#pragma unroll
+        for (int q = 0; q < SG; ++q) {
+            const int idx = tid + PT * q, row = idx / SG, col = ch * SG + idx % SG;
+            const int64_t r = (int64_t)g * PT + (row < wg_rows ? row : wg_rows - 1);
+            tmp[q] = a[(c0 + r) * lda + c0 + (col < w ? col : w - 1)];
+        }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < SG; ++q) {
+            const int idx = tid + PT * q;
+            s_stage[idx / SG][idx % SG] = tmp[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < SG; ++jj) v[ch * SG + jj] = s_stage[tid][jj];
+        __syncthreads();
     }
 
     __shared__ unsigned long long s_kv[2][WAVES];
@@ -175,6 +195,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     bool pending = false;
     int skip_prev = 0;
 
+    HS_PHASE(5);  // the rows' loads and the set-up
 #pragma clang loop unroll(disable)
     for (int c = 0; c < w; ++c) {
         const int buf = c & 1;
@@ -323,31 +344,44 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         prev = piv;
         HS_PHASE(4);
     }
-    if (timed)
-        for (int q = 0; q < 8; ++q) atomicAdd((unsigned long long*)&prof[q], (unsigned long long)tp[q]);
-#undef HS_PHASE
     __syncthreads();  // every row of U is in s_u11
 
-    if (has) {
-        // The row at its final position: multipliers from scratch left of it, its row of U (pivot rows) from LDS.
-        // The scratch items are all loaded before any is used (see the first loads).
-        T* dst = a + (c0 + pos) * lda + c0;
+    // The row at its final position: multipliers from scratch left of it, its row of U (pivot rows) from LDS; the
+    // scratch items are all loaded before any is used (as the first loads). The rows go out through LDS, 16 columns
+    // at a time, each wave storing four rows' 16 items per store.
+    {
 #pragma unroll
-        for (int j = 0; j < PW; ++j) v[j] = scratch[(int64_t)(j < w ? j : w - 1) * rows + phys];
+        for (int j = 0; j < PW; ++j) v[j] = scratch[(int64_t)(j < w ? j : w - 1) * rows + (has ? phys : 0)];
         asm volatile("" ::: "memory");
         const int upos = pos < PW ? pos : PW - 1;
-        if (w == PW) {
 #pragma unroll
-            for (int j = 0; j < PW; ++j) dst[j] = j < pos ? v[j] : s_u11[upos][(j - pos) & (PW - 1)];
-        } else {
-            for (int j = 0; j < w; ++j) dst[j] = j < pos ? scratch[(int64_t)j * rows + phys] : s_u11[upos][j - pos];
+        for (int j = 0; j < PW; ++j) v[j] = j < pos ? v[j] : s_u11[upos][(j - pos) & (PW - 1)];
+        __shared__ int s_dest[PT];
+        s_dest[tid] = pos;
+#pragma unroll
+        for (int ch = 0; ch < PW / SG; ++ch) {
+#pragma unroll
+            for (int jj = 0; jj < SG; ++jj) s_stage[tid][jj] = v[ch * SG + jj];
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < SG; ++q) {
+                const int idx = tid + PT * q, row = idx / SG, col = ch * SG + idx % SG;
+                if (row < wg_rows && col < w) a[(c0 + s_dest[row]) * lda + c0 + col] = s_stage[row][idx % SG];
+            }
+            __syncthreads();
         }
-        if (pos != phys) {
+        if (has && pos != phys) {
             const long long k = atomicAdd((unsigned long long*)moves, 1ull);
             moves[1 + 2 * k] = c0 + pos;
             moves[2 + 2 * k] = c0 + phys;
         }
     }
+    if (timed) {
+        __builtin_amdgcn_s_waitcnt(0);  // the row's stores issued above have completed
+        HS_PHASE(6);                    // the rows written back at their final positions
+        for (int q = 0; q < 8; ++q) atomicAdd((unsigned long long*)&prof[q], (unsigned long long)tp[q]);
+    }
+#undef HS_PHASE
 }
 
 // Applies a panel's net row permutation to the columns outside it: for columns [0, c0) and [c1, n),

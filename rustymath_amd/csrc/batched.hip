@@ -280,13 +280,13 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                                 if (j0 < w - c) {  // items past the panel are never used
 #pragma unroll
                                     for (int j = j0; j < j0 + 16 && j < BW; ++j)
-                                        v[j - 1] = __builtin_fma(l, -lane_value(mine, j), v[j]);
+                                        v[j - 1] = __builtin_elementwise_fma(l, -lane_value(mine, j), v[j]);
                                 }
                             }
                         } else {
                             const int skip = leaf_end[c0 + c] - c0 - c;
 #pragma unroll
-                            for (int j = 1; j < BW; ++j) v[j - 1] = j >= skip ? __builtin_fma(l, -piv[j], v[j]) : v[j];
+                            for (int j = 1; j < BW; ++j) v[j - 1] = j >= skip ? __builtin_elementwise_fma(l, -piv[j], v[j]) : v[j];
                         }
                         v[BW - 1] = T(0);
                         a[row * n + c0 + c] = l;
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                         const T x = u[0];
                         if (live) a[perm[c0 + qq] * n + col] = x;
 #pragma unroll
-                        for (int j = 1; j < BW; ++j) u[j - 1] = __builtin_fma(-s_l11[qq + j < BW ? qq + j : BW - 1][qq], x, u[j]);
+                        for (int j = 1; j < BW; ++j) u[j - 1] = __builtin_elementwise_fma(-s_l11[qq + j < BW ? qq + j : BW - 1][qq], x, u[j]);
                         u[BW - 1] = T(0);
                     }
                     __syncthreads();
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                         if (j < h) {
                             if (tr && lane == j) x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
                             const T xj = lane_value(x, j);
-                            if (act && lane > j) x = __builtin_fma(xj, -mj[j], x);
+                            if (act && lane > j) x = __builtin_elementwise_fma(xj, -mj[j], x);
                         }
                     }
                     if (act) s_x[r][wave] = x;
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                         if (c < k) {
                             T acc = s_x[tid][c];
 #pragma unroll
-                            for (int j = 0; j < SB; ++j) if (j < h) acc = __builtin_fma(s_x[p0 + j][c], e[j], acc);
+                            for (int j = 0; j < SB; ++j) if (j < h) acc = __builtin_elementwise_fma(s_x[p0 + j][c], e[j], acc);
                             s_x[tid][c] = acc;
                         }
                     }
@@ -415,7 +415,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                             if (s < h) {
                                 if (!tr && lane == L) x = x / dg;
                                 const T xl = lane_value(x, L);
-                                if (act && lane < L) x = __builtin_fma(xl, -mj[L], x);
+                                if (act && lane < L) x = __builtin_elementwise_fma(xl, -mj[L], x);
                             }
                         }
                         if (act) s_x[r][wave] = x;
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                             if (c < k) {
                                 T acc = s_x[tid][c];
 #pragma unroll
-                                for (int j = SB - 1; j >= 0; --j) if (j < h) acc = __builtin_fma(s_x[q0 + j][c], e[j], acc);
+                                for (int j = SB - 1; j >= 0; --j) if (j < h) acc = __builtin_elementwise_fma(s_x[q0 + j][c], e[j], acc);
                                 s_x[tid][c] = acc;
                             }
                         }
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                             if (s < h) {
                                 if (lane == L) {
 #pragma unroll
-                                    for (int j = L + 1; j < SB; ++j) x = __builtin_fma(xs[j], -mj[j], x);
+                                    for (int j = L + 1; j < SB; ++j) x = __builtin_elementwise_fma(xs[j], -mj[j], x);
                                     x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
                                 }
                                 xs[L] = lane_value(x, L);
@@ -485,7 +485,7 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                                             T acc = s_x[tid][c];
 #pragma unroll
                                             for (int j = 0; j < SB; ++j)
-                                                if (j0 + j < a1) acc = __builtin_fma(s_x[j0 + j][c], e[j], acc);
+                                                if (j0 + j < a1) acc = __builtin_elementwise_fma(s_x[j0 + j][c], e[j], acc);
                                             s_x[tid][c] = acc;
                                         }
                                     }

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(64) void chol_diag_kernel(T* a, int64_t lda, int64_
             for (int j = c + 1; j < CB; ++j) {
                 if (j < nb) {
                     const T ljc = lane_value(v[c], j);
-                    if (lane >= j) v[j] = __builtin_fma(v[c], -ljc, v[j]);
+                    if (lane >= j) v[j] = __builtin_elementwise_fma(v[c], -ljc, v[j]);
                 }
             }
         }
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(T* a, int64_t lda, int6
         if (c < nb) {
             T acc = v[c];
 #pragma unroll
-            for (int p = 0; p < c; ++p) acc = __builtin_fma(v[p], -L[c][p], acc);
+            for (int p = 0; p < c; ++p) acc = __builtin_elementwise_fma(v[p], -L[c][p], acc);
             v[c] = divided(acc, L[c][c], R[c]);
         }
     }

@@ -289,8 +289,8 @@ __global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ 
         if (k <= 8) {
             T s0 = 0, s1 = 0;
             for (int64_t l = 0; l < k; ++l) {
-                if (l % 2 == 0) s0 = __builtin_fma(row[l], vec[l * vstride], s0);
-                else s1 = __builtin_fma(row[l], vec[l * vstride], s1);
+                if (l % 2 == 0) s0 = __builtin_elementwise_fma(row[l], vec[l * vstride], s0);
+                else s1 = __builtin_elementwise_fma(row[l], vec[l * vstride], s1);
             }
             y = s0 + s1;
         } else {
@@ -298,7 +298,7 @@ __global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ 
             int64_t whole = k / 8 * 8;
             for (int64_t c = 0; c < whole; c += 8)
 #pragma unroll
-                for (int l = 0; l < 8; ++l) s[l] = __builtin_fma(row[c + l], vec[(c + l) * vstride], s[l]);
+                for (int l = 0; l < 8; ++l) s[l] = __builtin_elementwise_fma(row[c + l], vec[(c + l) * vstride], s[l]);
             T tsum;
             if (i < rows / 4 * 4) tsum = ((s[0] + s[4]) + (s[2] + s[6])) + ((s[1] + s[5]) + (s[3] + s[7]));
             else {
@@ -306,7 +306,7 @@ __global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ 
 #pragma unroll
                 for (int l = 0; l < 8; ++l) tsum = tsum + s[l];
             }
-            for (int64_t p = whole; p < k; ++p) tsum = __builtin_fma(row[p], vec[p * vstride], tsum);
+            for (int64_t p = whole; p < k; ++p) tsum = __builtin_elementwise_fma(row[p], vec[p * vstride], tsum);
             y = tsum;
         }
     } else if (rows <= 8) {
@@ -317,17 +317,17 @@ __global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ 
         int64_t quads = whole / 4;
         for (int64_t q = 0; q < quads; ++q)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) acc[s] = __builtin_fma(mat[(4 * q + s) * ld + i], vec[(4 * q + s) * vstride], acc[s]);
-        for (int64_t p = quads * 4; p < k; ++p) acc[0] = __builtin_fma(mat[p * ld + i], vec[p * vstride], acc[0]);
+            for (int s = 0; s < 4; ++s) acc[s] = __builtin_elementwise_fma(mat[(4 * q + s) * ld + i], vec[(4 * q + s) * vstride], acc[s]);
+        for (int64_t p = quads * 4; p < k; ++p) acc[0] = __builtin_elementwise_fma(mat[p * ld + i], vec[p * vstride], acc[0]);
         y = (acc[0] + acc[2]) + (acc[1] + acc[3]);
     } else {
         T a = 0;
-        for (int64_t p = 0; p < k; ++p) a = __builtin_fma(vec[p * vstride], mat[p * ld + i], a);
+        for (int64_t p = 0; p < k; ++p) a = __builtin_elementwise_fma(vec[p * vstride], mat[p * ld + i], a);
         y = a;
     }
     if (alpha != T(1)) y = alpha * y;
     T* z = &C[i * step];
-    *z = beta == T(0) ? y : __builtin_fma(beta, *z, y);
+    *z = beta == T(0) ? y : __builtin_elementwise_fma(beta, *z, y);
 }
 
 }  // namespace

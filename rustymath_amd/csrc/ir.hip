@@ -23,12 +23,12 @@ __global__ __launch_bounds__(256) void residual_kernel(int64_t n, const double* 
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int64_t j = lane;
     for (; j + 192 < n; j += 256) {
-        s0 = __builtin_fma(ar[j], x[j], s0);
-        s1 = __builtin_fma(ar[j + 64], x[j + 64], s1);
-        s2 = __builtin_fma(ar[j + 128], x[j + 128], s2);
-        s3 = __builtin_fma(ar[j + 192], x[j + 192], s3);
+        s0 = __builtin_elementwise_fma(ar[j], x[j], s0);
+        s1 = __builtin_elementwise_fma(ar[j + 64], x[j + 64], s1);
+        s2 = __builtin_elementwise_fma(ar[j + 128], x[j + 128], s2);
+        s3 = __builtin_elementwise_fma(ar[j + 192], x[j + 192], s3);
     }
-    for (; j < n; j += 64) s0 = __builtin_fma(ar[j], x[j], s0);
+    for (; j < n; j += 64) s0 = __builtin_elementwise_fma(ar[j], x[j], s0);
     double s = (s0 + s1) + (s2 + s3);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) r[row] = b[row] - s;
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void sumsq_kernel(int64_t count, const double*
     __shared__ double s[4];
     double acc = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-        acc = __builtin_fma(v[i], v[i], acc);
+        acc = __builtin_elementwise_fma(v[i], v[i], acc);
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -67,7 +67,7 @@ __global__ void scale_to_f32(int64_t n, const double* __restrict__ r, double sca
 // x += d32 / scale, in FP64.
 __global__ void add_correction(int64_t n, const float* __restrict__ d32, double inv_scale, double* __restrict__ x) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = __builtin_fma((double)d32[i], inv_scale, x[i]);
+    if (i < n) x[i] = __builtin_elementwise_fma((double)d32[i], inv_scale, x[i]);
 }
 
 // y = A v, one wave per row (as residual_kernel).
@@ -80,12 +80,12 @@ __global__ __launch_bounds__(256) void matvec_kernel(int64_t n, const double* __
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int64_t j = lane;
     for (; j + 192 < n; j += 256) {
-        s0 = __builtin_fma(ar[j], v[j], s0);
-        s1 = __builtin_fma(ar[j + 64], v[j + 64], s1);
-        s2 = __builtin_fma(ar[j + 128], v[j + 128], s2);
-        s3 = __builtin_fma(ar[j + 192], v[j + 192], s3);
+        s0 = __builtin_elementwise_fma(ar[j], v[j], s0);
+        s1 = __builtin_elementwise_fma(ar[j + 64], v[j + 64], s1);
+        s2 = __builtin_elementwise_fma(ar[j + 128], v[j + 128], s2);
+        s3 = __builtin_elementwise_fma(ar[j + 192], v[j + 192], s3);
     }
-    for (; j < n; j += 64) s0 = __builtin_fma(ar[j], v[j], s0);
+    for (; j < n; j += 64) s0 = __builtin_elementwise_fma(ar[j], v[j], s0);
     double s = (s0 + s1) + (s2 + s3);
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) y[row] = s;
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(256) void dots_kernel(int64_t n, const double* __re
     __shared__ double part[4];
     const double* v = V + (int64_t)blockIdx.x * n;
     double acc = 0;
-    for (int64_t e = threadIdx.x; e < n; e += 256) acc = __builtin_fma(v[e], w[e], acc);
+    for (int64_t e = threadIdx.x; e < n; e += 256) acc = __builtin_elementwise_fma(v[e], w[e], acc);
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void combine_kernel(int64_t n, const double* _
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     double acc = 0;
-    for (int i = 0; i < count; ++i) acc = __builtin_fma(coef[i], V[(int64_t)i * n + e], acc);
-    out[e] = __builtin_fma(sign, acc, out[e]);
+    for (int i = 0; i < count; ++i) acc = __builtin_elementwise_fma(coef[i], V[(int64_t)i * n + e], acc);
+    out[e] = __builtin_elementwise_fma(sign, acc, out[e]);
 }
 
 // dst = src · f.

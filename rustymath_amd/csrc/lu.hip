@@ -242,7 +242,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     item = h[lane];
                 } else {
                     const int j = lane + 1 < PW ? lane + 1 : PW - 1;
-                    const T u = __builtin_fma(h[PW + 1], -prev[j], h[j]);
+                    const T u = __builtin_elementwise_fma(h[PW + 1], -prev[j], h[j]);
                     item = lane == 0 ? h[PW] : (lane == PW - 1 ? T(0) : (j >= skip_prev ? u : h[j]));
                 }
                 if (nwg > 1) {
@@ -268,14 +268,14 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     if (j0 <= rem) {
 #pragma unroll
                         for (int j = j0; j < j0 + 16 && j < PW; ++j)
-                            v[j - 1] = __builtin_fma(l_prev, -lane_value(mine, j), v[j]);
+                            v[j - 1] = __builtin_elementwise_fma(l_prev, -lane_value(mine, j), v[j]);
                     }
                 }
             } else {  // a zero pivot: the rest of its leaf (at most 16 columns) keeps its items
 #pragma unroll
-                for (int j = 2; j < 16; ++j) v[j - 1] = j >= skip_prev ? __builtin_fma(l_prev, -pp[j], v[j]) : v[j];
+                for (int j = 2; j < 16; ++j) v[j - 1] = j >= skip_prev ? __builtin_elementwise_fma(l_prev, -pp[j], v[j]) : v[j];
 #pragma unroll
-                for (int j = 16; j < PW; ++j) v[j - 1] = __builtin_fma(l_prev, -pp[j], v[j]);
+                for (int j = 16; j < PW; ++j) v[j - 1] = __builtin_elementwise_fma(l_prev, -pp[j], v[j]);
             }
             v[PW - 1] = T(0);
             pending = false;
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
             const bool zero = pivot == T(0);
             const int skip = zero ? s_skip[c] : 0;
             const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * (T(1) / pivot) : x0 / pivot);
-            x0 = 1 >= skip ? __builtin_fma(l, -piv[1], v[1]) : v[1];
+            x0 = 1 >= skip ? __builtin_elementwise_fma(l, -piv[1], v[1]) : v[1];
             scratch[(int64_t)c * rows + phys] = l;
             l_prev = l;
             skip_prev = skip;
@@ -463,7 +463,7 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(T* __restrict__ a, int6
 #pragma unroll
             for (int q = 0; q < 16; ++q)
 #pragma unroll
-                for (int r = q + 1; r < 16; ++r) x[r] = __builtin_fma(-Ls[k0 + r][k0 + q], x[q], x[r]);
+                for (int r = q + 1; r < 16; ++r) x[r] = __builtin_elementwise_fma(-Ls[k0 + r][k0 + q], x[q], x[r]);
 #pragma unroll
             for (int i = 0; i < 16; ++i) Xs[k0 + i][lane] = x[i];
         }

@@ -141,14 +141,13 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
             __builtin_amdgcn_s_sleep(2);  // many blocks wait on the same producer: poll its words gently
             if (++spins > SPIN_LIMIT) { if (lane == 0) atomicExch(status, ERR_LIMIT); return; }
         }
+        // Rows past the producer's end hold +0 with zero items beside them: their terms add -0, which changes no sum.
         if (fwd) {
 #pragma unroll
-            for (int j = 0; j < B1; ++j)
-                if (j < ph) acc = __builtin_fma(lane_value(xv, j), -e[j], acc);
+            for (int j = 0; j < B1; ++j) acc = __builtin_elementwise_fma(lane_value(xv, j), -e[j], acc);
         } else {
 #pragma unroll
-            for (int j = B1 - 1; j >= 0; --j)
-                if (j < ph) acc = __builtin_fma(lane_value(xv, j), -e[j], acc);
+            for (int j = B1 - 1; j >= 0; --j) acc = __builtin_elementwise_fma(lane_value(xv, j), -e[j], acc);
         }
 #pragma unroll
         for (int j = 0; j < B1; ++j) e[j] = en[j];
@@ -165,7 +164,7 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
             if (j < h) {
                 if (mode == FWD_DIAG && lane == j) acc = fabs(dg) >= tiny ? acc * rdg : acc / dg;
                 const T xj = lane_value(acc, j);
-                if (lane > j) acc = __builtin_fma(xj, -d[j], acc);
+                if (lane > j) acc = __builtin_elementwise_fma(xj, -d[j], acc);
             }
         }
     } else {
@@ -174,7 +173,7 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
             if (j < h) {
                 if (mode == BWD_SINGLE && lane == j) acc = acc / dg;
                 const T xj = lane_value(acc, j);
-                if (lane < j) acc = __builtin_fma(xj, -d[j], acc);
+                if (lane < j) acc = __builtin_elementwise_fma(xj, -d[j], acc);
             }
         }
     }
@@ -285,7 +284,7 @@ __global__ __launch_bounds__(64 * SW_WAVES) void sweep_kernel(int64_t r0, int m,
             }
             if (MODE == BWD_UNIT ? me < tt : me > tt) {
 #pragma unroll
-                for (int j = 0; j < SW_NC; ++j) xv[j] = __builtin_fma(bc[j], -e[tt], xv[j]);
+                for (int j = 0; j < SW_NC; ++j) xv[j] = __builtin_elementwise_fma(bc[j], -e[tt], xv[j]);
             }
         }
 #pragma unroll
@@ -337,7 +336,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             for (int q = split; q < m; ++q) {
                 const T xq = xb[q][lane];
 #pragma unroll
-                for (int j = 0; j < BR / 4; ++j) acc[j] = __builtin_fma(xq, -E[wave + 4 * j][q], acc[j]);
+                for (int j = 0; j < BR / 4; ++j) acc[j] = __builtin_elementwise_fma(xq, -E[wave + 4 * j][q], acc[j]);
             }
 #pragma unroll
             for (int j = 0; j < BR / 4; ++j)
@@ -366,7 +365,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                     __builtin_amdgcn_sched_barrier(0);  // keep those loads ahead of this row's chain
                     T acc = xr[u];
 #pragma unroll
-                    for (int q = u + 1; q < BR; ++q) acc = __builtin_fma(xr[q], -ec[q], acc);
+                    for (int q = u + 1; q < BR; ++q) acc = __builtin_elementwise_fma(xr[q], -ec[q], acc);
                     xr[u] = acc * rc;
 #pragma unroll
                     for (int q = 0; q < BR; ++q) ec[q] = en[q];
@@ -378,7 +377,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             } else {
                 for (int row = hi - 1; row >= lo; --row) {
                     T acc = xb[row][lane];
-                    for (int q = row + 1; q < hi; ++q) acc = __builtin_fma(xb[q][lane], -E[row][q], acc);
+                    for (int q = row + 1; q < hi; ++q) acc = __builtin_elementwise_fma(xb[q][lane], -E[row][q], acc);
                     const T dg = E[row][row];
                     xb[row][lane] = fabs(dg) >= tiny ? acc * rdg[BR + row] : acc / dg;
                 }

@@ -364,3 +364,17 @@ def test_cholesky_failures_scaling_and_device_form(la, dtype):
     assert np.allclose(l, np.linalg.cholesky(sw), rtol=100 * fi.eps * n, atol=1e-30)
     h = la.cholesky(np.array([[4., 2], [2, 5]], np.float16))
     assert h.dtype == np.float16 and (h == np.array([[2, 0], [1, 2]], np.float16)).all()
+
+
+@pytest.mark.parametrize('k', [1, 3])
+def test_float_fused_multiply_adds_round_once(la, k):
+    """Every FP32 fused multiply-add rounds once, as rustymath's: the case of test_oracle's
+    test_float_products_round_once, where a product through double would round twice, through the single-column
+    and the block substitutions."""
+    f = np.float32
+    a = np.array([[1, 0], [-(0.5 + 2.0 ** -13), 1]], f)
+    b = np.repeat(np.array([[2 + 2.0 ** -11], [2.0 ** -60]], f), k, axis=1)
+    x = la.solve(a, b)
+    assert (x[1] == f(1 + 2.0 ** -11 + 2.0 ** -23)).all()
+    fac = la.lu(a)
+    assert bits_equal(fac.solve(b), oracle.lu_solve(*oracle.lu(a)[:2], b))

@@ -331,3 +331,16 @@ def test_cholesky_known_answers():
     assert bits_equal(oracle.cholesky(np.array([[4., 99], [2, 5]])), np.array([[2., 0], [1, 2]]))
     assert oracle.cholesky(-np.eye(2)) is None and oracle.cholesky(np.zeros((3, 3))) is None
     assert oracle.cholesky(np.array([[np.nan]])) is None
+
+
+def test_float_products_round_once():
+    """A float FMA whose product is a float32 midpoint and whose addend is tiny: rounding once gives the upper
+    neighbour, rounding through double first (an FP64 FMA of promoted operands) lands on the midpoint and then on the
+    even neighbour below. x1 = b1 - x0 L10 for L = [[1, 0], [-(1/2 + 2^-13), 1]], x0 = 2 + 2^-11, b1 = 2^-60."""
+    f = np.float32
+    a = np.array([[1, 0], [-(0.5 + 2.0 ** -13), 1]], f)
+    b = np.array([[2 + 2.0 ** -11], [2.0 ** -60]], f)
+    fac, perm, _ = oracle.lu(a)
+    x = oracle.lu_solve(fac, perm, b)
+    assert (perm == [0, 1]).all() and x[1, 0] == f(1 + 2.0 ** -11 + 2.0 ** -23)
+    assert x[1, 0] != f(np.float64(b[0, 0]) * np.float64(0.5 + 2.0 ** -13) + np.float64(b[1, 0]))

@@ -331,11 +331,12 @@ def main():
             'kernel': 'gemm_mfma<float> (LU trailing updates)', 'launches': gl.value,
             'flop_per_launch': round(per_launch), 'algorithmic_bytes_per_launch': round(gb.value / gl.value),
             'avg_launch_us': round(avg_s * 1e6, 2)}
-    panel_s = sum(ticks[:5]) / 1e8
+    panel_s = (sum(ticks[:5]) + ticks[7]) / 1e8
     panel = {'kernel': 'panel_kernel<float>', 'bound': 'latency (one pivot exchange per column)',
              'column_loop_ms': round(panel_s * 1e3, 3), 'us_per_column': round(panel_s / n * 1e6, 3),
-             'phases_ms': {k: round(v / 1e5, 3) for k, v in zip(('reduce', 'publish_update', 'exchange', 'barrier',
-                                                                  'pivot_update', 'row_loads', 'row_stores'), ticks[:7])}}
+             'phases_ms': {k: round(ticks[q] / 1e5, 3) for k, q in (('reduce', 0), ('publish', 7), ('update', 1),
+                                                                     ('exchange', 2), ('barrier', 3), ('pivot_update', 4),
+                                                                     ('row_loads', 5), ('row_stores', 6))}}
     batched = adj = plain = variants = None
     if not args.only_mixed:
         variants = precision_variants(a, b, x64, n)

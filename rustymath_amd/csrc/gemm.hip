@@ -7,6 +7,7 @@
 // accumulator per result for the whole of k and feeds the MFMA unit in order of p, so it computes the
 // same chain of roundings; a k tail pads with +0 · (-0) = -0, which adds nothing to any value.
 #include "hs_kernels.h"
+#include "gemm_tile.h"
 #include <hip/hip_ext.h>
 #include <mutex>
 #include <vector>
@@ -45,10 +46,12 @@ int gemm_profile(int on, double* ms, int64_t* launches, double* flops, double* b
 
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 16, THREADS = 256;
-
-template <typename T> struct Pad { static constexpr int v = 2; };
-template <> struct Pad<float> { static constexpr int v = 4; };
+constexpr int THREADS = 256;
+using tile::BM;
+using tile::BN;
+using tile::BK;
+using tile::Pad;
+using tile::gemm_tile;
 
 // Maps a launch index to an output tile so that each XCD (dispatch is round-robin over the 8 XCDs)
 // works on one contiguous run of tiles, and runs of 8 tile-rows share B's column panels in its L2.
@@ -57,192 +60,8 @@ __device__ __forceinline__ bool tile_of(int w, int tiles_m, int tiles_n, int& tm
     int per = (total + 7) / 8;
     int id = (w % 8) * per + w / 8;
     if (id >= total) return false;
-    constexpr int G = 8;
-    int group = id / (G * tiles_n);
-    int first = group * G;
-    int gm = tiles_m - first < G ? tiles_m - first : G;
-    int in = id - group * G * tiles_n;
-    tm = first + in % gm;
-    tn = in / gm;
+    tile::grouped(id, tiles_m, tiles_n, tm, tn);
     return true;
-}
-
-// One 128x128 tile of c = alpha op(a) op(b) + beta c. EDGE selects the path for tiles that cross the last row or
-// column: there every address is clamped into the matrix and the value replaced afterwards, so no load sits behind a
-// per-lane branch (a guarded load makes the compiler wait for each load before the next). Interior tiles load
-// without checks; a k tail (the last step when BK does not divide k) takes the clamped loads in either path.
-template <typename T, bool TA, bool TB, bool EDGE>
-__device__ __forceinline__ void gemm_tile(int64_t m, int64_t n, int64_t k, T alpha, const T* __restrict__ A, int64_t lda,
-                                          const T* __restrict__ B, int64_t ldb, T beta, T* __restrict__ C, int64_t ldc,
-                                          int64_t m0, int64_t n0, T (*As)[BK][BM + Pad<T>::v], T (*Bs)[BK][BN + Pad<T>::v]) {
-    typedef typename Mfma<T>::acc Acc;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const bool scale_a = alpha != T(1);
-
-    // Accumulators start at 0, or at beta c (c itself where beta is 1), as `native_gemm` scales c first.
-    Acc acc[4][4];
-    if (beta != T(0)) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
-                    T v;
-                    if (EDGE) {
-                        const int64_t rr = r < m ? r : m - 1, cc = col < n ? col : n - 1;
-                        v = C[rr * ldc + cc];
-                    } else {
-                        v = C[r * ldc + col];
-                    }
-                    acc[i][j][e] = beta != T(1) ? beta * v : v;
-                }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
-    }
-
-    // Global loads: each thread brings 8 items of A's tile and 8 of B's per step of BK. Items outside the matrix are
-    // +0 on A's side and, past k, -0 on B's side, so that padding terms add (+0)(-0) = -0, which changes no sum. alpha
-    // scales only the items that exist (bits of amask), when they are stored to LDS: the loads of the next step stay in
-    // flight while the MFMAs of this one run.
-    T ra[8], rb[8];
-    unsigned amask = 0xffu;
-    auto load = [&](int64_t p0) {
-        const bool checked = EDGE || p0 + BK > k;
-        if (!TA) {  // A row-major m×k: thread -> row t/2, steps (t%2)*8 ..+8
-            const int64_t r = m0 + (t >> 1), p = p0 + (t & 1) * 8;
-            if (!checked) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) ra[q] = A[r * lda + p + q];
-            } else {
-                const int64_t rr = r < m ? r : m - 1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int64_t pp = p + q < k ? p + q : k - 1;
-                    const T v = A[rr * lda + pp];
-                    ra[q] = (r < m && p + q < k) ? v : T(0);
-                }
-            }
-            amask = 0xffu;
-            if (checked) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) amask &= (r < m && p + q < k) ? 0xffu : ~(1u << q);
-            }
-        } else {  // A stored k×m: thread -> step t/16, rows (t%16)*8 ..+8
-            const int64_t p = p0 + (t >> 4), r = m0 + (t & 15) * 8;
-            if (!checked) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) ra[q] = A[p * lda + r + q];
-            } else {
-                const int64_t pp = p < k ? p : k - 1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int64_t rr = r + q < m ? r + q : m - 1;
-                    const T v = A[pp * lda + rr];
-                    ra[q] = (p < k && r + q < m) ? v : T(0);
-                }
-            }
-            amask = 0xffu;
-            if (checked) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) amask &= (p < k && r + q < m) ? 0xffu : ~(1u << q);
-            }
-        }
-        if (!TB) {  // B row-major k×n: thread -> step t/16, columns (t%16)*8 ..+8
-            const int64_t p = p0 + (t >> 4), c = n0 + (t & 15) * 8;
-            if (!checked) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) rb[q] = B[p * ldb + c + q];
-            } else {
-                const int64_t pp = p < k ? p : k - 1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int64_t cc = c + q < n ? c + q : n - 1;
-                    const T v = B[pp * ldb + cc];
-                    rb[q] = p < k ? ((c + q < n) ? v : T(0)) : T(-0.0);
-                }
-            }
-        } else {  // B stored n×k: thread -> column t/2, steps (t%2)*8 ..+8
-            const int64_t c = n0 + (t >> 1), p = p0 + (t & 1) * 8;
-            if (!checked) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) rb[q] = B[c * ldb + p + q];
-            } else {
-                const int64_t cc = c < n ? c : n - 1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int64_t pp = p + q < k ? p + q : k - 1;
-                    const T v = B[cc * ldb + pp];
-                    rb[q] = p + q < k ? ((c < n) ? v : T(0)) : T(-0.0);
-                }
-            }
-        }
-    };
-    auto stash = [&](int buf) {
-        if (scale_a) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) ra[q] = ((amask >> q) & 1u) ? alpha * ra[q] : ra[q];
-        }
-        if (!TA) {
-            const int r = t >> 1, p = (t & 1) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) As[buf][p + q][r] = ra[q];
-        } else {
-            const int p = t >> 4, r = (t & 15) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) As[buf][p][r + q] = ra[q];
-        }
-        if (!TB) {
-            const int p = t >> 4, c = (t & 15) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) Bs[buf][p][c + q] = rb[q];
-        } else {
-            const int c = t >> 1, p = (t & 1) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) Bs[buf][p + q][c] = rb[q];
-        }
-    };
-
-    const int64_t steps = (k + BK - 1) / BK;
-    load(0);
-    stash(0);
-    __syncthreads();
-    for (int64_t s = 0; s < steps; ++s) {
-        const int buf = s & 1;
-        if (s + 1 < steps) load((s + 1) * BK);
-#pragma unroll
-        for (int kk = 0; kk < BK / 4; ++kk) {
-            T fa[4], fb[4];
-            const int kr = kk * 4 + (lane >> 4), li = lane & 15;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) fa[i] = As[buf][kr][wm * 64 + i * 16 + li];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) fb[j] = Bs[buf][kr][wn * 64 + j * 16 + li];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::step(fa[i], fb[j], acc[i][j]);
-        }
-        if (s + 1 < steps) stash(buf ^ 1);
-        __syncthreads();
-    }
-
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t r = m0 + wm * 64 + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
-                if (!EDGE || (r < m && col < n)) C[r * ldc + col] = acc[i][j][e];
-            }
 }
 
 template <typename T, bool TA, bool TB>
@@ -257,9 +76,9 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, in
     if (!tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn)) return;
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     if (m0 + BM <= m && n0 + BN <= n)
-        gemm_tile<T, TA, TB, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
+        gemm_tile<T, TA, TB, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs, threadIdx.x, true);
     else
-        gemm_tile<T, TA, TB, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs);
+        gemm_tile<T, TA, TB, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs, threadIdx.x, true);
 }
 
 // `scale` (core/src/linalg.rs:1000-1008): c = beta c, zeros for beta = 0.

@@ -14,6 +14,7 @@
 // ("row j holds column j", src/linalg/stacked.rs:25).
 #include "hs_kernels.h"
 #include "hs_pivot.h"
+#include "gemm_tile.h"
 #include <vector>
 #include <mutex>
 #include <algorithm>
@@ -112,6 +113,103 @@ __device__ __forceinline__ bool get_item(const unsigned long long* r, int j, uns
     }
 }
 
+// A trailing product left to run beside the panels (look-ahead): C (m×n) less A (m×k) B (k×n), all row-major, in
+// 128x128 tiles numbered as tile::grouped numbers them. Workers take tiles in order of number from *next; the panel's
+// workgroup 0 writes its launch tag to *stop when its columns are done, after which no worker takes another tile, and
+// trail_rest computes the tiles from *next on. C = nullptr: no product.
+template <typename T>
+struct TrailJob {
+    int64_t m, n, k;
+    const T* A;
+    int64_t lda;
+    const T* B;
+    int64_t ldb;
+    T* C;
+    int64_t ldc;
+    int tiles_m, tiles_n;
+    int* next;
+    unsigned* stop;
+};
+
+// The panel kernel's LDS, and the workers', in one allocation.
+template <typename T>
+struct PanelLds {
+    T s_stage[PT][SG + 1];
+    unsigned long long s_kv[2][PT / 64];
+    int s_kp[2][PT / 64];
+    T s_piv[2][PW];  // the pivot rows of the last two columns
+    T s_hand[PT / 64][PW + 2];
+    T s_u11[PW][PW];
+    int s_p[2], s_abort, s_skip[PW], s_dest[PT];
+};
+template <typename T>
+struct WorkLds {
+    T As[2][tile::BK][tile::BM + tile::Pad<T>::v];
+    T Bs[2][tile::BK][tile::BN + tile::Pad<T>::v];
+    int id;
+};
+template <typename T>
+constexpr size_t lds_bytes() { return sizeof(WorkLds<T>) > sizeof(PanelLds<T>) ? sizeof(WorkLds<T>) : sizeof(PanelLds<T>); }
+
+// A worker of the panel kernel (a workgroup of 512 threads outside XCD 0): takes the next tile of the trailing
+// product, all eight waves on it, until the tiles run out or the panel's columns are done, then returns.
+template <typename T>
+__device__ void trail_work(const TrailJob<T>& J, unsigned stop_tag, WorkLds<T>& W) {
+    const int total = J.tiles_m * J.tiles_n;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int id = -1;
+            if (__hip_atomic_load(J.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != stop_tag) {
+                id = atomicAdd(J.next, 1);
+                if (id >= total) id = -1;
+            }
+            W.id = id;
+        }
+        __syncthreads();
+        const int id = W.id;
+        __syncthreads();  // W.id is written again at the next round
+        if (id < 0) return;
+        int tm, tn;
+        tile::grouped(id, J.tiles_m, J.tiles_n, tm, tn);
+        const int64_t m0 = (int64_t)tm * tile::BM, n0 = (int64_t)tn * tile::BN;
+        if (m0 + tile::BM <= J.m && n0 + tile::BN <= J.n)
+            tile::gemm_tile<T, false, false, false, 8>(J.m, J.n, J.k, T(-1), J.A, J.lda, J.B, J.ldb, T(1), J.C, J.ldc, m0,
+                                                       n0, W.As, W.Bs, threadIdx.x, true);
+        else
+            tile::gemm_tile<T, false, false, true, 8>(J.m, J.n, J.k, T(-1), J.A, J.lda, J.B, J.ldb, T(1), J.C, J.ldc, m0, n0,
+                                                      W.As, W.Bs, threadIdx.x, true);
+    }
+}
+
+// Launch index w -> item of [0, count), each XCD a contiguous run of ceil(count / 8); count (past the end) for the
+// launch indices that a larger grid has beyond those runs.
+__device__ __forceinline__ int xcd_run_of(int w, int count) {
+    const int per = (count + 7) / 8;
+    return w / 8 < per ? (w % 8) * per + w / 8 : count;
+}
+
+// The tiles of a trailing product that the panels' workers left: numbers [*next, total), split over the XCDs in
+// contiguous runs. Launched on the same stream after the panels, so *next no longer moves.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void trail_rest(TrailJob<T> J) {
+    __shared__ T As[2][tile::BK][tile::BM + tile::Pad<T>::v];
+    __shared__ T Bs[2][tile::BK][tile::BN + tile::Pad<T>::v];
+    const int total = J.tiles_m * J.tiles_n;
+    const int taken = __hip_atomic_load(J.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int first = taken < total ? taken : total;
+    const int id = first + xcd_run_of(blockIdx.x, total - first);
+    if (id >= total) return;
+    int tm, tn;
+    tile::grouped(id, J.tiles_m, J.tiles_n, tm, tn);
+    const int64_t m0 = (int64_t)tm * tile::BM, n0 = (int64_t)tn * tile::BN;
+    if (m0 + tile::BM <= J.m && n0 + tile::BN <= J.n)
+        tile::gemm_tile<T, false, false, false>(J.m, J.n, J.k, T(-1), J.A, J.lda, J.B, J.ldb, T(1), J.C, J.ldc, m0, n0, As,
+                                                Bs, threadIdx.x, true);
+    else
+        tile::gemm_tile<T, false, false, true>(J.m, J.n, J.k, T(-1), J.A, J.lda, J.B, J.ldb, T(1), J.C, J.ldc, m0, n0, As,
+                                               Bs, threadIdx.x, true);
+}
+
 // Wall-clock ticks (100 MHz) for the optional per-phase profile of the panel kernel.
 __device__ __forceinline__ long long ticks() { return __builtin_amdgcn_s_memrealtime(); }
 
@@ -129,9 +227,10 @@ __device__ __forceinline__ long long ticks() { return __builtin_amdgcn_s_memreal
 // a barrier, the workgroups' best keys compared, the winner's row fetched, a second barrier.
 //
 // With one workgroup, the winner's row goes through LDS. With several (launch indices 0, 8, 16, ..., one XCD
-// under round-robin dispatch, one workgroup per CU, so all are resident; the other launch indices return at
-// once), each workgroup publishes its best key and, from the thread holding it, its row as tagged words
-// (Exchange); wave 0 of every workgroup reads all keys, picks the winner and fetches its row.
+// under round-robin dispatch, one workgroup per CU, so all are resident), each workgroup publishes its best key
+// and, from the thread holding it, its row as tagged words (Exchange); wave 0 of every workgroup reads all keys,
+// picks the winner and fetches its row. The launch indices on the other XCDs return at once, or, where `job`
+// holds a trailing product, work on its tiles (trail_work) until the columns are done.
 //
 // Writes pivots[c0 + c] (absolute rows) and `moves` (count, then pairs dst, src: the rows whose position
 // changed), for the row-swap kernel. `leaf_end[c]` is the end of the reference leaf of column c; a zero
@@ -142,9 +241,25 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                                                    int64_t* __restrict__ pivots, long long* __restrict__ moves,
                                                    const int* __restrict__ leaf_end, T* __restrict__ scratch,
                                                    Exchange* __restrict__ ex, unsigned tag0, int nwg, int* status,
-                                                   long long* prof) {
-    if (nwg > 1 && (blockIdx.x & 7)) return;
-    const int g = nwg > 1 ? (int)(blockIdx.x >> 3) : 0;
+                                                   long long* prof, TrailJob<T> job) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<T>()];
+    if (blockIdx.x & 7) {  // outside XCD 0: a worker of the trailing product, if there is one
+        if (job.C) trail_work<T>(job, tag0, *reinterpret_cast<WorkLds<T>*>(lds));
+        return;
+    }
+    const int g = (int)(blockIdx.x >> 3);
+    if (g >= nwg) return;
+    PanelLds<T>& L = *reinterpret_cast<PanelLds<T>*>(lds);
+    auto& s_stage = L.s_stage;
+    auto& s_kv = L.s_kv;
+    auto& s_kp = L.s_kp;
+    auto& s_piv = L.s_piv;
+    auto& s_hand = L.s_hand;
+    auto& s_u11 = L.s_u11;
+    auto& s_p = L.s_p;
+    auto& s_abort = L.s_abort;
+    auto& s_skip = L.s_skip;
+    auto& s_dest = L.s_dest;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int WAVES = PT / 64;
     const bool timed = prof != nullptr && g == 0 && threadIdx.x == 0;
@@ -161,7 +276,6 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     // The rows come in through LDS, 16 columns at a time, so that the global loads run along rows (a wave reads four
     // rows' 16 items per load) rather than one item of 64 rows per load.
     T v[PW];
-    __shared__ T s_stage[PT][SG + 1];
     const int wg_rows = (int)(rows - (int64_t)g * PT < PT ? rows - (int64_t)g * PT : PT);
 #pragma unroll
     for (int ch = 0; ch < PW / SG; ++ch) {
@@ -184,12 +298,6 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         __syncthreads();
     }
 
-    __shared__ unsigned long long s_kv[2][WAVES];
-    __shared__ int s_kp[2][WAVES];
-    __shared__ T s_piv[2][PW];  // the pivot rows of the last two columns
-    __shared__ T s_hand[WAVES][PW + 2];
-    __shared__ T s_u11[PW][PW];
-    __shared__ int s_p[2], s_abort, s_skip[PW];
     if (tid == 0) s_abort = 0;
     if (tid < PW) s_skip[tid] = tid < w ? leaf_end[c0 + tid] - (int)c0 - tid : 0;  // a zero pivot's skip, by column
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
@@ -352,6 +460,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         HS_PHASE(4);
     }
     __syncthreads();  // every row of U is in s_u11
+    if (job.C && g == 0 && tid == 0) __hip_atomic_store(job.stop, tag0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // The row at its final position: multipliers from scratch left of it, its row of U (pivot rows) from LDS; the
     // scratch items are all loaded before any is used (as the first loads). The rows go out through LDS, 16 columns
@@ -363,7 +472,6 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         const int upos = pos < PW ? pos : PW - 1;
 #pragma unroll
         for (int j = 0; j < PW; ++j) v[j] = j < pos ? v[j] : s_u11[upos][(j - pos) & (PW - 1)];
-        __shared__ int s_dest[PT];
         s_dest[tid] = pos;
 #pragma unroll
         for (int ch = 0; ch < PW / SG; ++ch) {
@@ -391,12 +499,12 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
 #undef HS_PHASE
 }
 
-// Applies a panel's net row permutation to the columns outside it: for columns [0, c0) and [c1, n),
-// every moved row takes the old item of its source row. Each block of 64 columns reads all sources into
-// LDS before writing, so chains of swaps cost one read and one write per moved item.
+// Applies a panel's net row permutation to the columns [lo, hi) outside [x0, x1) (x0 <= x1 within [lo, hi)): every
+// moved row takes the old item of its source row. Each block of 64 columns reads all sources into LDS before writing,
+// so chains of swaps cost one read and one write per moved item.
 template <typename T>
-__global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t lda, int64_t n, int64_t c0, int64_t c1,
-                                                   const long long* __restrict__ moves) {
+__global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t lda, int64_t lo, int64_t hi, int64_t x0,
+                                                   int64_t x1, const long long* __restrict__ moves) {
     __shared__ T tmp[2 * PW][64];
     __shared__ long long mv[1 + 4 * PW];
     for (int q = threadIdx.x; q < 1 + 4 * PW; q += blockDim.x) mv[q] = moves[q];
@@ -404,9 +512,9 @@ __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t ld
     const int nm = (int)mv[0];
     if (nm == 0) return;
     const int tid = threadIdx.x, col_l = tid & 63, grp = tid >> 6;
-    int64_t col = (int64_t)blockIdx.x * 64 + col_l;  // over the columns outside [c0, c1)
-    if (col >= c0) col += c1 - c0;
-    const bool live = col < n;
+    int64_t col = lo + (int64_t)blockIdx.x * 64 + col_l;  // over the columns of [lo, hi) outside [x0, x1)
+    if (col >= x0) col += x1 - x0;
+    const bool live = col < hi;
     const int64_t lc = live ? col : 0;
     // All sources in flight at once: past the last move, a load repeats the first source (addresses stay inside
     // the matrix and no load waits behind a per-lane condition).
@@ -632,7 +740,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
     }
     const size_t np = panels.size();
     const size_t move_words = 1 + 4 * PW;
-    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 1;
+    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 3;  // + status, tile counter, stop mark
     int* d_leaf = (int*)pool_get(n * sizeof(int));
     T* scratch = (T*)pool_get(sizeof(T) * (size_t)n * PW);
     char* xbuf = (char*)pool_get(clear_words * 8);
@@ -643,6 +751,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
     Exchange* ex = (Exchange*)xbuf;
     long long* moves = (long long*)(xbuf + sizeof(Exchange));
     int* status = (int*)(moves + np * move_words);
+    int* trail_next = status + 2;
+    unsigned* trail_stop = (unsigned*)(status + 4);
     HS_HIP(hipMemcpyAsync(d_leaf, leaf_end.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
     fill_words<<<(unsigned)((clear_words + 255) / 256), 256, 0, s>>>((unsigned long long*)xbuf, (int64_t)clear_words, 0ull);
     HS_HIP(hipGetLastError());
@@ -654,30 +764,57 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
         HS_HIP(hipGetLastError());
         return OK;
     };
-    for (auto [p0, p1] : blocks) {
+    // The row moves of a panel on the columns [lo, hi) outside [x0, x1).
+    auto swap = [&](const long long* mv, int64_t lo, int64_t hi, int64_t x0, int64_t x1) {
+        const int64_t cols = (hi - lo) - (x1 - x0);
+        if (cols <= 0) return OK;
+        swap_kernel<T><<<(unsigned)((cols + 63) / 64), 256, 0, s>>>(a, lda, lo, hi, x0, x1, mv);
+        HS_HIP(hipGetLastError());
+        return OK;
+    };
+    // Look-ahead. After block b, the trailing product runs in two parts: the next block's columns at once (on the whole
+    // chip), and the columns right of the next block (`job`) by the workers of the next block's panel launches, which
+    // fill XCDs 1-7 while the panel's workgroups run on XCD 0; trail_rest takes the tiles left when the block's panels
+    // are done. Meanwhile a panel's row moves reach only its own block's columns; the other columns take them after the
+    // product (row moves and the product touch every item in the same order either way). Every tile keeps its terms in
+    // order of p, so every item takes the same operations as with one product.
+    TrailJob<T> job{};
+    const TrailJob<T> none{};
+    for (size_t b = 0; b < blocks.size() && rc == OK; ++b) {
+        const auto [p0, p1] = blocks[b];
         const int64_t B0 = panels[p0].first, B1 = panels[p1 - 1].second;
+        const bool ahead = job.C != nullptr;
+        if (ahead) {
+            fill_words<<<1, 64, 0, s>>>((unsigned long long*)job.next, 1, 0ull);
+            HS_HIP(hipGetLastError());
+        }
         for (size_t i = p0; i < p1 && rc == OK; ++i) {
             const auto [c0, c1] = panels[i];
             const int w = (int)(c1 - c0);
             const int64_t rows = n - c0;
             const int nwg = (int)((rows + PT - 1) / PT);
-            const unsigned grid = nwg > 1 ? (unsigned)(8 * nwg) : 1u;
+            const unsigned grid = ahead ? 8u * MAX_WG : (nwg > 1 ? (unsigned)(8 * nwg) : 1u);
             long long* mv = moves + i * move_words;
             panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, mv, d_leaf, scratch, ex,
-                                                 (unsigned)(i * PW + 1), nwg, status, st->on ? st->prof : nullptr);
+                                                 (unsigned)(i * PW + 1), nwg, status, st->on ? st->prof : nullptr,
+                                                 ahead ? job : none);
             HS_HIP(hipGetLastError());
-            const int64_t outside = n - w;
-            if (outside > 0) {
-                swap_kernel<T><<<(unsigned)((outside + 63) / 64), 256, 0, s>>>(a, lda, n, c0, c1, mv);
-                HS_HIP(hipGetLastError());
-            }
+            if ((rc = ahead ? swap(mv, B0, B1, c0, c1) : swap(mv, 0, n, c0, c1))) break;
             if (c1 < B1) {
                 if ((rc = trsm(c0, w, c1, B1))) break;
                 rc = gemm<T>(false, false, n - c1, B1 - c1, w, T(-1), a + c1 * lda + c0, lda, a + c0 * lda + c1, lda, T(1),
                              a + c1 * lda + c1, lda, s);
             }
         }
-        if (rc || B1 >= n) continue;
+        if (rc) break;
+        if (ahead) {
+            trail_rest<T><<<(unsigned)(((job.tiles_m * job.tiles_n + 7) / 8) * 8), 256, 0, s>>>(job);
+            HS_HIP(hipGetLastError());
+            job = none;
+            for (size_t i = p0; i < p1 && rc == OK; ++i) rc = swap(moves + i * move_words, 0, n, B0, B1);
+            if (rc) break;
+        }
+        if (B1 >= n) continue;
         // Rows of U right of the block, panel by panel, then the trailing product.
         for (size_t i = p0; i < p1 && rc == OK; ++i) {
             const auto [c0, c1] = panels[i];
@@ -686,17 +823,36 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
                              T(1), a + c0 * lda + B1, lda, s);
             if (rc == OK) rc = trsm(c0, (int)(c1 - c0), B1, n);
         }
-        if (rc == OK) {
-            if constexpr (std::is_same<T, float>::value) {
-                if (bf16_trailing) {
-                    rc = gemm_bf16_update(n - B1, n - B1, B1 - B0, a + B1 * lda + B0, lda, a + B0 * lda + B1, lda,
-                                          a + B1 * lda + B1, lda, s);
-                    continue;
-                }
+        if (rc) break;
+        if constexpr (std::is_same<T, float>::value) {
+            if (bf16_trailing) {
+                rc = gemm_bf16_update(n - B1, n - B1, B1 - B0, a + B1 * lda + B0, lda, a + B0 * lda + B1, lda,
+                                      a + B1 * lda + B1, lda, s);
+                continue;
             }
+        }
+        const int64_t B2 = panels[blocks[b + 1].second - 1].second;
+        if (B2 >= n || n - B1 < 2) {
             rc = gemm<T>(false, false, n - B1, n - B1, B1 - B0, T(-1), a + B1 * lda + B0, lda, a + B0 * lda + B1, lda, T(1),
                          a + B1 * lda + B1, lda, s);
+            continue;
         }
+        // The next block's columns now; the rest beside the next block's panels.
+        rc = gemm<T>(false, false, n - B1, B2 - B1, B1 - B0, T(-1), a + B1 * lda + B0, lda, a + B0 * lda + B1, lda, T(1),
+                     a + B1 * lda + B1, lda, s, true);
+        job.m = n - B1;
+        job.n = n - B2;
+        job.k = B1 - B0;
+        job.A = a + B1 * lda + B0;
+        job.lda = lda;
+        job.B = a + B0 * lda + B2;
+        job.ldb = lda;
+        job.C = a + B1 * lda + B2;
+        job.ldc = lda;
+        job.tiles_m = (int)((job.m + tile::BM - 1) / tile::BM);
+        job.tiles_n = (int)((job.n + tile::BN - 1) / tile::BN);
+        job.next = trail_next;
+        job.stop = trail_stop;
     }
     if (rc == OK) {
         dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);

@@ -116,6 +116,19 @@ def test_lu_matches_the_oracle_bit_for_bit(la, dtype, n):
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [2100, 4100])
+def test_lu_with_look_ahead_matches_the_oracle_bit_for_bit(la, dtype, n):
+    """Sizes with many 256-column blocks: the FP32 factorization runs each block's trailing product beside the next
+    block's panels (tiles taken by the panel launches' workers, the rest afterwards) and moves rows of the other columns
+    late; pivots and factors still equal the oracle's item for item."""
+    a = random(n, n, n + 3, dtype)
+    f = la.lu(a)
+    want, perm, _ = oracle.lu(a)
+    assert (f.p == perm).all()
+    assert bits_equal(f.factors, want), f'max diff {np.abs(f.factors - want).max()}'
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
 def test_lu_edge_matrices_match_the_oracle(la, dtype):
     "Zero columns (a zero pivot skips only the rest of its reference leaf), NaN, infinities and a permutation matrix."
     rng = np.random.default_rng(11)

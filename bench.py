@@ -14,11 +14,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'FP64-accurate solve GFLOP/s: low-prec LU+IR vs plain FP64 LU; batched solves/s'
 PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'hbm': 8.0e12}
-# Measured HBM bytes per launch of the FP32 trailing products in one mixed solve of order n: two rocprofv3 --pmc passes
-# (FETCH_SIZE, then WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --only-mixed`, each scaled by what the
-# same counter reads on a 1 GiB device copy (`bench.py --copy-probe 1024`: FETCH_SIZE counts half the bytes read,
-# WRITE_SIZE all the bytes written); the summary is profiles/gemm_f32_lu_update_n16384_traffic.md.
-TRAFFIC = {('gemm_f32', 16384): 139606084}
+# Measured HBM bytes per launch of the FP32 trailing-tile kernel (trail_rest<float>) in one mixed solve of order n: two
+# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --only-mixed`, each
+# scaled by what the same counter reads on a 1 GiB device copy (`bench.py --copy-probe 1024`: FETCH_SIZE counts half
+# the bytes read, WRITE_SIZE all the bytes written); the summary is profiles/trail_rest_f32_lu_n16384_traffic.md.
+TRAFFIC = {('trail_f32', 16384): 294728562}
 
 
 def householder_matrix(n, seed, cond=1e6, reflectors=8):
@@ -311,26 +311,37 @@ def main():
     t = sum(times) / len(times)
     fwd = ((x - x64).norm() / x64.norm()).item()
 
-    # The roofline of the MFMA kernel that carries the factorization's flops: the FP32 trailing product, timed live
-    # with HIP events on its stream by the library over one more solve (hs_profile_gemm), with its algorithmic flops and
-    # bytes per launch; `traffic` is the measured HBM bytes per launch (TRAFFIC). The panel kernel, which takes a
-    # comparable share of the step, is latency-bound (one pivot exchange per column): its per-column time comes from
-    # its own phase counters (hs_profile_panel, workgroup 0) over the same solve.
-    gms, gl, gf, gb = ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)
+    # The roofline of the MFMA kernel that carries the factorization's flops. Each 256-column block's trailing product
+    # runs in three parts: the next block's columns as one product launch (gemm_mfma), the tiles that the next block's
+    # panel launches take on their idle workgroups, and the tiles those leave, in trail_rest. trail_rest is the pure
+    # MFMA kernel with the most of them: timed live with HIP events on its stream by the library over one more solve
+    # (hs_profile_trail), with the flops and algorithmic bytes of the tiles each launch computed; `traffic` is its
+    # measured HBM bytes per launch (TRAFFIC). The product launches (hs_profile_gemm) are reported beside it, and the
+    # panel kernel, latency-bound (one pivot exchange per column), by its own phase counters (hs_profile_panel,
+    # workgroup 0) over the same solve.
+    def prof(fn):
+        v = [ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)]
+        fn(0, *[ctypes.byref(x) for x in v])
+        return v[0].value, v[1].value, v[2].value, v[3].value
     ticks = (ctypes.c_int64 * 8)()
+    lib.hs_profile_trail(1, None, None, None, None)
     lib.hs_profile_gemm(1, None, None, None, None)
     lib.hs_profile_panel(1, None)
     dv.dsgesv(a, b, tol=tol); torch.cuda.synchronize()
-    lib.hs_profile_gemm(0, ctypes.byref(gms), ctypes.byref(gl), ctypes.byref(gf), ctypes.byref(gb))
+    tms, tl, tf, tb = prof(lib.hs_profile_trail)
+    gms, gl, gf, gb = prof(lib.hs_profile_gemm)
     lib.hs_profile_panel(0, ticks)
-    per_launch = gf.value / gl.value
-    avg_s = gms.value / 1e3 / gl.value
-    achieved = per_launch / avg_s / 1e12
+    factor_flops = 2 / 3 * n ** 3
+    achieved = tf / (tms / 1e3) / 1e12
     roof = {'bound': 'mfma', 'achieved': round(achieved, 2), 'peak': PEAK['fp32_mfma'] / 1e12, 'unit': 'TFLOP/s',
-            'frac': round(achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': TRAFFIC.get(('gemm_f32', n)),
-            'kernel': 'gemm_mfma<float> (LU trailing updates)', 'launches': gl.value,
-            'flop_per_launch': round(per_launch), 'algorithmic_bytes_per_launch': round(gb.value / gl.value),
-            'avg_launch_us': round(avg_s * 1e6, 2)}
+            'frac': round(achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': TRAFFIC.get(('trail_f32', n)),
+            'kernel': 'trail_rest<float> (the LU trailing products\' tiles left by the panel launches)', 'launches': tl,
+            'flop_per_launch': round(tf / tl), 'algorithmic_bytes_per_launch': round(tb / tl),
+            'avg_launch_us': round(tms / tl * 1e3, 2),
+            'share_of_factorization_flops': round(tf / factor_flops, 4),
+            'product_launches': {'kernel': 'gemm_mfma<float>', 'launches': gl, 'ms': round(gms, 3),
+                                 'tflops': round(gf / (gms / 1e3) / 1e12, 2),
+                                 'share_of_factorization_flops': round(gf / factor_flops, 4)}}
     panel_s = (sum(ticks[:5]) + ticks[7]) / 1e8
     panel = {'kernel': 'panel_kernel<float>', 'bound': 'latency (one pivot exchange per column)',
              'column_loop_ms': round(panel_s * 1e3, 3), 'us_per_column': round(panel_s / n * 1e6, 3),

@@ -169,6 +169,13 @@ int hs_profile_panel(int on, int64_t* out);
  * stream, on = 0 stops; gives the launches' summed time (ms), their count, their algorithmic flops (2 m n k) and
  * algorithmic bytes (a and b read once, c written once and, where beta is not 0, read once). Any output may be NULL. */
 int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops, double* bytes);
+/* Profiling of the LU's trailing-tile kernel (no counterpart in the reference): the FP32 and FP64 factorizations of
+ * order above 512 run each block's trailing product partly on the panel launches' idle workgroups and the remaining
+ * tiles in trail_rest. on = 1 clears and starts bracketing every trail_rest launch with HIP events on its stream,
+ * on = 0 stops; gives the launches' summed time (ms), their count, the flops of the tiles they computed (2 rows cols k
+ * each) and the algorithmic bytes (the tiles' items of c read and written once, the rows of a and columns of b they
+ * touch read once). Any output may be NULL. */
+int hs_profile_trail(int on, double* ms, int64_t* launches, double* flops, double* bytes);
 /* Profiling of the batched solves: on = 1 clears and starts per-phase timers in workgroup 0 (100 MHz ticks: setup,
  * panel columns, U rows and trailing update, regularity check and L, U), so that runs launch directly rather than
  * replay their graph; on = 0 stops. `out` (8 values, may be NULL) gets the sums, out[7] the systems timed. */

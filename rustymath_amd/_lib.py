@@ -55,6 +55,7 @@ SIGNATURES = {
     'hs_dev_gesv_batched_free': (_i32, [_i64]),
     'hs_profile_panel': (_i32, [_i32, _vp]),
     'hs_profile_gemm': (_i32, [_i32, _vp, _vp, _vp, _vp]),
+    'hs_profile_trail': (_i32, [_i32, _vp, _vp, _vp, _vp]),
     'hs_profile_batched': (_i32, [_i32, _vp]),
 }
 

@@ -636,5 +636,8 @@ int hs_profile_batched(int on, int64_t* out) { return batched_profile(on, (long 
 int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
     return gemm_profile(on, ms, launches, flops, bytes);
 }
+int hs_profile_trail(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
+    return trail_profile(on, ms, launches, flops, bytes);
+}
 
 }  // extern "C"

@@ -15,6 +15,7 @@
 #include "hs_kernels.h"
 #include "hs_pivot.h"
 #include "gemm_tile.h"
+#include <hip/hip_ext.h>
 #include <vector>
 #include <mutex>
 #include <algorithm>
@@ -129,6 +130,7 @@ struct TrailJob {
     int tiles_m, tiles_n;
     int* next;
     unsigned* stop;
+    int* log;  // trail_rest writes the first tile it computes here, where not null (profiling)
 };
 
 // The panel kernel's LDS, and the workers', in one allocation.
@@ -197,6 +199,7 @@ __global__ __launch_bounds__(256, 2) void trail_rest(TrailJob<T> J) {
     const int total = J.tiles_m * J.tiles_n;
     const int taken = __hip_atomic_load(J.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int first = taken < total ? taken : total;
+    if (J.log && blockIdx.x == 0 && threadIdx.x == 0) *J.log = first;
     const int id = first + xcd_run_of(blockIdx.x, total - first);
     if (id >= total) return;
     int tm, tn;
@@ -681,6 +684,32 @@ int panel_prof(PanelProfile*& st) {
     return OK;
 }
 
+// The live profile of trail_rest: while on, each launch is bracketed by HIP events on its stream and writes its first
+// tile to a slot of `first`; the launch's flops and bytes are counted from that tile on when the profile is read.
+struct TrailProfile {
+    std::mutex mu;
+    bool on = false;
+    int* first = nullptr;  // device, SLOTS ints
+    static constexpr int SLOTS = 4096;
+    struct Rec {
+        hipEvent_t a, b;
+        int64_t m, n, k;
+        int tiles_m, tiles_n;
+        size_t item;
+    };
+    std::vector<Rec> recs;
+} g_trail;
+
+// Tile number -> (tm, tn) as tile::grouped numbers tiles, on the host.
+void grouped_host(int id, int tiles_m, int tiles_n, int& tm, int& tn) {
+    constexpr int G = 8;
+    const int group = id / (G * tiles_n), first = group * G;
+    const int gm = tiles_m - first < G ? tiles_m - first : G;
+    const int in = id - group * G * tiles_n;
+    tm = first + in % gm;
+    tn = in / gm;
+}
+
 __global__ void fill_words(unsigned long long* p, int64_t n, unsigned long long v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -697,6 +726,47 @@ int panel_profile(int on, long long* out) {
     if (out) HS_HIP(hipMemcpy(out, st->prof, 8 * sizeof(long long), hipMemcpyDeviceToHost));
     if (on) HS_HIP(hipMemset(st->prof, 0, 8 * sizeof(long long)));
     st->on = on != 0;
+    return OK;
+}
+
+// trail_rest's live profile (see TrailProfile): on = 1 clears and starts, 0 stops; gives the summed event time (ms),
+// the launches, and the flops (2 rows cols k per tile computed) and algorithmic bytes (each computed tile's items of c
+// read and written, and the rows of a and columns of b that the launch's tiles touch, read once) since the last call.
+int trail_profile(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
+    std::lock_guard<std::mutex> lk(g_trail.mu);
+    double total = 0, fl = 0, by = 0;
+    std::vector<int> first(g_trail.recs.size());
+    if (!g_trail.recs.empty())
+        HS_HIP(hipMemcpy(first.data(), g_trail.first, first.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < g_trail.recs.size(); ++r) {
+        auto& R = g_trail.recs[r];
+        float e = 0;
+        HS_HIP(hipEventSynchronize(R.b));
+        HS_HIP(hipEventElapsedTime(&e, R.a, R.b));
+        hipEventDestroy(R.a);
+        hipEventDestroy(R.b);
+        total += e;
+        std::vector<char> row_seen(R.tiles_m, 0), col_seen(R.tiles_n, 0);
+        double c_items = 0, a_rows = 0, b_cols = 0;
+        for (int id = first[r]; id < R.tiles_m * R.tiles_n; ++id) {
+            int tm, tn;
+            grouped_host(id, R.tiles_m, R.tiles_n, tm, tn);
+            const double rows = (double)std::min<int64_t>(tile::BM, R.m - (int64_t)tm * tile::BM);
+            const double cols = (double)std::min<int64_t>(tile::BN, R.n - (int64_t)tn * tile::BN);
+            fl += 2.0 * rows * cols * (double)R.k;
+            c_items += rows * cols;
+            if (!row_seen[tm]) { row_seen[tm] = 1; a_rows += rows; }
+            if (!col_seen[tn]) { col_seen[tn] = 1; b_cols += cols; }
+        }
+        by += (double)R.item * (2.0 * c_items + (a_rows + b_cols) * (double)R.k);
+    }
+    g_trail.recs.clear();
+    if (ms) *ms = total;
+    if (launches) *launches = (int64_t)first.size();
+    if (flops) *flops = fl;
+    if (bytes) *bytes = by;
+    g_trail.on = on != 0;
+    if (g_trail.on && !g_trail.first) HS_HIP(hipMalloc(&g_trail.first, TrailProfile::SLOTS * sizeof(int)));
     return OK;
 }
 
@@ -808,7 +878,19 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
         }
         if (rc) break;
         if (ahead) {
-            trail_rest<T><<<(unsigned)(((job.tiles_m * job.tiles_n + 7) / 8) * 8), 256, 0, s>>>(job);
+            const unsigned grid = (unsigned)(((job.tiles_m * job.tiles_n + 7) / 8) * 8);
+            std::unique_lock<std::mutex> lk(g_trail.mu);
+            if (g_trail.on && g_trail.recs.size() < (size_t)TrailProfile::SLOTS) {
+                TrailProfile::Rec R{nullptr, nullptr, job.m, job.n, job.k, job.tiles_m, job.tiles_n, sizeof(T)};
+                HS_HIP(hipEventCreate(&R.a));
+                HS_HIP(hipEventCreate(&R.b));
+                job.log = g_trail.first + g_trail.recs.size();
+                hipExtLaunchKernelGGL(trail_rest<T>, dim3(grid), dim3(256), 0, s, R.a, R.b, 0, job);
+                g_trail.recs.push_back(R);
+            } else {
+                trail_rest<T><<<grid, 256, 0, s>>>(job);
+            }
+            lk.unlock();
             HS_HIP(hipGetLastError());
             job = none;
             for (size_t i = p0; i < p1 && rc == OK; ++i) rc = swap(moves + i * move_words, 0, n, B0, B1);

@@ -22,32 +22,35 @@ __device__ __forceinline__ unsigned long long mag_key(T x) {
 __device__ __forceinline__ bool better(unsigned long long kv, int kp, unsigned long long kv2, int kp2) {
     return (kv2 > kv) | ((kv2 == kv) & (kp2 < kp));
 }
-// The wave's best key: the largest kv through DPP row shifts and broadcasts (the maximum lands in lane 63), then
-// among the lanes holding it the smallest position (one lane, save for ties). Returns wave-uniform values; `lane_of`
-// gets the winning lane (-1 where no lane is a candidate: kp = INT_MAX everywhere).
+// The wave's maximum of a 32-bit value through DPP row shifts and broadcasts (it lands in lane 63): one max per step,
+// with the row shift or broadcast as the DPP operand (0 is the identity of the maximum, for lanes with no source).
+// An FP32 item's key (bits of |x| plus one) fits in 32 bits; a 64-bit key takes two such passes.
+__device__ __forceinline__ unsigned dpp_max_u32(unsigned x) {
+#define HS_DPP_MAX(ctrl, rm, bm) \
+    x = max(x, (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, ctrl, rm, bm, false))
+    HS_DPP_MAX(0x111, 0xf, 0xf);  // row_shr:1
+    HS_DPP_MAX(0x112, 0xf, 0xf);  // row_shr:2
+    HS_DPP_MAX(0x114, 0xf, 0xf);  // row_shr:4
+    HS_DPP_MAX(0x118, 0xf, 0xf);  // row_shr:8
+    HS_DPP_MAX(0x142, 0xa, 0xf);  // row_bcast:15
+    HS_DPP_MAX(0x143, 0xc, 0xf);  // row_bcast:31
+#undef HS_DPP_MAX
+    return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
 __device__ __forceinline__ unsigned long long dpp_max_u64(unsigned long long x) {
-    unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
-#define HS_DPP_STEP(ctrl, rm, bm)                                                                      \
-    do {                                                                                               \
-        const unsigned lo2 = (unsigned)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, rm, bm, false);   \
-        const unsigned hi2 = (unsigned)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, rm, bm, false);   \
-        const bool gt = (hi2 > hi) | ((hi2 == hi) & (lo2 > lo));                                      \
-        lo = gt ? lo2 : lo;                                                                            \
-        hi = gt ? hi2 : hi;                                                                            \
-    } while (0)
-    HS_DPP_STEP(0x111, 0xf, 0xf);  // row_shr:1
-    HS_DPP_STEP(0x112, 0xf, 0xf);  // row_shr:2
-    HS_DPP_STEP(0x114, 0xf, 0xf);  // row_shr:4
-    HS_DPP_STEP(0x118, 0xf, 0xf);  // row_shr:8: lane 15 of each row holds the row's maximum
-    HS_DPP_STEP(0x142, 0xa, 0xf);  // row_bcast:15 into rows 1 and 3
-    HS_DPP_STEP(0x143, 0xc, 0xf);  // row_bcast:31 into rows 2 and 3: lane 63 holds the wave's maximum
-#undef HS_DPP_STEP
-    const unsigned blo = (unsigned)__builtin_amdgcn_readlane((int)lo, 63);
-    const unsigned bhi = (unsigned)__builtin_amdgcn_readlane((int)hi, 63);
+    // The larger high word first, then the largest low word among the lanes that hold it.
+    const unsigned hi = (unsigned)(x >> 32), bhi = dpp_max_u32(hi);
+    const unsigned blo = dpp_max_u32(hi == bhi ? (unsigned)x : 0u);
     return ((unsigned long long)bhi << 32) | blo;
 }
-__device__ __forceinline__ void wave_best(unsigned long long& kv, int& kp, int& lane_of) {
-    const unsigned long long best = dpp_max_u64(kv);
+// The wave's best key: the largest kv (32 or 64 bits), then among the lanes holding it the smallest position (one
+// lane, save for ties). Returns wave-uniform values; `lane_of` gets the winning lane (-1 where no lane is a
+// candidate: kp = INT_MAX everywhere).
+template <typename K>
+__device__ __forceinline__ void wave_best(K& kv, int& kp, int& lane_of) {
+    K best;
+    if constexpr (sizeof(K) == 4) best = dpp_max_u32(kv);
+    else best = dpp_max_u64(kv);
     unsigned long long m = __ballot(kv == best && kp != 0x7fffffff);
     if (m == 0ull) { kv = 0ull; kp = 0x7fffffff; lane_of = -1; return; }
     int L = __builtin_ctzll(m);

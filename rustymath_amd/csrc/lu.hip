@@ -140,6 +140,7 @@ struct PanelLds {
     unsigned long long s_kv[2][PT / 64];
     int s_kp[2][PT / 64];
     T s_piv[2][PW];  // the pivot rows of the last two columns
+    T s_rcp[2];      // their pivots' reciprocals
     T s_hand[PT / 64][PW + 2];
     T s_u11[PW][PW];
     int s_p[2], s_abort, s_skip[PW], s_dest[PT];
@@ -257,6 +258,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     auto& s_kv = L.s_kv;
     auto& s_kp = L.s_kp;
     auto& s_piv = L.s_piv;
+    auto& s_rcp = L.s_rcp;
     auto& s_hand = L.s_hand;
     auto& s_u11 = L.s_u11;
     auto& s_p = L.s_p;
@@ -265,6 +267,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     auto& s_dest = L.s_dest;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int WAVES = PT / 64;
+    typedef typename std::conditional<sizeof(T) == 4, unsigned, unsigned long long>::type Key;  // a candidate's key
     const bool timed = prof != nullptr && g == 0 && threadIdx.x == 0;
     long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tl = timed ? ticks() : 0;
 #define HS_PHASE(q) do { if (timed) { long long t_ = ticks(); tp[q] += t_ - tl; tl = t_; } } while (0)
@@ -320,22 +323,20 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         const unsigned tag = tag0 + (unsigned)c;
         // This row's candidacy for column c and the wave's best.
         const bool cand = has && pos >= c;
-        unsigned long long kv = cand ? mag_key(x0) : 0ull;
+        Key kv = cand ? (Key)mag_key(x0) : Key(0);
         int kp = cand ? pos : 0x7fffffff, hl = 0;
         wave_best(kv, kp, hl);
         if (lane == 0) { s_kv[buf][wave] = kv; s_kp[buf][wave] = kp; }
         __syncthreads();
         HS_PHASE(0);
-        unsigned long long bkv = s_kv[buf][0];
-        int bkp = s_kp[buf][0];
-#pragma unroll
-        for (int q = 1; q < WAVES; ++q) {
-            const unsigned long long v2 = s_kv[buf][q];
-            const int p2 = s_kp[buf][q];
-            const bool take = better(bkv, bkp, v2, p2);
-            bkv = take ? v2 : bkv;
-            bkp = take ? p2 : bkp;
-        }
+        // The best of the waves' keys: lane q takes wave q's, and one more wave reduction picks it (larger key, then
+        // smaller position), in every wave.
+        Key bk = (Key)s_kv[buf][lane & (WAVES - 1)];
+        int bkp = s_kp[buf][lane & (WAVES - 1)], bl = 0;
+        bk = lane < WAVES ? bk : Key(0);
+        bkp = lane < WAVES ? bkp : 0x7fffffff;
+        wave_best(bk, bkp, bl);
+        const unsigned long long bkv = bk;
         if (nwg > 1 && tid == 0) put_key<T>(ex->key[buf][g], tag, bkv, bkp);
 
         // The workgroup's best row goes out before the rest of the update runs: the holding lane writes its items,
@@ -366,6 +367,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     if (lane < rem) put_item<T>(ex->row[buf][g], lane, tag, item);
                 } else {
                     s_piv[buf][lane] = item;
+                    if (lane == 0) s_rcp[buf] = T(1) / item;
                 }
             }
         }
@@ -414,7 +416,8 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     }
                 }
                 bool all = __all(ok);
-                wave_best(gkv, gkp, gl);
+                Key gk = (Key)gkv;
+                wave_best(gk, gkp, gl);
                 T item = T(0);
                 if (all && lane < rem) {
                     uint64_t spins = 0;
@@ -425,6 +428,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                 all = __all(ok);
                 s_piv[buf][lane] = item;
                 if (lane == 0) {
+                    s_rcp[buf] = T(1) / item;
                     s_p[buf] = gkp;
                     if (!all) { s_abort = 1; atomicExch(status, ERR_LIMIT); }
                 }
@@ -452,7 +456,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         if (has && pos > c) {
             const bool zero = pivot == T(0);
             const int skip = zero ? s_skip[c] : 0;
-            const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * (T(1) / pivot) : x0 / pivot);
+            const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * s_rcp[buf] : x0 / pivot);
             x0 = 1 >= skip ? __builtin_elementwise_fma(l, -piv[1], v[1]) : v[1];
             scratch[(int64_t)c * rows + phys] = l;
             l_prev = l;

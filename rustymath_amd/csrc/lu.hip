@@ -27,6 +27,7 @@ namespace {
 constexpr int PW = 64;            // widest panel
 constexpr int PT = 512;           // threads (rows) of a panel workgroup
 constexpr int SG = 16;            // columns a panel workgroup stages through LDS at a time, loading and storing its rows
+constexpr int STOP_AHEAD = 8;     // columns before a panel's end at which its workers stop taking tiles
 constexpr int MAX_WG = 32;        // panel workgroups: one per CU of one XCD, one flag per lane of a wave
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
 
@@ -464,10 +465,13 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
             pending = true;
         }
         prev = piv;
+        // The workers take no new tile once STOP_AHEAD columns remain, so that the tiles in progress end about when
+        // the columns do.
+        if (job.C && g == 0 && tid == 0 && c == (w > STOP_AHEAD ? w - 1 - STOP_AHEAD : 0))
+            __hip_atomic_store(job.stop, tag0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         HS_PHASE(4);
     }
     __syncthreads();  // every row of U is in s_u11
-    if (job.C && g == 0 && tid == 0) __hip_atomic_store(job.stop, tag0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 
     // The row at its final position: multipliers from scratch left of it, its row of U (pivot rows) from LDS; the
     // scratch items are all loaded before any is used (as the first loads). The rows go out through LDS, 16 columns

@@ -19,9 +19,6 @@ __device__ __forceinline__ unsigned long long mag_key(T x) {
         return ax != ax ? 0ull : (unsigned long long)__builtin_bit_cast(unsigned, ax) + 1ull;
     }
 }
-__device__ __forceinline__ bool better(unsigned long long kv, int kp, unsigned long long kv2, int kp2) {
-    return (kv2 > kv) | ((kv2 == kv) & (kp2 < kp));
-}
 // The wave's maximum of a 32-bit value through DPP row shifts and broadcasts (it lands in lane 63): one max per step,
 // with the row shift or broadcast as the DPP operand (0 is the identity of the maximum, for lanes with no source).
 // An FP32 item's key (bits of |x| plus one) fits in 32 bits; a 64-bit key takes two such passes.

@@ -118,9 +118,9 @@ def test_lu_matches_the_oracle_bit_for_bit(la, dtype, n):
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 @pytest.mark.parametrize('n', [2100, 4100])
 def test_lu_with_look_ahead_matches_the_oracle_bit_for_bit(la, dtype, n):
-    """Sizes with many 256-column blocks: the FP32 factorization runs each block's trailing product beside the next
-    block's panels (tiles taken by the panel launches' workers, the rest afterwards) and moves rows of the other columns
-    late; pivots and factors still equal the oracle's item for item."""
+    """Sizes with many 256-column blocks: the factorization runs each block's trailing product beside the next block's
+    panels (tiles taken by the panel launches' workers, the rest afterwards) and moves rows of the other columns late;
+    pivots and factors still equal the oracle's item for item."""
     a = random(n, n, n + 3, dtype)
     f = la.lu(a)
     want, perm, _ = oracle.lu(a)
@@ -391,3 +391,33 @@ def test_float_fused_multiply_adds_round_once(la, k):
     assert (x[1] == f(1 + 2.0 ** -11 + 2.0 ** -23)).all()
     fac = la.lu(a)
     assert bits_equal(fac.solve(b), oracle.lu_solve(*oracle.lu(a)[:2], b))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_profiles_account_for_a_factorization(la, dtype):
+    """The live profiles `bench.py` reads (trailing tiles, product launches, panel phases) count a factorization of
+    order 2100 within its flops, and profiling leaves the factors the oracle's."""
+    import ctypes
+    import torch
+    from rustymath_amd import _lib, device as dv
+    lib = _lib.load()
+    n = 2100
+    a = random(n, n, 7, dtype)
+    v = [ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)]
+    g = [ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)]
+    ticks = (ctypes.c_int64 * 8)()
+    assert lib.hs_profile_trail(1, None, None, None, None) == 0
+    assert lib.hs_profile_gemm(1, None, None, None, None) == 0
+    assert lib.hs_profile_panel(1, None) == 0
+    f, perm = dv.getrf(torch.from_numpy(a.copy()).cuda())
+    torch.cuda.synchronize()
+    assert lib.hs_profile_trail(0, *[ctypes.byref(x) for x in v]) == 0
+    assert lib.hs_profile_gemm(0, *[ctypes.byref(x) for x in g]) == 0
+    assert lib.hs_profile_panel(0, ticks) == 0
+    # one trail_rest per block (at most 256 columns of whole panels) with a product beside the next block's panels
+    assert 0 < v[1].value < n // 128
+    assert v[0].value > 0 and g[0].value > 0 and g[1].value > 0 and sum(ticks) > 0
+    assert 0 <= v[2].value and v[2].value + g[2].value <= 2 / 3 * n ** 3
+    assert v[3].value > 0 or v[2].value == 0
+    want, wperm, _ = oracle.lu(a)
+    assert bits_equal(f.cpu().numpy(), want) and (perm.cpu().numpy() == wperm).all()

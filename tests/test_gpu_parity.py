@@ -183,7 +183,7 @@ def test_lu_solve_matches_the_oracle_bit_for_bit(la, dtype, n, k):
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 @pytest.mark.parametrize('n', [530, 1030])
 def test_blocked_sizes_have_small_residuals(la, dtype, n):
-    "tests/linalg.rs:569-590 on the GPU, plus the north-star bound 10·n·eps on the solve's backward error."
+    "tests/linalg.rs:569-590 on the GPU, plus the acceptance bound 10·n·eps on the solve's backward error."
     a = random(n, n, n, dtype); aw = a.astype(np.float64)
     b = random(n, 3, 5, dtype)
     x = la.solve(a, b).astype(np.float64)

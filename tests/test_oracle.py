@@ -258,7 +258,7 @@ def test_reference_generator_matches_known_bits():
 
 
 def test_backward_error_of_a_plain_solve():
-    "The north-star acceptance bound, 10·n·eps64 on the normwise backward error, at a config-1-like size."
+    "The acceptance bound of the FP64 solve, 10·n·eps64 on the normwise backward error, on a 300×300 N(0,1) system."
     a = np.random.default_rng(1).standard_normal((300, 300)); b = np.random.default_rng(1).standard_normal(300)
     x = oracle.solve(a, b)
     assert backward_error(a, x, b) <= 10 * 300 * EPS[np.float64]

@@ -300,35 +300,35 @@ def main():
     torch.cuda.synchronize()
     lib = _lib.load()
     import ctypes
-    prof = (ctypes.c_int64 * 8)()
+    def prof(fn):
+        v = [ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)]
+        fn(0, *[ctypes.byref(x) for x in v])
+        return v[0].value, v[1].value, v[2].value, v[3].value
     times, iters, berr, x = [], 0, 0.0, None
+    lib.hs_profile_trail(1, None, None, None, None)  # dispatch-bound events on the roofline kernel's launches
     for _ in range(args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         x, iters, berr = dv.dsgesv(a, b, tol=tol)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
+    tms, tl, tf, tb = prof(lib.hs_profile_trail)
     t = sum(times) / len(times)
     fwd = ((x - x64).norm() / x64.norm()).item()
 
     # The roofline of the MFMA kernel that carries the factorization's flops. Each 256-column block's trailing product
     # runs in three parts: the next block's columns as one product launch (gemm_mfma), the tiles that the next block's
     # panel launches take on their idle workgroups, and the tiles those leave, in trail_rest. trail_rest is the pure
-    # MFMA kernel with the most of them: timed live with HIP events on its stream by the library over one more solve
-    # (hs_profile_trail), with the flops and algorithmic bytes of the tiles each launch computed; `traffic` is its
-    # measured HBM bytes per launch (TRAFFIC). The product launches (hs_profile_gemm) are reported beside it, and the
-    # panel kernel, latency-bound (one pivot exchange per column), by its own phase counters (hs_profile_panel,
-    # workgroup 0) over the same solve.
-    def prof(fn):
-        v = [ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)]
-        fn(0, *[ctypes.byref(x) for x in v])
-        return v[0].value, v[1].value, v[2].value, v[3].value
+    # MFMA kernel with the most of them: timed live over the timed steps with HIP events that the library records at
+    # each launch's dispatch and end on its stream (hs_profile_trail), with the flops and algorithmic bytes of the tiles
+    # each launch computed, per launch averaged over the steps; `traffic` is its measured HBM bytes per launch
+    # (TRAFFIC). The product launches (hs_profile_gemm) are reported beside it, and the panel kernel, latency-bound
+    # (one pivot exchange per column), by its own phase counters (hs_profile_panel, workgroup 0), both over one more
+    # solve (the panel's counters slow it).
     ticks = (ctypes.c_int64 * 8)()
-    lib.hs_profile_trail(1, None, None, None, None)
     lib.hs_profile_gemm(1, None, None, None, None)
     lib.hs_profile_panel(1, None)
     dv.dsgesv(a, b, tol=tol); torch.cuda.synchronize()
-    tms, tl, tf, tb = prof(lib.hs_profile_trail)
     gms, gl, gf, gb = prof(lib.hs_profile_gemm)
     lib.hs_profile_panel(0, ticks)
     factor_flops = 2 / 3 * n ** 3
@@ -338,7 +338,7 @@ def main():
             'kernel': 'trail_rest<float> (the LU trailing products\' tiles left by the panel launches)', 'launches': tl,
             'flop_per_launch': round(tf / tl), 'algorithmic_bytes_per_launch': round(tb / tl),
             'avg_launch_us': round(tms / tl * 1e3, 2),
-            'share_of_factorization_flops': round(tf / factor_flops, 4),
+            'share_of_factorization_flops': round(tf / args.steps / factor_flops, 4),
             'product_launches': {'kernel': 'gemm_mfma<float>', 'launches': gl, 'ms': round(gms, 3),
                                  'tflops': round(gf / (gms / 1e3) / 1e12, 2),
                                  'share_of_factorization_flops': round(gf / factor_flops, 4)}}

@@ -510,42 +510,47 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
 #undef HS_PHASE
 }
 
-// Applies a panel's net row permutation to the columns [lo, hi) outside [x0, x1) (x0 <= x1 within [lo, hi)): every
-// moved row takes the old item of its source row. Each block of 64 columns reads all sources into LDS before writing,
-// so chains of swaps cost one read and one write per moved item.
+// Applies the net row permutations of `count` consecutive panels (move lists `stride` words apart), in order, to the
+// columns [lo, hi) outside [x0, x1) (x0 <= x1 within [lo, hi)): every moved row takes the old item of its source row.
+// Each block of 64 columns reads all of a list's sources into LDS before writing, so chains of swaps cost one read and
+// one write per moved item; the block's barrier orders one list's writes before the next list's reads.
 template <typename T>
 __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t lda, int64_t lo, int64_t hi, int64_t x0,
-                                                   int64_t x1, const long long* __restrict__ moves) {
+                                                   int64_t x1, const long long* __restrict__ moves, int64_t stride,
+                                                   int count) {
     __shared__ T tmp[2 * PW][64];
     __shared__ long long mv[1 + 4 * PW];
-    for (int q = threadIdx.x; q < 1 + 4 * PW; q += blockDim.x) mv[q] = moves[q];
-    __syncthreads();
-    const int nm = (int)mv[0];
-    if (nm == 0) return;
     const int tid = threadIdx.x, col_l = tid & 63, grp = tid >> 6;
     int64_t col = lo + (int64_t)blockIdx.x * 64 + col_l;  // over the columns of [lo, hi) outside [x0, x1)
     if (col >= x0) col += x1 - x0;
     const bool live = col < hi;
     const int64_t lc = live ? col : 0;
-    // All sources in flight at once: past the last move, a load repeats the first source (addresses stay inside
-    // the matrix and no load waits behind a per-lane condition).
-    T vals[2 * PW / 4];
+    for (int list = 0; list < count; ++list) {
+        __syncthreads();  // the previous list's reads of mv and tmp, and its writes, are done
+        for (int q = threadIdx.x; q < 1 + 4 * PW; q += blockDim.x) mv[q] = moves[list * stride + q];
+        __syncthreads();
+        const int nm = (int)mv[0];
+        if (nm == 0) continue;
+        // All sources in flight at once: past the last move, a load repeats the first source (addresses stay inside
+        // the matrix and no load waits behind a per-lane condition).
+        T vals[2 * PW / 4];
 #pragma unroll
-    for (int q = 0; q < 2 * PW / 4; ++q) {
-        const int k = grp + 4 * q;
-        vals[q] = a[mv[2 + 2 * (k < nm ? k : 0)] * lda + lc];
-    }
-    asm volatile("" ::: "memory");
+        for (int q = 0; q < 2 * PW / 4; ++q) {
+            const int k = grp + 4 * q;
+            vals[q] = a[mv[2 + 2 * (k < nm ? k : 0)] * lda + lc];
+        }
+        asm volatile("" ::: "memory");
 #pragma unroll
-    for (int q = 0; q < 2 * PW / 4; ++q) {
-        const int k = grp + 4 * q;
-        if (k < nm) tmp[k][col_l] = vals[q];
-    }
-    __syncthreads();
+        for (int q = 0; q < 2 * PW / 4; ++q) {
+            const int k = grp + 4 * q;
+            if (k < nm) tmp[k][col_l] = vals[q];
+        }
+        __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 2 * PW / 4; ++q) {
-        const int k = grp + 4 * q;
-        if (k < nm && live) a[mv[1 + 2 * k] * lda + col] = tmp[k][col_l];
+        for (int q = 0; q < 2 * PW / 4; ++q) {
+            const int k = grp + 4 * q;
+            if (k < nm && live) a[mv[1 + 2 * k] * lda + col] = tmp[k][col_l];
+        }
     }
 }
 
@@ -842,11 +847,12 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
         HS_HIP(hipGetLastError());
         return OK;
     };
-    // The row moves of a panel on the columns [lo, hi) outside [x0, x1).
-    auto swap = [&](const long long* mv, int64_t lo, int64_t hi, int64_t x0, int64_t x1) {
+    // The row moves of `count` consecutive panels, in order, on the columns [lo, hi) outside [x0, x1).
+    auto swap = [&](const long long* mv, int64_t lo, int64_t hi, int64_t x0, int64_t x1, int count = 1) {
         const int64_t cols = (hi - lo) - (x1 - x0);
         if (cols <= 0) return OK;
-        swap_kernel<T><<<(unsigned)((cols + 63) / 64), 256, 0, s>>>(a, lda, lo, hi, x0, x1, mv);
+        swap_kernel<T><<<(unsigned)((cols + 63) / 64), 256, 0, s>>>(a, lda, lo, hi, x0, x1, mv, (int64_t)move_words,
+                                                                     count);
         HS_HIP(hipGetLastError());
         return OK;
     };
@@ -901,8 +907,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
             lk.unlock();
             HS_HIP(hipGetLastError());
             job = none;
-            for (size_t i = p0; i < p1 && rc == OK; ++i) rc = swap(moves + i * move_words, 0, n, B0, B1);
-            if (rc) break;
+            if ((rc = swap(moves + p0 * move_words, 0, n, B0, B1, (int)(p1 - p0)))) break;
         }
         if (B1 >= n) continue;
         // Rows of U right of the block, panel by panel, then the trailing product.

@@ -74,7 +74,9 @@ struct Mat {
 };
 
 // The trailing update of one panel on MFMA: A22 (rows and columns [c1, n)) += (−L21) U12, with L21 the panel's
-// multipliers (columns [c0, c0 + w)) and U12 its rows of U. Each wave owns 32×32 blocks (2×2 MFMA tiles).
+// multipliers (columns [c0, c0 + w)) and U12 its rows of U, in 32×32 blocks (2×2 MFMA tiles per wave). With at least
+// as many 32-row blocks as waves, each wave takes whole rows of blocks and keeps its rows' L21 in registers across
+// them (the update reads memory for c and U12 only); otherwise the blocks are dealt out one by one.
 template <typename T, int BW>
 __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, int w, int c1, const short* perm,
                                                 int wave, int lane) {
@@ -83,8 +85,26 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
     const int lr = lane & 15, lk = lane >> 4;
     // The MFMA takes l · (−u): terms outside the matrix or past the panel load as +0 on the L side and −0 on the
     // U side, so padding adds (+0)·(−0) = −0, and (−u)·l is the reference's term (its α = −1 folds into U12).
-    for (int blk = wave; blk < nb * nb; blk += WAVES) {
-        const int rb = c1 + 32 * (blk / nb), cb = c1 + 32 * (blk % nb);
+    // All BW / 4 steps run, also past a narrower panel's end (padding then).
+    T av[BW / 4][2];
+    auto load_l = [&](int rb) {  // the rows' L21, every load issued before any is used
+        int arow[2];
+        bool aok[2];
+#pragma unroll
+        for (int ti = 0; ti < 2; ++ti) {
+            const int r = rb + 16 * ti + lr;
+            aok[ti] = r < n;
+            arow[ti] = perm[r < n ? r : last] * n + c0;
+        }
+#pragma unroll
+        for (int s = 0; s < BW / 4; ++s) {
+            const int kk = 4 * s + lk;
+            const bool kok = kk < w;
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti) av[s][ti] = A.ld(A.at(kok && aok[ti], arow[ti] + kk));
+        }
+    };
+    auto block = [&](int rb, int cb) {  // c's block and U12's columns in flight together, then the MFMAs in order
         V acc[2][2];
         unsigned cidx[2][2][4];
 #pragma unroll
@@ -105,24 +125,12 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) acc[ti][tj][e] = A.ld(cidx[ti][tj][e]);
-        int arow[2];
-        bool aok[2];
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti) {
-            const int r = rb + 16 * ti + lr;
-            aok[ti] = r < n;
-            arow[ti] = perm[r < n ? r : last] * n + c0;
-        }
-        // All BW / 4 steps, also past a narrower panel's end (padding then); every operand is loaded before the
-        // first MFMA, so that the block's loads are in flight together.
-        T av[BW / 4][2], bv[BW / 4][2];
+        T bv[BW / 4][2];
 #pragma unroll
         for (int s = 0; s < BW / 4; ++s) {
             const int kk = 4 * s + lk;
             const bool kok = kk < w;
             const int brow = perm[c0 + (kok ? kk : 0)] * n;
-#pragma unroll
-            for (int ti = 0; ti < 2; ++ti) av[s][ti] = A.ld(A.at(kok && aok[ti], arow[ti] + kk));
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj) {
                 const int col = cb + 16 * tj + lr;
@@ -141,6 +149,17 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) A.st(cidx[ti][tj][e], acc[ti][tj][e]);
+    };
+    if (nb >= WAVES) {
+        for (int rbk = wave; rbk < nb; rbk += WAVES) {
+            load_l(c1 + 32 * rbk);
+            for (int cbk = 0; cbk < nb; ++cbk) block(c1 + 32 * rbk, c1 + 32 * cbk);
+        }
+    } else {
+        for (int blk = wave; blk < nb * nb; blk += WAVES) {
+            load_l(c1 + 32 * (blk / nb));
+            block(c1 + 32 * (blk / nb), c1 + 32 * (blk % nb));
+        }
     }
 }
 

@@ -3,7 +3,7 @@
 The bar: the same bits as the oracle's restatement of rustymath wherever the arithmetic is defined item by item
 (exact products, pivots, factors, solves, including signed zeros), and the reference's own invariants (residual
 bounds, power-of-two scaling, zero columns) beside them. Sizes stay where the oracle finishes in well under a
-second.
+second, save the benchmark's own orders (FP64 8192, FP32 16384: 10 to 30 s of the oracle).
 """
 import math
 import numpy as np
@@ -122,6 +122,18 @@ def test_lu_with_look_ahead_matches_the_oracle_bit_for_bit(la, dtype, n):
     panels (tiles taken by the panel launches' workers, the rest afterwards) and moves rows of the other columns late;
     pivots and factors still equal the oracle's item for item."""
     a = random(n, n, n + 3, dtype)
+    f = la.lu(a)
+    want, perm, _ = oracle.lu(a)
+    assert (f.p == perm).all()
+    assert bits_equal(f.factors, want), f'max diff {np.abs(f.factors - want).max()}'
+
+
+@pytest.mark.parametrize('dtype,n', [(np.float64, 8192), (np.float32, 16384)])
+def test_lu_at_the_benchmark_orders_matches_the_oracle_bit_for_bit(la, dtype, n):
+    """The factorizations the benchmark times: FP64 of order 8192 (the plain solve) and FP32 of order 16384 (the
+    mixed solve's), with every block's look-ahead and the panel kernel at its full 32 workgroups; pivots and factors
+    equal the oracle's item for item (the oracle takes about 10 to 30 s here)."""
+    a = random(n, n, 11, dtype)
     f = la.lu(a)
     want, perm, _ = oracle.lu(a)
     assert (f.p == perm).all()

@@ -132,12 +132,15 @@ def test_lu_with_look_ahead_matches_the_oracle_bit_for_bit(la, dtype, n):
 def test_lu_at_the_benchmark_orders_matches_the_oracle_bit_for_bit(la, dtype, n):
     """The factorizations the benchmark times: FP64 of order 8192 (the plain solve) and FP32 of order 16384 (the
     mixed solve's), with every block's look-ahead and the panel kernel at its full 32 workgroups; pivots and factors
-    equal the oracle's item for item (the oracle takes about 10 to 30 s here)."""
+    equal the oracle's item for item (the oracle takes about 10 to 30 s here), and so do the single-column solves with
+    them (the refinement's)."""
     a = random(n, n, 11, dtype)
     f = la.lu(a)
     want, perm, _ = oracle.lu(a)
     assert (f.p == perm).all()
     assert bits_equal(f.factors, want), f'max diff {np.abs(f.factors - want).max()}'
+    b = random(n, 1, 13, dtype)[:, 0]
+    assert bits_equal(f.solve(b), oracle.lu_solve(want, perm, b))
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])

@@ -64,6 +64,13 @@ def test_products_match_the_oracle_bit_for_bit(dtype, ops, ab):
         assert bits_equal(got, want), f'{(m, k, n)}: max diff {np.abs(got - want).max()}'
 
 
+def test_benchmark_product_matches_the_oracle_bit_for_bit(la):
+    "The product the benchmark times, FP64 8192 x 8192 x 8192, equals the oracle's in-order fold item for item."
+    n = 8192
+    a = random(n, n, 0, np.float64); b = random(n, n, 1, np.float64)
+    assert bits_equal(la.matmul(a, b), oracle.matmul(a, b))
+
+
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
 def test_vector_products_follow_by_vector(la, dtype):
     "`by_vector` (core/src/linalg.rs:1020): matrix times vector in the reference's summation orders, bit for bit."

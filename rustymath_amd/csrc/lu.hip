@@ -823,7 +823,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
     }
     const size_t np = panels.size();
     const size_t move_words = 1 + 4 * PW;
-    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 3;  // + status, tile counter, stop mark
+    // + status and stop mark, then one tile counter per block (for the product run beside that block's panels)
+    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 2 + (blocks.size() + 1) / 2;
     int* d_leaf = (int*)pool_get(n * sizeof(int));
     T* scratch = (T*)pool_get(sizeof(T) * (size_t)n * PW);
     char* xbuf = (char*)pool_get(clear_words * 8);
@@ -834,8 +835,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
     Exchange* ex = (Exchange*)xbuf;
     long long* moves = (long long*)(xbuf + sizeof(Exchange));
     int* status = (int*)(moves + np * move_words);
-    int* trail_next = status + 2;
-    unsigned* trail_stop = (unsigned*)(status + 4);
+    unsigned* trail_stop = (unsigned*)(status + 2);
+    int* trail_next = status + 4;  // blocks.size() counters
     HS_HIP(hipMemcpyAsync(d_leaf, leaf_end.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
     fill_words<<<(unsigned)((clear_words + 255) / 256), 256, 0, s>>>((unsigned long long*)xbuf, (int64_t)clear_words, 0ull);
     HS_HIP(hipGetLastError());
@@ -868,10 +869,6 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
         const auto [p0, p1] = blocks[b];
         const int64_t B0 = panels[p0].first, B1 = panels[p1 - 1].second;
         const bool ahead = job.C != nullptr;
-        if (ahead) {
-            fill_words<<<1, 64, 0, s>>>((unsigned long long*)job.next, 1, 0ull);
-            HS_HIP(hipGetLastError());
-        }
         for (size_t i = p0; i < p1 && rc == OK; ++i) {
             const auto [c0, c1] = panels[i];
             const int w = (int)(c1 - c0);
@@ -946,7 +943,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
         job.ldc = lda;
         job.tiles_m = (int)((job.m + tile::BM - 1) / tile::BM);
         job.tiles_n = (int)((job.n + tile::BN - 1) / tile::BN);
-        job.next = trail_next;
+        job.next = trail_next + b + 1;  // the next block's panels take its tiles
         job.stop = trail_stop;
     }
     if (rc == OK) {

@@ -18,7 +18,7 @@ PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'hbm': 8.0e12}
 # rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --only-mixed`, each
 # scaled by what the same counter reads on a 1 GiB device copy (`bench.py --copy-probe 1024`: FETCH_SIZE counts half
 # the bytes read, WRITE_SIZE all the bytes written); the summary is profiles/trail_rest_f32_lu_n16384_traffic.md.
-TRAFFIC = {('trail_f32', 16384): 399651212}
+TRAFFIC = {('trail_f32', 16384): 395992297}
 
 
 def householder_matrix(n, seed, cond=1e6, reflectors=8):

@@ -68,17 +68,24 @@ __device__ __forceinline__ void publish1(unsigned long long* pub, int64_t row, T
         __hip_atomic_store(&pub[2 * row + 1], (epoch << 32) | (u >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
+// A row's words are loaded (load1) and decoded (decode1) apart, so that other loads can be issued in between.
 template <typename T>
-__device__ __forceinline__ bool fetch1(unsigned long long* pub, int64_t row, unsigned long long epoch, T& v) {
+__device__ __forceinline__ void load1(unsigned long long* pub, int64_t row, unsigned long long* w) {
     if constexpr (sizeof(T) == 4) {
-        const unsigned long long w = __hip_atomic_load(&pub[row], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        v = __builtin_bit_cast(T, (unsigned)w);
-        return (w >> 32) == epoch;
+        w[0] = __hip_atomic_load(&pub[row], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
-        const unsigned long long w0 = __hip_atomic_load(&pub[2 * row], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long w1 = __hip_atomic_load(&pub[2 * row + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        v = __builtin_bit_cast(T, (w1 << 32) | (w0 & 0xffffffffull));
-        return (w0 >> 32) == epoch && (w1 >> 32) == epoch;
+        w[0] = __hip_atomic_load(&pub[2 * row], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w[1] = __hip_atomic_load(&pub[2 * row + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+template <typename T>
+__device__ __forceinline__ bool decode1(const unsigned long long* w, unsigned long long epoch, T& v) {
+    if constexpr (sizeof(T) == 4) {
+        v = __builtin_bit_cast(T, (unsigned)w[0]);
+        return (w[0] >> 32) == epoch;
+    } else {
+        v = __builtin_bit_cast(T, (w[1] << 32) | (w[0] & 0xffffffffull));
+        return (w[0] >> 32) == epoch && (w[1] >> 32) == epoch;
     }
 }
 template <typename T>
@@ -112,36 +119,58 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
     const bool act = lane < h;
     const int64_t i = r0 + (act ? lane : 0);
     T acc = act ? x[i] : T(0);
-    // Tile of producer block pq: e[j] = entry (i, pq B1 + j), which is F[pq B1 + j][i] or, row-wise, F[i][pq B1 + j];
-    // padded with zeros.
+    // Tile of producer block pq: e[j] = entry (i, pq B1 + j), which is F[pq B1 + j][i] or, row-wise, F[i][pq B1 + j].
+    // Columns past n read column n - 1 here and are zeroed where the tile is used, so that no instruction touches
+    // the loaded values before then: the wait for a producer's rows (a load issued before the tile's loads) then
+    // waits for that load alone, and the tile's loads overlap the wait.
     auto load_tile = [&](int pq, T* e) {
         const int64_t p0 = (int64_t)pq * B1;
 #pragma unroll
         for (int j = 0; j < B1; ++j) {
             const int64_t pc = p0 + j < n ? p0 + j : n - 1;
-            const T v = rowwise ? f[i * ld + pc] : f[pc * ld + i];
-            e[j] = p0 + j < n ? v : T(0);
+            e[j] = rowwise ? f[i * ld + pc] : f[pc * ld + i];
         }
     };
     const int np = fwd ? q : nb - 1 - q;
     T e[B1], en[B1], d[B1];
-    load_tile(q, d);  // the diagonal tile, off the critical path
+    load_tile(q, d);  // the diagonal tile (its columns past n are never used), off the critical path
     if (np > 0) load_tile(fwd ? 0 : nb - 1, e);
     for (int t = 0; t < np; ++t) {
         const int pq = fwd ? t : nb - 1 - t;
-        if (t + 1 < np) load_tile(fwd ? t + 1 : nb - 2 - t, en);
         const int64_t p0 = (int64_t)pq * B1;
         const int ph = (int)(n - p0 < B1 ? n - p0 : B1);
+        // Every lane reads a word (lanes past the producer's rows read its first row and keep +0). The first look is
+        // issued before the next tile's loads and decoded after them, with scheduling barriers keeping that order,
+        // so that its wait leaves the tile's loads in flight (the last step reloads its own tile, unused, so that no
+        // branch stands between them).
         T xv = T(0);
-        bool ok = lane >= ph;
-        uint64_t spins = 0;
-        while (true) {
-            if (!ok) ok = fetch1(pub, p0 + lane, epoch, xv);
-            if (__all(ok)) break;
-            __builtin_amdgcn_s_sleep(2);  // many blocks wait on the same producer: poll its words gently
-            if (++spins > SPIN_LIMIT) { if (lane == 0) atomicExch(status, ERR_LIMIT); return; }
+        bool ok = false;
+        unsigned long long w[2];
+        const int64_t prow = p0 + (lane < ph ? lane : 0);
+        auto take = [&]() {
+            T v;
+            ok = lane >= ph || decode1(w, epoch, v);  // a published word keeps its value for the rest of the pass
+            xv = lane < ph ? v : T(0);
+        };
+        load1<T>(pub, prow, w);
+        __builtin_amdgcn_sched_barrier(0);
+        load_tile(t + 1 < np ? (fwd ? t + 1 : nb - 2 - t) : pq, en);
+        __builtin_amdgcn_sched_barrier(0);
+        take();
+        if (!__all(ok)) {  // outside the polling loop, so that the first decode waits for its own load alone
+            uint64_t spins = 0;
+            do {
+                __builtin_amdgcn_s_sleep(2);  // many blocks wait on the same producer: poll its words gently
+                if (++spins > SPIN_LIMIT) { if (lane == 0) atomicExch(status, ERR_LIMIT); return; }
+                load1<T>(pub, prow, w);
+                take();
+            } while (!__all(ok));
         }
-        // Rows past the producer's end hold +0 with zero items beside them: their terms add -0, which changes no sum.
+        // Rows past the producer's end hold +0 and their items are zeroed: their terms add -0, which changes no sum.
+        if (ph < B1) {
+#pragma unroll
+            for (int j = 0; j < B1; ++j) if (j >= ph) e[j] = T(0);
+        }
         if (fwd) {
 #pragma unroll
             for (int j = 0; j < B1; ++j) acc = __builtin_elementwise_fma(lane_value(xv, j), -e[j], acc);

@@ -242,7 +242,7 @@ def test_solves_with_subnormal_diagonals_match_the_oracle(dtype, k):
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
-@pytest.mark.parametrize('n', [1, 5, 100, 513, 1030])
+@pytest.mark.parametrize('n', [1, 5, 64, 100, 129, 513, 1030])
 @pytest.mark.parametrize('k', [1, 3, 16, 21, 70])
 def test_solves_and_transposed_solves_match_the_oracle_at_block_sizes(dtype, n, k):
     """Blocked substitutions: 64-row block solves joined by in-order MFMA products, for A x = b (`lu_solve`'s orders,

@@ -57,15 +57,18 @@ __global__ void scatter_rows(int64_t n, int k, const T* __restrict__ w, int64_t 
 // next producer is loaded before the wait. A row adds the producers' rows in the reference's order, in
 // increasing (L) or decreasing (U, `single`) column order, and the block then substitutes within itself
 // with register broadcasts: a quotient by the diagonal at each row's turn for U.
-template <typename T>
+// SCOPE is the store's: agent scope (sc1) for words that any XCD reads, wavefront scope (a plain store, which stays
+// in the XCD's L2) for words that only workgroups of the writer's XCD read; readers load both kinds at agent scope,
+// which reads the L2 of their own XCD.
+template <typename T, int SCOPE = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ void publish1(unsigned long long* pub, int64_t row, T v, unsigned long long epoch) {
     if constexpr (sizeof(T) == 4) {
         const unsigned long long w = (epoch << 32) | __builtin_bit_cast(unsigned, v);
-        __hip_atomic_store(&pub[row], w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&pub[row], w, __ATOMIC_RELAXED, SCOPE);
     } else {
         const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-        __hip_atomic_store(&pub[2 * row], (epoch << 32) | (u & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&pub[2 * row + 1], (epoch << 32) | (u >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&pub[2 * row], (epoch << 32) | (u & 0xffffffffull), __ATOMIC_RELAXED, SCOPE);
+        __hip_atomic_store(&pub[2 * row + 1], (epoch << 32) | (u >> 32), __ATOMIC_RELAXED, SCOPE);
     }
 }
 // A row's words are loaded (load1) and decoded (decode1) apart, so that other loads can be issued in between.
@@ -101,6 +104,7 @@ __device__ __forceinline__ T lane_value(T v, int l) {
 }
 
 constexpr int B1 = 32;
+constexpr int RUN1 = 16;  // consecutive blocks (in solve order) on one XCD
 
 // One column, one pass: a one-wave block per 32 rows (lane i holds row r0 + i), blocks numbered in solve order. A block
 // adds each earlier block's rows as soon as they are published (tagged words, so a stale word never matches), its
@@ -108,12 +112,21 @@ constexpr int B1 = 32;
 // increasing), BWD_SINGLE (U, decreasing, then a quotient: `single`, factor.rs:725), and for the transposed solve
 // FWD_DIAG (Uᵀ, increasing, then `divide`'s reciprocal multiply) and BWD_UNIT (Lᵀ, decreasing). The transposed modes
 // read rows of F where the others read columns.
+//
+// Launch index w runs on XCD w % 8, and takes solve-order block u, so that runs of `run` consecutive blocks share an
+// XCD: a block publishes its rows twice, to `loc` with plain stores that the next blocks of its run read from the
+// XCD's L2, and to `pub` at agent scope for the other XCDs. With run > 1 a block may start before blocks it waits on,
+// so the host takes run > 1 only for grids that fit on the GPU at once; run = 1 is the launch order.
 template <typename T>
 __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T* __restrict__ f, int64_t ld, T* x,
-                                                   unsigned long long* pub, unsigned long long epoch, int* status) {
-    const int nb = (int)((n + B1 - 1) / B1), b = blockIdx.x, lane = threadIdx.x;
+                                                   unsigned long long* pub, unsigned long long* loc, int run,
+                                                   unsigned long long epoch, int* status) {
+    const int nb = (int)((n + B1 - 1) / B1), lane = threadIdx.x;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int u = ((slot / run) * 8 + xcd) * run + slot % run;  // solve order; a bijection of the padded grid
+    if (u >= nb) return;
     const bool fwd = mode == FWD_UNIT || mode == FWD_DIAG, rowwise = mode == FWD_DIAG || mode == BWD_UNIT;
-    const int q = fwd ? b : nb - 1 - b;  // this block's rows [q B1, ...)
+    const int q = fwd ? u : nb - 1 - u;  // this block's rows [q B1, ...)
     const int64_t r0 = (int64_t)q * B1;
     const int h = (int)(n - r0 < B1 ? n - r0 : B1);
     const bool act = lane < h;
@@ -147,12 +160,13 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
         bool ok = false;
         unsigned long long w[2];
         const int64_t prow = p0 + (lane < ph ? lane : 0);
+        unsigned long long* src = ((t / run) & 7) == xcd ? loc : pub;  // producer t (solve order) on this XCD?
         auto take = [&]() {
             T v;
             ok = lane >= ph || decode1(w, epoch, v);  // a published word keeps its value for the rest of the pass
             xv = lane < ph ? v : T(0);
         };
-        load1<T>(pub, prow, w);
+        load1<T>(src, prow, w);
         __builtin_amdgcn_sched_barrier(0);
         load_tile(t + 1 < np ? (fwd ? t + 1 : nb - 2 - t) : pq, en);
         __builtin_amdgcn_sched_barrier(0);
@@ -162,7 +176,7 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
             do {
                 __builtin_amdgcn_s_sleep(2);  // many blocks wait on the same producer: poll its words gently
                 if (++spins > SPIN_LIMIT) { if (lane == 0) atomicExch(status, ERR_LIMIT); return; }
-                load1<T>(pub, prow, w);
+                load1<T>(src, prow, w);
                 take();
             } while (!__all(ok));
         }
@@ -207,6 +221,7 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
         }
     }
     if (act) {
+        publish1<T, __HIP_MEMORY_SCOPE_WAVEFRONT>(loc, i, acc, epoch);
         publish1(pub, i, acc, epoch);
         x[i] = acc;
     }
@@ -221,16 +236,22 @@ std::atomic<unsigned long long> g_epoch1{1};
 template <typename T>
 int trsv1_lu(bool trans, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
     const int nb = (int)((n + B1 - 1) / B1);
+    // Runs of RUN1 blocks per XCD when the padded grid fits on the GPU at once (one wave per block, at least two per
+    // CU at this kernel's register use), else the launch order, in which no block waits on one launched after it.
+    const int padded = (nb + 8 * RUN1 - 1) / (8 * RUN1) * (8 * RUN1);
+    const int run = padded <= 2 * device_cus() ? RUN1 : 1;
+    const int grid = run > 1 ? padded : (nb + 7) / 8 * 8;
     const size_t words = (size_t)n * (sizeof(T) == 8 ? 2 : 1);
-    char* buf = (char*)pool_get(words * 8 + 64);
+    char* buf = (char*)pool_get(2 * words * 8 + 64);
     if (!buf) return hip_status(hipErrorOutOfMemory, "pool_get");
     unsigned long long* pub = (unsigned long long*)buf;
-    int* status = (int*)(buf + words * 8);
-    HS_HIP(hipMemsetAsync(buf, 0, words * 8 + sizeof(int), s));
+    unsigned long long* loc = pub + words;
+    int* status = (int*)(buf + 2 * words * 8);
+    HS_HIP(hipMemsetAsync(buf, 0, 2 * words * 8 + sizeof(int), s));
     const unsigned long long e1 = (g_epoch1.fetch_add(2) % 0xfffffff0ull) | 1ull;  // odd, and e1 + 1 < 2^32
-    trsv1_kernel<T><<<nb, 64, 0, s>>>(trans ? FWD_DIAG : FWD_UNIT, n, f, ld, x, pub, e1, status);
+    trsv1_kernel<T><<<grid, 64, 0, s>>>(trans ? FWD_DIAG : FWD_UNIT, n, f, ld, x, pub, loc, run, e1, status);
     HS_HIP(hipGetLastError());
-    trsv1_kernel<T><<<nb, 64, 0, s>>>(trans ? BWD_UNIT : BWD_SINGLE, n, f, ld, x, pub, e1 + 1, status);
+    trsv1_kernel<T><<<grid, 64, 0, s>>>(trans ? BWD_UNIT : BWD_SINGLE, n, f, ld, x, pub, loc, run, e1 + 1, status);
     HS_HIP(hipGetLastError());
     int st = 0;
     HS_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -52,9 +52,9 @@ __global__ void scatter_rows(int64_t n, int k, const T* __restrict__ w, int64_t 
 
 // One right-hand side (the refinement's corrections, single-column lu_solve): one wave per 32-row block, in
 // solve order. Lane t holds row r0 + t. A solved row is published as 8-byte words that carry the call's
-// epoch beside the value (two words for FP64), stored and loaded with device-coherent (sc1) atomics, so one
-// load both tells that a row is ready and brings its value; the tile of matrix items a block needs from the
-// next producer is loaded before the wait. A row adds the producers' rows in the reference's order, in
+// epoch beside the value (two words for FP64), stored with device-coherent (sc1) atomics for other XCDs and with
+// plain stores for the next blocks on the same XCD, and loaded at device scope, so one load both tells that a row is
+// ready and brings its value; the tile of matrix items a block needs from the next producer loads during the wait. A row adds the producers' rows in the reference's order, in
 // increasing (L) or decreasing (U, `single`) column order, and the block then substitutes within itself
 // with register broadcasts: a quotient by the diagonal at each row's turn for U.
 // SCOPE is the store's: agent scope (sc1) for words that any XCD reads, wavefront scope (a plain store, which stays

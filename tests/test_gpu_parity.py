@@ -203,6 +203,24 @@ def test_lu_solve_matches_the_oracle_bit_for_bit(la, dtype, n, k):
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_single_column_solve_above_order_16384_matches_the_oracle(la, dtype):
+    """Order 16500: more 32-row blocks than the single-column solve can keep resident in runs per XCD, so its blocks
+    run in launch order. Synthetic factors (diagonal in [1, 2), other items N(0, 1)/n) and a random permutation,
+    against the oracle bit for bit."""
+    n = 16500
+    rng = np.random.default_rng(21)
+    f = rng.standard_normal((n, n)).astype(dtype)
+    f *= dtype(1.0 / n)
+    f[np.diag_indices(n)] = (1 + rng.random(n)).astype(dtype)
+    perm = rng.permutation(n).astype(np.int64)
+    b = rng.standard_normal(n).astype(dtype)
+    got = la.lu_solve(f, perm, b)
+    want = oracle.lu_solve(f, perm, b)
+    assert np.isfinite(want).all()
+    assert bits_equal(got, want), f'max diff {np.abs(got - want).max()}'
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
 @pytest.mark.parametrize('n', [530, 1030])
 def test_blocked_sizes_have_small_residuals(la, dtype, n):
     "tests/linalg.rs:569-590 on the GPU, plus the acceptance bound 10·n·eps on the solve's backward error."

@@ -8,8 +8,9 @@
  *     L strictly below the diagonal with a unit diagonal implied, U on and above; perm[i] is the source
  *     row of row i (core/src/linalg/factor.rs:132-134);
  *   - every entry point returns 0, or -i where argument i is invalid (LAPACK style), or > 0 for a HIP
- *     runtime failure (100000 = a device-side bounded wait ran out); hs_last_error() gives the text.
- *     A zero pivot is not an error (factor.rs:606); NaN propagates;
+ *     runtime failure (100000 = a device-side bounded wait ran out; 100001 = valid arguments beyond what this
+ *     library handles, such as an LU above order 16384); hs_last_error() gives the text.
+ *     A zero pivot is not an error (factor.rs:606): the factorizations report the first one in `info`; NaN propagates;
  *   - the caller owns every buffer it passes. Host-pointer entry points copy in and out through a
  *     device-memory pool the library keeps across calls (as LuWork keeps buffers, factor.rs:196-201).
  *     hs_dev_* entry points take device pointers and a hipStream_t (NULL = the null stream) and
@@ -31,6 +32,12 @@ extern "C" {
 #define HS_NO_TRANS 111
 #define HS_TRANS 112
 #define HS_CONJ_TRANS 113
+#define HS_UPPER 121
+#define HS_LOWER 122
+#define HS_NON_UNIT 131
+#define HS_UNIT 132
+#define HS_LEFT 141
+#define HS_RIGHT 142
 
 /* Library set-up and tear-down; hs_finalize frees the device-memory pool. */
 int hs_init(void);
@@ -48,11 +55,26 @@ int hs_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, 
 int hs_sgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
              const float* b, int64_t ldb, float beta, float* c, int64_t ldc);
 
+/* Triangular solves in place on x (row-major m×n, ld ldx): op(A) X = B (side 141, A m×m) or X op(A) = B (side 142,
+ * A n×n), A row-major with ld lda, of which only the triangle `uplo` names is read (121 upper, 122 lower; diag 131
+ * non-unit, 132 unit diagonal taken as ones); trans 111 / 112 / 113 (113 is 112 for real items).
+ * Replaces `Tiled::triangular` (core/src/linalg.rs:781-787), one of `dispatch!`'s three sites, and its vendor form
+ * `accelerate::triangular` (core/src/linalg/accelerate.rs:67-70: the same codes, alpha = 1). Orders of operations are
+ * `triangular_native`'s (core/src/linalg/factor.rs:711-721): `single` for one contiguous column (ldx = 1), the halving
+ * `forward` / `backward` otherwise, a right-side solve as the left-side solve of the transposed view. The reference's
+ * own callers pass (122, 131) or (121, 132); the other two pairs follow the same native orders. */
+int hs_dtrsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const double* a, int64_t lda, double* x,
+             int64_t ldx);
+int hs_strsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const float* a, int64_t lda, float* x,
+             int64_t ldx);
+
 /* The LU factorization with partial pivoting of the row-major n×n matrix a: factors (n×n, column-major),
- * perm (n) and, where `pivots` is not NULL, the row swapped with row c at step c (n).
+ * perm (n) and, where `pivots` is not NULL, the row swapped with row c at step c (n). Where `info` is not NULL it
+ * gets 0, or 1 + the first column whose pivot is exactly zero (LAPACK's convention; the column is left undivided,
+ * as factor.rs:606 leaves it, and the factorization goes on).
  * Replaces `Factor::matrix_lu_into` / `lu_factor` (core/src/linalg/factor.rs:85-87, :617-630). */
-int hs_dlu_factor(const double* a, int64_t n, double* factors, int64_t* perm, int64_t* pivots);
-int hs_slu_factor(const float* a, int64_t n, float* factors, int64_t* perm, int64_t* pivots);
+int hs_dlu_factor(const double* a, int64_t n, double* factors, int64_t* perm, int64_t* pivots, int64_t* info);
+int hs_slu_factor(const float* a, int64_t n, float* factors, int64_t* perm, int64_t* pivots, int64_t* info);
 
 /* x (n×k, row-major) with A x = b, from factors and perm as hs_*lu_factor gives them; an out-of-range
  * row position is argument -2. Replaces `Factor::matrix_lu_solve` / `lu_solve` (factor.rs:88, :633-641). */
@@ -99,12 +121,26 @@ int hs_dev_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alp
                  const double* b, int64_t ldb, double beta, double* c, int64_t ldc, void* stream);
 int hs_dev_sgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
                  const float* b, int64_t ldb, float beta, float* c, int64_t ldc, void* stream);
+/* hs_dtrsm on device memory (a, x) and `stream`; synchronises `stream` before returning. */
+int hs_dev_dtrsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const double* a, int64_t lda, double* x,
+                 int64_t ldx, void* stream);
+int hs_dev_strsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const float* a, int64_t lda, float* x,
+                 int64_t ldx, void* stream);
 /* Factors the row-major a (ld lda) in place into column-major factors (same buffer, same ld) and writes
- * perm (device, n int64). Synchronises `stream` before returning. */
-int hs_dev_dgetrf(int64_t n, double* a, int64_t lda, int64_t* perm, void* stream);
-int hs_dev_sgetrf(int64_t n, float* a, int64_t lda, int64_t* perm, void* stream);
-/* Solves in place on b (n×k row-major, ld ldb) with device factors: A x = b (trans = 0) in the reference's
- * substitution orders, or Aᵀ x = b (trans = 1, for adjoints). */
+ * perm (device, n int64) and, where not NULL, *info (host) as hs_dlu_factor gives it. Synchronises `stream` before
+ * returning. */
+int hs_dev_dgetrf(int64_t n, double* a, int64_t lda, int64_t* perm, int64_t* info, void* stream);
+int hs_dev_sgetrf(int64_t n, float* a, int64_t lda, int64_t* perm, int64_t* info, void* stream);
+/* `LuWork::solve` (factor.rs:207-226) on device memory: factors a (ld lda) in place as hs_dev_dgetrf does, writes perm
+ * (device), and sets *regular (host) to 1 where every pivot exceeds eps·n·max|a| (NaN counting as not negligible), then
+ * solves b (n×k row-major, ld ldb, device) in place; *regular = 0 and b untouched where the reference would take least
+ * squares, which this library leaves to the caller. Synchronises `stream`. */
+int hs_dev_dgesv(int64_t n, double* a, int64_t lda, int64_t* perm, double* b, int64_t k, int64_t ldb, int* regular,
+                 void* stream);
+int hs_dev_sgesv(int64_t n, float* a, int64_t lda, int64_t* perm, float* b, int64_t k, int64_t ldb, int* regular,
+                 void* stream);
+/* Solves in place on b (n×k row-major, ld ldb >= k) with device factors (ld ldf >= n): A x = b (trans = 0) in the
+ * reference's substitution orders, or Aᵀ x = b (trans = 1, for adjoints). */
 int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_t ldf, const int64_t* perm, double* b,
                   int64_t ldb, void* stream);
 int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t ldf, const int64_t* perm, float* b,

@@ -38,6 +38,7 @@ def lib():
                 f.argtypes = [vp, i64, vp, vp, vp]
                 f.restype = i32
             f = getattr(_lib, 'or_lu_solve_' + s); f.argtypes = [vp, vp, i64, vp, i64, vp]; f.restype = i32
+            f = getattr(_lib, 'or_trsm_' + s); f.argtypes = [i32, i32, i32, i32, i64, i64, vp, i64, vp, i64]; f.restype = i32
             f = getattr(_lib, 'or_lu_solve_t_' + s); f.argtypes = [vp, vp, i64, vp, i64, vp]; f.restype = i32
             f = getattr(_lib, 'or_lu_inverse_' + s); f.argtypes = [vp, vp, i64, vp]; f.restype = i32
             f = getattr(_lib, 'or_det_' + s); f.argtypes = [vp, vp, i64]; f.restype = t
@@ -119,6 +120,30 @@ def lu_solve(factors, perm, b):
     st = getattr(lib(), 'or_lu_solve_' + s)(_p(factors), _p(perm), n, _p(b2), k, _p(x))
     if st: raise IndexError('row positions out of range')
     return x[:, 0] if vec else x
+
+
+SIDES = {'L': 141, 'R': 142}
+UPLOS = {'U': 121, 'L': 122}
+DIAGS = {'N': 131, 'U': 132}
+
+
+def trsm(a, x, side='L', uplo='L', trans='N', diag='N', ldx=None):
+    """`Tiled::triangular` (core/src/linalg.rs:781) as `triangular_native` (factor.rs:711-721) computes it: solves
+    op(a) X = B (side 'L') or X op(a) = B ('R') for the row-major B held in `x` (m×n), reading only a's triangle `uplo`
+    ('U' or 'L', with a unit diagonal for diag 'U'); trans 'N', 'T' or 'C'. `ldx` (>= n) lays x out with that leading
+    dimension, as a strided view would hold it (one column with ldx > 1 takes `forward`/`backward` rather than
+    `single`). Returns the solution, a new m×n array."""
+    a = np.ascontiguousarray(a)
+    s = _suffix(a.dtype)
+    x = np.asarray(x, dtype=a.dtype)
+    m, n = x.shape
+    ld = n if ldx is None else ldx
+    buf = np.zeros(max((m - 1) * ld + n, 1) if m and n else 1, a.dtype)
+    for i in range(m): buf[i * ld:i * ld + n] = x[i]
+    st = getattr(lib(), 'or_trsm_' + s)(SIDES[side], UPLOS[uplo], OPS[trans], DIAGS[diag], m, n, _p(a),
+                                        max(a.shape[1], 1), _p(buf), max(ld, 1))
+    if st: raise ValueError(f'trsm rejected argument {-st}')
+    return np.array([buf[i * ld:i * ld + n] for i in range(m)], a.dtype).reshape(m, n)
 
 
 def lu_solve_t(factors, perm, b):

@@ -328,14 +328,53 @@ static void F(single_)(T* x, F(view_) a, int64_t m, int lower, int unit) {
 #undef SOLVED
 }
 
-/* `triangular_native`, factor.rs:711-721, for a left-side solve: `stored_lower` is Triangle::Lower,
- * otherwise Triangle::UnitUpper, of the stored matrix before the view's transpose. */
-static void F(triangular_)(T* x, int64_t ldx, int64_t xlen, F(view_) a, int64_t m, int64_t n, int stored_lower) {
+/* The left-side solve of `triangular_native`, factor.rs:711-721, once the effective triangle (`lower`, the stored
+ * triangle's side exchanged by a transposing view) and the diagonal (`unit`) are known: `single` for one contiguous
+ * column, else `forward` (leaves of LEAF = 16 rows where x is at least as wide as the matrix is tall, else NS = 32) or
+ * `backward`. */
+static void F(solve_left_)(T* x, int64_t ldx, int64_t xlen, F(view_) a, int64_t m, int64_t n, int lower, int unit) {
     if (m == 0 || n == 0) return;
-    int lower = stored_lower != a.trans, unit = !stored_lower;
     if (n == 1 && ldx == 1) { F(single_)(x, a, m, lower, unit); return; }
     if (lower) F(forward_)(a, 0, m, x, xlen, ldx, n, n >= m ? 16 : 32, 0, unit);
     else F(backward_)(a, 0, m, x, ldx, n, unit);
+}
+
+/* `triangular_native`, factor.rs:711-721, for a left-side solve: `stored_lower` is Triangle::Lower,
+ * otherwise Triangle::UnitUpper, of the stored matrix before the view's transpose. */
+static void F(triangular_)(T* x, int64_t ldx, int64_t xlen, F(view_) a, int64_t m, int64_t n, int stored_lower) {
+    F(solve_left_)(x, ldx, xlen, a, m, n, stored_lower != a.trans, !stored_lower);
+}
+
+/* `Tiled::triangular` (core/src/linalg.rs:781-787) as `triangular_native` computes it, under the CBLAS codes of its
+ * vendor seam (core/src/linalg/accelerate.rs:67-70): side 141 / 142, uplo 121 / 122 (the stored triangle), trans
+ * 111 / 112 / 113 (113 is 112 for real items), diag 131 / 132. The reference's Triangle::Lower is (122, 131) and
+ * Triangle::UnitUpper (121, 132); the other two pairs take the same orders from (lower, unit). A right-side solve
+ * transposes x into a compact copy, solves on the left with the transposed view and transposes back (:713-717). x is
+ * row-major m×n with ld ldx, a row-major with ld lda; 0, or -i for invalid argument i. */
+int F(or_trsm_)(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const T* a, int64_t lda, T* x, int64_t ldx) {
+    if (side != 141 && side != 142) return -1;
+    if (uplo != 121 && uplo != 122) return -2;
+    if (trans != 111 && trans != 112 && trans != 113) return -3;
+    if (diag != 131 && diag != 132) return -4;
+    if (m < 0) return -5;
+    if (n < 0) return -6;
+    int64_t order = side == 141 ? m : n;
+    if (m > 0 && n > 0 && lda < order) return -8;
+    if (m > 0 && n > 0 && ldx < n) return -10;
+    if (m == 0 || n == 0) return 0;
+    int stored_lower = uplo == 122, unit = diag == 132, ta = trans != 111;
+    F(view_) v = {a, lda, ta, (order - 1) * lda + order};
+    if (side == 141) {
+        F(solve_left_)(x, ldx, (m - 1) * ldx + n, v, m, n, stored_lower != ta, unit);
+        return 0;
+    }
+    T* xt = (T*)malloc(sizeof(T) * (size_t)(m * n));
+    for (int64_t i = 0; i < m; ++i) for (int64_t j = 0; j < n; ++j) xt[j * m + i] = x[i * ldx + j];
+    v.trans = !ta;
+    F(solve_left_)(xt, m, n * m, v, n, m, stored_lower != !ta, unit);
+    for (int64_t i = 0; i < m; ++i) for (int64_t j = 0; j < n; ++j) x[i * ldx + j] = xt[j * m + i];
+    free(xt);
+    return 0;
 }
 
 static int64_t F(stride_)(int64_t n) { return n > 0 && n % 128 == 0 ? n + 8 : n; }

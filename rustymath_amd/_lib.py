@@ -9,6 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PATH = os.path.join(HERE, 'libhip_sys.so')
 
 NO_TRANS, TRANS, CONJ_TRANS = 111, 112, 113
+UPPER, LOWER, NON_UNIT, UNIT, LEFT, RIGHT = 121, 122, 131, 132, 141, 142
 
 _i32, _i64, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
 _d, _f = ctypes.c_double, ctypes.c_float
@@ -20,8 +21,12 @@ SIGNATURES = {
     'hs_last_error': (ctypes.c_char_p, []),
     'hs_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
     'hs_sgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64]),
-    'hs_dlu_factor': (_i32, [_vp, _i64, _vp, _vp, _vp]),
-    'hs_slu_factor': (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    'hs_dtrsm': (_i32, [_i32, _i32, _i32, _i32, _i64, _i64, _vp, _i64, _vp, _i64]),
+    'hs_strsm': (_i32, [_i32, _i32, _i32, _i32, _i64, _i64, _vp, _i64, _vp, _i64]),
+    'hs_dev_dtrsm': (_i32, [_i32, _i32, _i32, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    'hs_dev_strsm': (_i32, [_i32, _i32, _i32, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),
+    'hs_dlu_factor': (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    'hs_slu_factor': (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
     'hs_dlu_solve': (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
     'hs_slu_solve': (_i32, [_vp, _vp, _i64, _vp, _i64, _vp]),
     'hs_dlu_inverse': (_i32, [_vp, _vp, _i64, _vp]),
@@ -40,8 +45,10 @@ SIGNATURES = {
     'hs_ssolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),
     'hs_dev_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
     'hs_dev_sgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64, _vp]),
-    'hs_dev_dgetrf': (_i32, [_i64, _vp, _i64, _vp, _vp]),
-    'hs_dev_sgetrf': (_i32, [_i64, _vp, _i64, _vp, _vp]),
+    'hs_dev_dgetrf': (_i32, [_i64, _vp, _i64, _vp, _vp, _vp]),
+    'hs_dev_sgetrf': (_i32, [_i64, _vp, _i64, _vp, _vp, _vp]),
+    'hs_dev_dgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
+    'hs_dev_sgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
     'hs_dev_dgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_sgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_dgetri': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
@@ -87,8 +94,14 @@ def load():
     return _lib
 
 
+UNSUPPORTED = 100001  # valid arguments beyond what the library handles (include/hip_sys.h)
+
+
 def check(status, what):
-    "Turns a status into Python's errors: ValueError for a bad argument, HipSysError for a device failure."
+    """Turns a status into Python's errors: ValueError for a bad argument, NotImplementedError for valid arguments the
+    library does not handle, HipSysError for a device failure."""
     if status == 0: return
     if status < 0: raise ValueError(f'{what}: argument {-status} is invalid')
-    raise HipSysError(f'{what}: {load().hs_last_error().decode(errors="replace")} (status {status})')
+    text = load().hs_last_error().decode(errors="replace")
+    if status == UNSUPPORTED: raise NotImplementedError(f'{what}: {text}')
+    raise HipSysError(f'{what}: {text} (status {status})')

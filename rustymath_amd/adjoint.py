@@ -47,8 +47,11 @@ def _factor_solve(a, b):
         return fa.view(batch, n, n), perms.view(batch, n), x, 'rows'
     fs, ps, xs = [], [], []
     for i in range(batch):
-        f, perm = dv.getrf(a[i].detach().contiguous().clone())
-        x = dv.getrs(f, perm, b[i].detach().contiguous().clone())
+        x = b[i].detach().contiguous().clone()
+        f, perm, regular = dv.gesv(a[i].detach().contiguous().clone(), x)
+        if not regular:
+            raise NotImplementedError('the matrix is singular to working precision: rustymath would take least squares '
+                                      'there')
         fs.append(f); ps.append(perm); xs.append(x)
     return torch.stack(fs), torch.stack(ps), torch.stack(xs), 'columns'
 

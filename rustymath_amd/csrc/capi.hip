@@ -8,6 +8,7 @@
 #include <vector>
 #include <cstring>
 #include <cmath>
+#include <algorithm>
 
 namespace hs {
 
@@ -73,6 +74,11 @@ void pool_release_all() {
     g_free.clear();
 }
 
+std::mutex& resident_mutex() {
+    static std::mutex mu;
+    return mu;
+}
+
 int device_cus() {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
@@ -82,12 +88,14 @@ int device_cus() {
 
 namespace {
 
-// `greatest(a, modulus)` (core/src/linalg/factor.rs:488) over n×n items: NaN ignored, 0 for none.
+// `greatest(a, modulus)` (core/src/linalg/factor.rs:488) over the rows×cols items of a (rows ld apart): NaN ignored,
+// 0 for none.
 template <typename T>
-__global__ void maxabs_kernel(int64_t count, const T* a, unsigned long long* out) {
+__global__ void maxabs_kernel(int64_t rows, int64_t cols, int64_t ld, const T* a, unsigned long long* out) {
     T m = T(0);
+    const int64_t count = rows * cols;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
-        T v = fabs(a[i]);
+        T v = fabs(a[(i / cols) * ld + i % cols]);
         m = v > m ? v : m;
     }
     for (int off = 32; off > 0; off >>= 1) { T o = __shfl_xor(m, off); m = o > m ? o : m; }
@@ -107,16 +115,32 @@ __global__ void regular_kernel(int64_t n, const T* f, int64_t ld, const unsigned
 
 __global__ void fill_int(int* p, int v) { *p = v; }
 
+// The first column whose pivot is exactly zero (the diagonal of the column-major factors), as 1 + its index in *out,
+// which starts at n + 1 (LAPACK's `info`: factor.rs:606 leaves such a column undivided and goes on).
 template <typename T>
-int dev_getrf_perm(int64_t n, T* a, int64_t lda, int64_t* d_perm, int64_t* h_perm_out, int64_t* h_piv_out, hipStream_t s) {
-    int64_t* d_piv = (int64_t*)pool_get(sizeof(int64_t) * (size_t)(n ? n : 1));
+__global__ void first_zero_pivot(int64_t n, const T* __restrict__ f, int64_t ld, unsigned long long* out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (f[i * ld + i] == T(0)) atomicMin(out, (unsigned long long)(i + 1));
+}
+__global__ void fill_u64(unsigned long long* p, unsigned long long v) { *p = v; }
+
+template <typename T>
+int dev_getrf_perm(int64_t n, T* a, int64_t lda, int64_t* d_perm, int64_t* h_perm_out, int64_t* h_piv_out, hipStream_t s,
+                   int64_t* info = nullptr) {
+    if (info) *info = 0;
+    int64_t* d_piv = (int64_t*)pool_get(sizeof(int64_t) * (size_t)((n ? n : 1) + 1));
     if (!d_piv) return hip_status(hipErrorOutOfMemory, "pool_get");
     int rc = getrf<T>(n, a, lda, d_piv, s);
     if (rc) { pool_put(d_piv); return rc; }
-    std::vector<int64_t> piv(n), perm(n);
-    HS_HIP(hipMemcpyAsync(piv.data(), d_piv, sizeof(int64_t) * n, hipMemcpyDeviceToHost, s));
+    std::vector<int64_t> piv(n + 1), perm(n);
+    unsigned long long* d_zero = (unsigned long long*)(d_piv + n);
+    fill_u64<<<1, 1, 0, s>>>(d_zero, (unsigned long long)n + 1);
+    first_zero_pivot<T><<<(unsigned)std::min<int64_t>((n + 255) / 256, 1024), 256, 0, s>>>(n, a, lda, d_zero);
+    HS_HIP(hipGetLastError());
+    HS_HIP(hipMemcpyAsync(piv.data(), d_piv, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, s));
     HS_HIP(hipStreamSynchronize(s));
     pool_put(d_piv);
+    if (info) *info = piv[n] > n ? 0 : piv[n];
     for (int64_t i = 0; i < n; ++i) perm[i] = i;
     for (int64_t c = 0; c < n; ++c) std::swap(perm[c], perm[piv[c]]);
     if (d_perm) HS_HIP(hipMemcpyAsync(d_perm, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
@@ -143,6 +167,58 @@ int check_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, int64_t lda, i
     if (k > 0 && n > 0 && ldb < (tb ? k : n)) return -10;
     if (m > 0 && n > 0 && ldc < n) return -13;
     return 0;
+}
+
+// The CBLAS codes of a triangular solve (core/src/linalg/accelerate.rs:67-70): side 141 / 142 (left / right), uplo
+// 121 / 122 (upper / lower stored triangle), trans 111 / 112 / 113, diag 131 / 132 (non-unit / unit).
+int check_trsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, int64_t lda, int64_t ldx, bool& right,
+               bool& lower, bool& ta, bool& unit) {
+    if (side != 141 && side != 142) return -1;
+    if (uplo != 121 && uplo != 122) return -2;
+    if (op_of(trans, ta)) return -3;
+    if (diag != 131 && diag != 132) return -4;
+    if (m < 0) return -5;
+    if (n < 0) return -6;
+    right = side == 142;
+    lower = uplo == 122;
+    unit = diag == 132;
+    const int64_t order = right ? n : m;
+    if (m > 0 && n > 0 && lda < order) return -8;
+    if (m > 0 && n > 0 && ldx < n) return -10;
+    return 0;
+}
+
+template <typename T>
+int host_trsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const T* a, int64_t lda, T* x, int64_t ldx) {
+    bool right, lower, ta, unit;
+    int e = check_trsm(side, uplo, trans, diag, m, n, lda, ldx, right, lower, ta, unit);
+    if (e) return e;
+    if (m == 0 || n == 0) return OK;
+    const int64_t order = right ? n : m;
+    hipStream_t s = thread_stream();
+    T* da = (T*)pool_get(sizeof(T) * (size_t)(order * order));
+    T* dx = (T*)pool_get(sizeof(T) * (size_t)(m * n));
+    int rc = OK;
+    if (!da || !dx) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && (hipMemcpy2DAsync(da, order * sizeof(T), a, lda * sizeof(T), order * sizeof(T), order, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpy2DAsync(dx, n * sizeof(T), x, ldx * sizeof(T), n * sizeof(T), m, hipMemcpyHostToDevice, s) != hipSuccess))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    // x is compact on the device; whether the caller's x is one contiguous column still picks `single`'s orders.
+    if (!rc) rc = trsm<T>(right, lower, ta, unit, m, n, da, order, dx, n, ldx == 1, s);
+    if (!rc && hipMemcpy2DAsync(x, ldx * sizeof(T), dx, n * sizeof(T), n * sizeof(T), m, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = hip_status(hipErrorUnknown, "D2H");
+    hipStreamSynchronize(s);
+    pool_put(da); pool_put(dx);
+    return rc;
+}
+
+template <typename T>
+int dev_trsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const T* a, int64_t lda, T* x, int64_t ldx,
+             void* stream) {
+    bool right, lower, ta, unit;
+    int e = check_trsm(side, uplo, trans, diag, m, n, lda, ldx, right, lower, ta, unit);
+    if (e) return e;
+    return trsm<T>(right, lower, ta, unit, m, n, a, lda, x, ldx, ldx == 1, (hipStream_t)stream);
 }
 
 template <typename T>
@@ -183,15 +259,16 @@ done:
 }
 
 template <typename T>
-int host_lu_factor(const T* a, int64_t n, T* factors, int64_t* perm, int64_t* pivots) {
+int host_lu_factor(const T* a, int64_t n, T* factors, int64_t* perm, int64_t* pivots, int64_t* info = nullptr) {
     if (n < 0) return -2;
+    if (info) *info = 0;
     if (n == 0) return OK;
     hipStream_t s = thread_stream();
     T* d = (T*)pool_get(sizeof(T) * (size_t)(n * n));
     if (!d) return hip_status(hipErrorOutOfMemory, "pool_get");
     int rc = OK;
     if (hipMemcpyAsync(d, a, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "H2D");
-    if (!rc) rc = dev_getrf_perm<T>(n, d, n, nullptr, perm, pivots, s);
+    if (!rc) rc = dev_getrf_perm<T>(n, d, n, nullptr, perm, pivots, s, info);
     if (!rc && hipMemcpyAsync(factors, d, sizeof(T) * n * n, hipMemcpyDeviceToHost, s) != hipSuccess) rc = hip_status(hipErrorUnknown, "D2H");
     hipStreamSynchronize(s);
     pool_put(d);
@@ -228,7 +305,7 @@ int host_factor_regular(const T* a, int64_t n, T* d, int64_t* dp, unsigned long 
     reg = 0;
     if (hipMemcpyAsync(d, a, sizeof(T) * n * n, hipMemcpyHostToDevice, s) != hipSuccess) return hip_status(hipErrorUnknown, "H2D");
     hipMemsetAsync(dm, 0, 64, s);
-    maxabs_kernel<T><<<256, 256, 0, s>>>(n * n, d, dm);
+    maxabs_kernel<T><<<256, 256, 0, s>>>(n, n, n, d, dm);
     int rc = dev_getrf_perm<T>(n, d, n, dp, nullptr, nullptr, s);
     if (rc) return rc;
     int* dreg = (int*)(dm + 1);
@@ -260,6 +337,34 @@ int host_solve(const T* a, int64_t n, const T* b, int64_t k, T* x, int* regular)
     hipStreamSynchronize(s);
     pool_put(d); pool_put(db); pool_put(dp); pool_put(dm);
     if (regular) *regular = rc ? 0 : reg;
+    return rc;
+}
+
+// `LuWork::solve` (factor.rs:207-226) on device memory: a (ld lda) factored in place, perm written, `regular` decided
+// (every pivot exceeds eps·n·max|a|, NaN not negligible) and, where regular, b (n×k, ld ldb) solved in place.
+template <typename T>
+int dev_gesv(int64_t n, T* a, int64_t lda, int64_t* perm, T* b, int64_t k, int64_t ldb, int* regular, hipStream_t s) {
+    *regular = 0;
+    if (n == 0) return OK;
+    unsigned long long* dm = (unsigned long long*)pool_get(64);
+    if (!dm) return hip_status(hipErrorOutOfMemory, "pool_get");
+    int rc = OK, reg = 0;
+    fill_u64<<<1, 1, 0, s>>>(dm, 0ull);
+    maxabs_kernel<T><<<256, 256, 0, s>>>(n, n, lda, a, dm);
+    if (hipGetLastError() != hipSuccess) rc = hip_status(hipErrorUnknown, "maxabs");
+    if (!rc) rc = dev_getrf_perm<T>(n, a, lda, perm, nullptr, nullptr, s);
+    if (!rc) {
+        int* dreg = (int*)(dm + 1);
+        fill_int<<<1, 1, 0, s>>>(dreg, 1);
+        regular_kernel<T><<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, a, lda, dm, dreg);
+        if (hipMemcpyAsync(&reg, dreg, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = hip_status(hipErrorUnknown, "regular");
+    }
+    if (!rc && reg && k) rc = getrs<T>(false, n, k, a, lda, perm, b, ldb, s);
+    hipStreamSynchronize(s);
+    pool_put(dm);
+    *regular = rc ? 0 : reg;
     return rc;
 }
 
@@ -468,11 +573,27 @@ int hs_sgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha, con
              int64_t ldb, float beta, float* c, int64_t ldc) {
     return host_gemm<float>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
 }
-int hs_dlu_factor(const double* a, int64_t n, double* factors, int64_t* perm, int64_t* pivots) {
-    return host_lu_factor<double>(a, n, factors, perm, pivots);
+int hs_dtrsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const double* a, int64_t lda, double* x,
+             int64_t ldx) {
+    return host_trsm<double>(side, uplo, trans, diag, m, n, a, lda, x, ldx);
 }
-int hs_slu_factor(const float* a, int64_t n, float* factors, int64_t* perm, int64_t* pivots) {
-    return host_lu_factor<float>(a, n, factors, perm, pivots);
+int hs_strsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const float* a, int64_t lda, float* x,
+             int64_t ldx) {
+    return host_trsm<float>(side, uplo, trans, diag, m, n, a, lda, x, ldx);
+}
+int hs_dev_dtrsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const double* a, int64_t lda, double* x,
+                 int64_t ldx, void* stream) {
+    return dev_trsm<double>(side, uplo, trans, diag, m, n, a, lda, x, ldx, stream);
+}
+int hs_dev_strsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const float* a, int64_t lda, float* x,
+                 int64_t ldx, void* stream) {
+    return dev_trsm<float>(side, uplo, trans, diag, m, n, a, lda, x, ldx, stream);
+}
+int hs_dlu_factor(const double* a, int64_t n, double* factors, int64_t* perm, int64_t* pivots, int64_t* info) {
+    return host_lu_factor<double>(a, n, factors, perm, pivots, info);
+}
+int hs_slu_factor(const float* a, int64_t n, float* factors, int64_t* perm, int64_t* pivots, int64_t* info) {
+    return host_lu_factor<float>(a, n, factors, perm, pivots, info);
 }
 int hs_dlu_solve(const double* factors, const int64_t* perm, int64_t n, const double* b, int64_t k, double* x) {
     return host_lu_solve<double>(factors, perm, n, b, k, x);
@@ -533,22 +654,42 @@ int hs_dev_sgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha,
                  int64_t ldb, float beta, float* c, int64_t ldc, void* stream) {
     return dev_gemm<float>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream);
 }
-int hs_dev_dgetrf(int64_t n, double* a, int64_t lda, int64_t* perm, void* stream) {
+int hs_dev_dgetrf(int64_t n, double* a, int64_t lda, int64_t* perm, int64_t* info, void* stream) {
     if (n < 0) return -1;
     if (lda < n) return -3;
     hipStream_t s = (hipStream_t)stream;
-    return dev_getrf_perm<double>(n, a, lda, perm, nullptr, nullptr, s);
+    return dev_getrf_perm<double>(n, a, lda, perm, nullptr, nullptr, s, info);
 }
-int hs_dev_sgetrf(int64_t n, float* a, int64_t lda, int64_t* perm, void* stream) {
+int hs_dev_sgetrf(int64_t n, float* a, int64_t lda, int64_t* perm, int64_t* info, void* stream) {
     if (n < 0) return -1;
     if (lda < n) return -3;
     hipStream_t s = (hipStream_t)stream;
-    return dev_getrf_perm<float>(n, a, lda, perm, nullptr, nullptr, s);
+    return dev_getrf_perm<float>(n, a, lda, perm, nullptr, nullptr, s, info);
+}
+int hs_dev_dgesv(int64_t n, double* a, int64_t lda, int64_t* perm, double* b, int64_t k, int64_t ldb, int* regular,
+                 void* stream) {
+    if (n < 0) return -1;
+    if (n > 0 && lda < n) return -3;
+    if (k < 0) return -6;
+    if (n > 0 && k > 0 && ldb < k) return -7;
+    if (!regular) return -8;
+    return dev_gesv<double>(n, a, lda, perm, b, k, ldb, regular, (hipStream_t)stream);
+}
+int hs_dev_sgesv(int64_t n, float* a, int64_t lda, int64_t* perm, float* b, int64_t k, int64_t ldb, int* regular,
+                 void* stream) {
+    if (n < 0) return -1;
+    if (n > 0 && lda < n) return -3;
+    if (k < 0) return -6;
+    if (n > 0 && k > 0 && ldb < k) return -7;
+    if (!regular) return -8;
+    return dev_gesv<float>(n, a, lda, perm, b, k, ldb, regular, (hipStream_t)stream);
 }
 int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_t ldf, const int64_t* perm, double* b, int64_t ldb,
                   void* stream) {
     if (n < 0) return -2;
     if (k < 0) return -3;
+    if (n > 0 && ldf < n) return -5;
+    if (n > 0 && k > 0 && ldb < k) return -8;
     hipStream_t s = (hipStream_t)stream;
     return getrs<double>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
 }
@@ -556,6 +697,8 @@ int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t
                   void* stream) {
     if (n < 0) return -2;
     if (k < 0) return -3;
+    if (n > 0 && ldf < n) return -5;
+    if (n > 0 && k > 0 && ldb < k) return -8;
     hipStream_t s = (hipStream_t)stream;
     return getrs<float>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
 }

@@ -6,13 +6,16 @@
 #include <cstddef>
 #include <string>
 #include <type_traits>
+#include <mutex>
 
 namespace hs {
 
 // Status codes of every entry point: 0 = ok, -i = argument i is invalid (LAPACK style),
-// > 0 = a HIP runtime failure (the hipError_t value), HS_ERR_LIMIT = a device-side bounded wait ran out.
+// > 0 = a HIP runtime failure (the hipError_t value), ERR_LIMIT = a device-side bounded wait ran out,
+// ERR_UNSUPPORTED = valid arguments beyond what the library handles.
 constexpr int OK = 0;
 constexpr int ERR_LIMIT = 100000;
+constexpr int ERR_UNSUPPORTED = 100001;  // valid arguments that this library does not handle (the message says which)
 
 void set_error(const std::string& what);
 const char* last_error();
@@ -29,6 +32,11 @@ void pool_release_all();
 
 // Number of compute units of the current device (for grid sizing of persistent kernels).
 int device_cus();
+
+// Held by every launch sequence whose workgroups wait on one another and so must all be resident at once (the LU's
+// panel launches, the single-column solves' runs): one such sequence at a time in the process, so that two of them on
+// different streams never each hold part of the CUs the other needs.
+std::mutex& resident_mutex();
 
 int hip_status(hipError_t e, const char* where);
 

@@ -23,7 +23,8 @@ int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t ld
 // LU with partial pivoting of the row-major n×n matrix `a` (ld `lda`) in place: L strictly below the
 // diagonal (unit diagonal implicit), U on and above; pivots[c] = the row swapped with row c (absolute,
 // 0-based, LAPACK ipiv style). The arithmetic is rustymath's lu_factor (core/src/linalg/factor.rs:617),
-// item for item. `info` (device, may be null) gets 0 or 1 + the first zero pivot.
+// item for item. A zero pivot leaves its column undivided (factor.rs:606); the C ABI's `info` reports the first one
+// from the factors' diagonal afterwards. Orders above 16384 give ERR_UNSUPPORTED.
 // bf16_trailing (FP32 only): the trailing products after each 256-column block take their operands rounded to
 // bfloat16 (gemm_bf16_update); panels, row moves and the rows of U stay FP32. Not the reference's arithmetic: the
 // low-precision factors of the BF16 mixed solve.
@@ -36,6 +37,18 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
 template <typename T>
 int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int64_t* pivots, T* b, int64_t ldb,
           hipStream_t s);
+
+// `Tiled::triangular` (core/src/linalg.rs:781-787) with rustymath's native orders (`triangular_native`,
+// core/src/linalg/factor.rs:711-721): solves op(a) x = b (right = false, a m×m) or x op(a) = b (right = true, a n×n)
+// in place on x (m×n, ld ldx). The stored triangle is the lower one (stored_lower) or the upper one; ta reads a
+// transposed; unit takes the diagonal as ones. `contiguous` says whether the caller's x is one column with ldx = 1,
+// which takes `single`'s orders (x may be a compact device copy of a strided column). Synchronises `s`.
+template <typename T>
+int trsm(bool right, bool stored_lower, bool ta, bool unit, int64_t m, int64_t n, const T* a, int64_t lda, T* x,
+         int64_t ldx, bool contiguous, hipStream_t s);
+// `single` (factor.rs:725-748) for a triangle read by columns (entry (i, j) = f[j ld + i]), one contiguous column x.
+template <typename T>
+int trsv1_columns(bool lower, bool unit, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s);
 
 // The inverse from factors and perm as getrf leaves them (`lu_inverse`, factor.rs:645-664): out (n×n, row-major, ld
 // ldo) = U⁻¹ L⁻¹ with its columns permuted back, in the reference's orders of operations.

@@ -248,8 +248,11 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                                                    Exchange* __restrict__ ex, unsigned tag0, int nwg, int* status,
                                                    long long* prof, TrailJob<T> job) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<T>()];
+    // After a wait that ran out in an earlier launch the factorization is void: the launch returns rather than wait
+    // again. The panel's workgroups test the word once their rows are in (its load then adds no latency of its own).
+    const int void_run = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (blockIdx.x & 7) {  // outside XCD 0: a worker of the trailing product, if there is one
-        if (job.C) trail_work<T>(job, tag0, *reinterpret_cast<WorkLds<T>*>(lds));
+        if (job.C && !void_run) trail_work<T>(job, tag0, *reinterpret_cast<WorkLds<T>*>(lds));
         return;
     }
     const int g = (int)(blockIdx.x >> 3);
@@ -305,6 +308,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         __syncthreads();
     }
 
+    if (void_run) return;  // the same word for every workgroup of the launch, so all of them return
     if (tid == 0) s_abort = 0;
     if (tid < PW) s_skip[tid] = tid < w ? leaf_end[c0 + tid] - (int)c0 - tid : 0;  // a zero pivot's skip, by column
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
@@ -792,7 +796,14 @@ int trail_profile(int on, double* ms, int64_t* launches, double* flops, double* 
 template <typename T>
 int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf16_trailing) {
     if (n == 0) return OK;
-    if (n > (int64_t)MAX_WG * PT) { set_error("n above 16384: the panel kernel holds at most 32 x 512 rows"); return -1; }
+    if (n > (int64_t)MAX_WG * PT) {
+        set_error("LU of order above 16384 is not supported: the panel kernel holds at most 32 x 512 rows");
+        return ERR_UNSUPPORTED;
+    }
+    // The panel launches' workgroups wait on one another: one factorization with such panels at a time (see
+    // resident_mutex). Every path below synchronises the stream before it returns.
+    std::unique_lock<std::mutex> resident(resident_mutex(), std::defer_lock);
+    if (n > PT) resident.lock();
     PanelProfile* st;
     int e = panel_prof(st);
     if (e) return e;

@@ -17,6 +17,7 @@
 #include "hs_kernels.h"
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 
 namespace hs {
 
@@ -26,11 +27,13 @@ constexpr int BR = 32;  // NS of factor.rs:82: the halving's leaves have at most
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
 
 enum Mode : int {
-    FWD_UNIT = 0,    // L (unit lower of F): increasing order, no division
-    BWD_SINGLE = 1,  // U, one column: decreasing order, then a quotient
-    BWD_LEAVES = 2,  // U, several columns: the halving order, then a reciprocal multiply
-    FWD_DIAG = 3,    // Uᵀ (lower, diagonal of U): increasing order, then a reciprocal multiply
-    BWD_UNIT = 4,    // Lᵀ (upper, unit): decreasing order, no division
+    FWD_UNIT = 0,     // L (unit lower of F): increasing order, no division
+    BWD_SINGLE = 1,   // U, one column: decreasing order, then a quotient
+    BWD_LEAVES = 2,   // U, several columns: the halving order, then a reciprocal multiply
+    FWD_DIAG = 3,     // Uᵀ (lower, diagonal of U): increasing order, then a reciprocal multiply
+    BWD_UNIT = 4,     // Lᵀ (upper, unit): decreasing order, no division
+    FWD_QUOT = 5,     // a lower triangle read by columns, one column: increasing order, then a quotient (`single`)
+    BWD_COL_UNIT = 6, // a unit upper triangle read by columns, one column: decreasing order, no division (`single`)
 };
 
 
@@ -111,7 +114,8 @@ constexpr int RUN1 = 16;  // consecutive blocks (in solve order) on one XCD
 // tiles of the matrix loaded one producer ahead, then substitutes within itself and publishes. Orders: FWD_UNIT (L,
 // increasing), BWD_SINGLE (U, decreasing, then a quotient: `single`, factor.rs:725), and for the transposed solve
 // FWD_DIAG (Uᵀ, increasing, then `divide`'s reciprocal multiply) and BWD_UNIT (Lᵀ, decreasing). The transposed modes
-// read rows of F where the others read columns.
+// read rows of F where the others read columns. FWD_QUOT and BWD_COL_UNIT are `single`'s two other cases of a
+// triangle read by columns (factor.rs:730-741), for the triangular-solve entry points.
 //
 // Launch index w runs on XCD w % 8, and takes solve-order block u, so that runs of `run` consecutive blocks share an
 // XCD: a block publishes its rows twice, to `loc` with plain stores that the next blocks of its run read from the
@@ -125,7 +129,9 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     const int u = ((slot / run) * 8 + xcd) * run + slot % run;  // solve order; a bijection of the padded grid
     if (u >= nb) return;
-    const bool fwd = mode == FWD_UNIT || mode == FWD_DIAG, rowwise = mode == FWD_DIAG || mode == BWD_UNIT;
+    // After a wait that ran out (here or in the first pass) the result is void: return at once rather than wait again.
+    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const bool fwd = mode == FWD_UNIT || mode == FWD_DIAG || mode == FWD_QUOT, rowwise = mode == FWD_DIAG || mode == BWD_UNIT;
     const int q = fwd ? u : nb - 1 - u;  // this block's rows [q B1, ...)
     const int64_t r0 = (int64_t)q * B1;
     const int h = (int)(n - r0 < B1 ? n - r0 : B1);
@@ -228,6 +234,7 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
         for (int j = 0; j < B1; ++j) {
             if (j < h) {
                 if (mode == FWD_DIAG && lane == j) acc = fabs(dg) >= tiny ? acc * rdg : acc / dg;
+                if (mode == FWD_QUOT && lane == j) acc = acc / dg;
                 const T xj = lane_value(acc, j);
                 if (lane > j) acc = __builtin_elementwise_fma(xj, -d[j], acc);
             }
@@ -253,10 +260,21 @@ __global__ __launch_bounds__(64) void trsv1_kernel(int mode, int64_t n, const T*
 // match (the published words are also cleared before each pass).
 std::atomic<unsigned long long> g_epoch1{1};
 
-// L then U (lu_solve's order for one column) in place on x (n items, contiguous), or Uᵀ then Lᵀ (trans), with one
-// status check.
+__global__ void zero_words(unsigned long long* p, int64_t count) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+// Bytes of the single-column passes' workspace for order n: the published words twice (agent scope, and the XCD's
+// L2) and the status word.
 template <typename T>
-int trsv1_lu(bool trans, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
+size_t trsv1_bytes(int64_t n) { return 2 * (size_t)n * (sizeof(T) == 8 ? 2 : 1) * 8 + 64; }
+
+// L then U (lu_solve's order for one column) in place on x (n items, contiguous), or Uᵀ then Lᵀ (trans), with one
+// status check. `buf` (trsv1_bytes(n) bytes of device memory) is the workspace, or null for one from the pool. The
+// words are cleared by a fill kernel, so the sequence holds no memset node when a caller captures it.
+template <typename T>
+int trsv1_lu(bool trans, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s, char* buf = nullptr) {
     const int nb = (int)((n + B1 - 1) / B1);
     // Runs of RUN1 blocks per XCD when the padded grid fits on the GPU at once (one wave per block, at least two per
     // CU at this kernel's register use), else the launch order, in which no block waits on one launched after it.
@@ -264,12 +282,18 @@ int trsv1_lu(bool trans, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s)
     const int run = padded <= 2 * device_cus() ? RUN1 : 1;
     const int grid = run > 1 ? padded : (nb + 7) / 8 * 8;
     const size_t words = (size_t)n * (sizeof(T) == 8 ? 2 : 1);
-    char* buf = (char*)pool_get(2 * words * 8 + 64);
-    if (!buf) return hip_status(hipErrorOutOfMemory, "pool_get");
+    const bool pooled = buf == nullptr;
+    if (pooled && !(buf = (char*)pool_get(trsv1_bytes<T>(n)))) return hip_status(hipErrorOutOfMemory, "pool_get");
     unsigned long long* pub = (unsigned long long*)buf;
     unsigned long long* loc = pub + words;
     int* status = (int*)(buf + 2 * words * 8);
-    HS_HIP(hipMemsetAsync(buf, 0, 2 * words * 8 + sizeof(int), s));
+    // With runs, a block waits on blocks that must be resident beside it: one such solve at a time in this process
+    // (a concurrent one could hold the CUs its blocks need).
+    std::unique_lock<std::mutex> lk(resident_mutex(), std::defer_lock);
+    if (run > 1) lk.lock();
+    zero_words<<<(unsigned)std::min<int64_t>((2 * words + 8 + 255) / 256, 1024), 256, 0, s>>>(
+        (unsigned long long*)buf, (int64_t)(2 * words + 8));
+    HS_HIP(hipGetLastError());
     const unsigned long long e1 = (g_epoch1.fetch_add(2) % 0xfffffff0ull) | 1ull;  // odd, and e1 + 1 < 2^32
     trsv1_kernel<T><<<grid, 64, 0, s>>>(trans ? FWD_DIAG : FWD_UNIT, n, f, ld, x, pub, loc, run, e1, status);
     HS_HIP(hipGetLastError());
@@ -278,7 +302,7 @@ int trsv1_lu(bool trans, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s)
     int st = 0;
     HS_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
     HS_HIP(hipStreamSynchronize(s));
-    pool_put(buf);
+    if (pooled) pool_put(buf);
     if (st) { set_error("triangular solve wait ran out"); return ERR_LIMIT; }
     return OK;
 }
@@ -638,18 +662,55 @@ int bwd_halving(int64_t r0, int64_t r1, int64_t n, int64_t k, const T* lu, int64
 
 }  // namespace
 
+// `single` (factor.rs:725-748) for a triangle that a transposed view reads by columns: entry (i, j) is f[j ld + i], one
+// contiguous column x of n items, in place. lower: increasing order, then (unless unit) a quotient by the diagonal;
+// otherwise decreasing order, then (unless unit) a quotient.
 template <typename T>
-int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int64_t* perm, T* b, int64_t ldb, hipStream_t s) {
+int trsv1_columns(bool lower, bool unit, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s) {
+    if (n == 0) return OK;
+    const int nb = (int)((n + B1 - 1) / B1);
+    const int padded = (nb + 8 * RUN1 - 1) / (8 * RUN1) * (8 * RUN1);
+    const int run = padded <= 2 * device_cus() ? RUN1 : 1;
+    const int grid = run > 1 ? padded : (nb + 7) / 8 * 8;
+    const size_t words = (size_t)n * (sizeof(T) == 8 ? 2 : 1);
+    char* buf = (char*)pool_get(trsv1_bytes<T>(n));
+    if (!buf) return hip_status(hipErrorOutOfMemory, "pool_get");
+    unsigned long long* pub = (unsigned long long*)buf;
+    int* status = (int*)(buf + 2 * words * 8);
+    std::unique_lock<std::mutex> lk(resident_mutex(), std::defer_lock);
+    if (run > 1) lk.lock();
+    zero_words<<<(unsigned)std::min<int64_t>((2 * words + 8 + 255) / 256, 1024), 256, 0, s>>>(pub, (int64_t)(2 * words + 8));
+    const int mode = lower ? (unit ? FWD_UNIT : FWD_QUOT) : (unit ? BWD_COL_UNIT : BWD_SINGLE);
+    const unsigned long long e1 = (g_epoch1.fetch_add(2) % 0xfffffff0ull) | 1ull;
+    trsv1_kernel<T><<<grid, 64, 0, s>>>(mode, n, f, ld, x, pub, pub + words, run, e1, status);
+    HS_HIP(hipGetLastError());
+    int st = 0;
+    HS_HIP(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HS_HIP(hipStreamSynchronize(s));
+    pool_put(buf);
+    if (st) { set_error("triangular solve wait ran out"); return ERR_LIMIT; }
+    return OK;
+}
+template int trsv1_columns<double>(bool, bool, int64_t, const double*, int64_t, double*, hipStream_t);
+template int trsv1_columns<float>(bool, bool, int64_t, const float*, int64_t, float*, hipStream_t);
+
+
+namespace {
+// getrs with its workspace: w (n k items) holds the right-hand sides in solve order, buf (trsv1_bytes(n) bytes) the
+// single-column passes' words; null pointers take buffers from the pool.
+template <typename T>
+int getrs_ws(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int64_t* perm, T* b, int64_t ldb,
+             hipStream_t s, T* w, char* buf) {
     if (n == 0 || k == 0) return OK;
-    T* w = (T*)pool_get(sizeof(T) * (size_t)(n * k));
-    if (!w) return hip_status(hipErrorOutOfMemory, "pool_get");
+    const bool pooled = w == nullptr;
+    if (pooled && !(w = (T*)pool_get(sizeof(T) * (size_t)(n * k)))) return hip_status(hipErrorOutOfMemory, "pool_get");
     int rc = OK;
     const int64_t total = n * k;
     if (!trans) {
         gather_rows<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, (int)k, b, ldb, perm, w, k);
         HS_HIP(hipGetLastError());
         if (k == 1) {
-            rc = trsv1_lu<T>(false, n, lu, lda, w, s);
+            rc = trsv1_lu<T>(false, n, lu, lda, w, s, buf);
         } else {
             rc = fwd_blocked<T>(FWD_UNIT, n, k, lu, lda, w, s);
             if (rc == OK) rc = bwd_halving<T>(0, n, n, k, lu, lda, w, s);
@@ -660,7 +721,7 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
     } else {
         HS_HIP(hipMemcpy2DAsync(w, k * sizeof(T), b, ldb * sizeof(T), k * sizeof(T), n, hipMemcpyDeviceToDevice, s));
         if (k == 1) {
-            rc = trsv1_lu<T>(true, n, lu, lda, w, s);
+            rc = trsv1_lu<T>(true, n, lu, lda, w, s, buf);
         } else {
             rc = fwd_blocked<T>(FWD_DIAG, n, k, lu, lda, w, s);
             if (rc == OK) rc = bwd_unit_blocked<T>(n, k, lu, lda, w, s);
@@ -671,8 +732,14 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
         }
     }
     HS_HIP(hipStreamSynchronize(s));
-    pool_put(w);
+    if (pooled) pool_put(w);
     return rc;
+}
+}  // namespace
+
+template <typename T>
+int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int64_t* perm, T* b, int64_t ldb, hipStream_t s) {
+    return getrs_ws<T>(trans, n, k, lu, lda, perm, b, ldb, s, nullptr, nullptr);
 }
 
 namespace {

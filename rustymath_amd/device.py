@@ -41,16 +41,36 @@ def gemm(a, b, c=None, alpha=1.0, beta=0.0, trans_a=False, trans_b=False):
     return c
 
 
-def getrf(a):
+def getrf(a, info=False):
     """Factors the square row-major tensor `a` in place into rustymath's column-major packed factors and returns
-    (a, perm) with perm an int64 GPU tensor (core/src/linalg/factor.rs:617-630)."""
+    (a, perm) with perm an int64 GPU tensor (core/src/linalg/factor.rs:617-630); with `info`, (a, perm, info) where
+    info is 0 or 1 + the first column whose pivot is exactly zero."""
+    import ctypes
     _mat(a, 'a')
     n = a.shape[0]
     if a.shape[1] != n: raise ValueError('getrf needs a square matrix')
     perm = torch.empty(n, dtype=torch.int64, device=a.device)
+    first_zero = ctypes.c_int64(0)
     f = getattr(_lib.load(), f'hs_dev_{_sfx(a.dtype)}getrf')
-    _lib.check(f(n, a.data_ptr(), a.stride(0), perm.data_ptr(), _stream()), 'getrf')
-    return a, perm
+    _lib.check(f(n, a.data_ptr(), a.stride(0), perm.data_ptr(), ctypes.byref(first_zero), _stream()), 'getrf')
+    return (a, perm, first_zero.value) if info else (a, perm)
+
+
+def gesv(a, b):
+    """`LuWork::solve` (core/src/linalg/factor.rs:207-226) in place on device tensors: factors the square row-major `a`
+    and, where every pivot exceeds eps·n·max|a|, solves the row-major b (n×k) in place. Returns (a, perm, regular);
+    b is untouched where regular is False (the reference would take least squares there)."""
+    import ctypes
+    _mat(a, 'a'); _mat(b, 'b')
+    n = a.shape[0]
+    if a.shape[1] != n or b.shape[0] != n: raise ValueError('gesv needs a square a and b with as many rows')
+    if a.dtype != b.dtype: raise TypeError('a and b must have one dtype')
+    perm = torch.empty(n, dtype=torch.int64, device=a.device)
+    reg = ctypes.c_int(0)
+    f = getattr(_lib.load(), f'hs_dev_{_sfx(a.dtype)}gesv')
+    _lib.check(f(n, a.data_ptr(), a.stride(0), perm.data_ptr(), b.data_ptr(), b.shape[1], max(b.stride(0), 1),
+                 ctypes.byref(reg), _stream()), 'gesv')
+    return a, perm, bool(reg.value) or n == 0
 
 
 def getrs(factors, perm, b, trans=False):

@@ -125,7 +125,7 @@ def _factor(a, name):
     else:
         f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_factor')
         for i in range(cells.shape[0]):
-            _lib.check(f(_ptr(cells[i]), n, _ptr(factors[i]), _ptr(perm[i]), None), name)
+            _lib.check(f(_ptr(cells[i]), n, _ptr(factors[i]), _ptr(perm[i]), None, None), name)
     return factors.reshape(frame + (n, n)), perm.reshape(frame + (n,)), store
 
 

@@ -54,3 +54,32 @@ def backward_error(a, x, b):
     a, x, b = (np.asarray(v, np.float64) for v in (a, x, b))
     r = b - a @ x
     return norm(r) / max(norm(a) * norm(x), np.finfo(np.float64).tiny)
+
+
+# The shapes of the triangular-solve checks, (side, m, n, ldx): left solves that halve with NS = 32 (n < m) and with
+# LEAF = 16 (n >= m) leaves, one contiguous column (`single`), one strided column (the halving with `by_vector`
+# products), and right-side solves, whose transposed copy is one contiguous column when m = 1.
+TRSM_SHAPES = [('L', 70, 3, None), ('L', 40, 45, None), ('L', 70, 1, None), ('L', 70, 1, 4), ('L', 5, 2, None),
+               ('R', 3, 70, None), ('R', 1, 40, None), ('R', 45, 40, 50), ('L', 1, 1, None), ('L', 33, 1, None)]
+TRSM_CODES = [(u, t, d) for u in 'UL' for t in 'NT' for d in 'NU']
+
+
+def triangle_system(side, m, n, uplo, trans, diag, dtype, seed):
+    """(a, x_true, b) with op(a) x_true = b (side 'L') or x_true op(a) = b ('R') holding exactly: whole numbers from -2
+    to 2, diagonal items ±1, ±2 or ±4 (ignored for a unit diagonal, which holds 7s instead), and NaN outside the
+    triangle that `uplo` names, which a solve must never read. Every partial sum is a whole number below 2^10 and every
+    division is by a power of two, so the solve is exact in any order of its terms."""
+    rng = np.random.default_rng(seed)
+    order = m if side == 'L' else n
+    t = rng.integers(-2, 3, size=(order, order)).astype(np.float64)
+    d = rng.choice([-4.0, -2.0, -1.0, 1.0, 2.0, 4.0], size=order)
+    tri = np.tril(t, -1) if uplo == 'L' else np.triu(t, 1)
+    logical_diag = np.ones(order) if diag == 'U' else d
+    full = tri + np.diag(logical_diag)
+    strict = np.tril(np.ones((order, order), bool), -1) if uplo == 'L' else np.triu(np.ones((order, order), bool), 1)
+    a = np.where(strict, tri, np.nan)
+    a[np.diag_indices(order)] = 7.0 if diag == 'U' else d
+    op = full.T if trans != 'N' else full
+    x = rng.integers(-2, 3, size=(m, n)).astype(np.float64)
+    b = op @ x if side == 'L' else x @ op
+    return a.astype(dtype), x.astype(dtype), b.astype(dtype)

@@ -115,3 +115,21 @@ def test_matches_the_fp64_cpu_gradient(adj, n, batch):
     want = cpu_gradients(a, c, d, w)
     for got, ref in zip((x.grad for x in xs), want):
         assert rel(got, ref) <= 1e-8, rel(got, ref)
+
+
+def test_singular_systems_raise_for_one_matrix_and_for_a_stack(adj):
+    """A matrix with a negligible pivot is where rustymath's `solve` takes least squares (core/src/linalg/factor.rs:208),
+    which this library leaves out: the adjoint's forward pass raises NotImplementedError for it, whether it comes alone
+    (the blocked factorization) or in a stack (the batched kernel)."""
+    import torch
+    a = torch.ones(64, 64, dtype=torch.float64, device='cuda')
+    b = torch.ones(64, 2, dtype=torch.float64, device='cuda')
+    with pytest.raises(NotImplementedError):
+        adj.solve(a, b)
+    a3 = torch.eye(40, dtype=torch.float64, device='cuda').repeat(3, 1, 1)
+    a3[1] = 1.0
+    with pytest.raises(NotImplementedError):
+        adj.solve(a3, torch.ones(3, 40, 2, dtype=torch.float64, device='cuda'))
+    big = torch.ones(600, 600, dtype=torch.float64, device='cuda')
+    with pytest.raises(NotImplementedError):
+        adj.solve(big.unsqueeze(0).repeat(2, 1, 1), torch.ones(2, 600, 1, dtype=torch.float64, device='cuda'))

@@ -31,7 +31,7 @@ def test_bad_arguments_are_rejected_before_any_device_work():
     assert lib.hs_dgemm(111, 111, -1, 2, 2, 1.0, None, 2, None, 2, 0.0, None, 2) == -3
     assert lib.hs_dgemm(111, 111, 4, 4, 4, 1.0, None, 3, None, 4, 0.0, None, 4) == -8
     assert lib.hs_dgemm(111, 111, 0, 4, 4, 1.0, None, 4, None, 4, 0.0, None, 4) == 0
-    assert lib.hs_dlu_factor(None, -1, None, None, None) == -2
+    assert lib.hs_dlu_factor(None, -1, None, None, None, None) == -2
     bad = (ctypes.c_int64 * 3)(0, 3, 1)
     assert lib.hs_dlu_inverse(None, None, -1, None) == -3
     assert lib.hs_dlu_inverse(None, bad, 3, None) == -2
@@ -99,3 +99,19 @@ def test_mixed_precision_solve_rejects_bad_arguments_on_the_host():
     assert lib.hs_dev_dbgesv(4, None, 4, None, None, 10, 0, 0.0, None, None, None, None) == -7
     assert lib.hs_dev_sgemm_bf16_update(-1, 1, 1, None, 1, None, 1, None, 1, None) == -1
     assert lib.hs_dev_sgemm_bf16_update(2, 3, 4, None, 3, None, 3, None, 3, None) == -5
+
+
+def test_triangular_solve_rejects_bad_codes_on_the_host():
+    from rustymath_amd import _lib
+    lib = _lib.load()
+    assert lib.hs_dtrsm(140, 121, 111, 131, 2, 2, None, 2, None, 2) == -1
+    assert lib.hs_dtrsm(141, 120, 111, 131, 2, 2, None, 2, None, 2) == -2
+    assert lib.hs_dtrsm(141, 121, 110, 131, 2, 2, None, 2, None, 2) == -3
+    assert lib.hs_strsm(141, 121, 111, 133, 2, 2, None, 2, None, 2) == -4
+    assert lib.hs_dtrsm(141, 121, 111, 131, -1, 2, None, 2, None, 2) == -5
+    assert lib.hs_dtrsm(142, 121, 111, 131, 2, -1, None, 2, None, 2) == -6
+    assert lib.hs_dtrsm(142, 121, 111, 131, 2, 5, None, 4, None, 5) == -8   # right side: a is n×n
+    assert lib.hs_dtrsm(141, 121, 111, 131, 5, 3, None, 5, None, 2) == -10
+    assert lib.hs_dtrsm(141, 121, 111, 131, 0, 3, None, 0, None, 3) == 0
+    assert lib.hs_dev_dgetrs(0, 4, 1, None, 3, None, None, 1, None) == -5
+    assert lib.hs_dev_sgetrs(0, 4, 2, None, 4, None, None, 1, None) == -8

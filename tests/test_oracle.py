@@ -12,7 +12,8 @@ import pytest
 import scipy.linalg
 
 import oracle
-from support import FOLD_SIZES, KC, MC, integer_operands, random, norm, distance, backward_error
+from support import (FOLD_SIZES, KC, MC, TRSM_CODES, TRSM_SHAPES, integer_operands, random, norm, distance,
+                     backward_error, triangle_system)
 
 EPS = {np.float64: np.finfo(np.float64).eps, np.float32: np.finfo(np.float32).eps}
 
@@ -344,3 +345,75 @@ def test_float_products_round_once():
     x = oracle.lu_solve(fac, perm, b)
     assert (perm == [0, 1]).all() and x[1, 0] == f(1 + 2.0 ** -11 + 2.0 ** -23)
     assert x[1, 0] != f(np.float64(b[0, 0]) * np.float64(0.5 + 2.0 ** -13) + np.float64(b[1, 0]))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('shape', TRSM_SHAPES)
+def test_triangular_solves_are_exact_on_whole_numbers(dtype, shape):
+    """`Tiled::triangular` (core/src/linalg.rs:781) for every code: side, stored triangle, transpose and unit diagonal.
+    On whole-number systems every order of the terms is exact, so the solution is the exact one; a dropped term, a
+    transposed view, the wrong triangle or a diagonal that should count as one all change it. NaN outside the named
+    triangle (and 7s on a unit diagonal) must never be read."""
+    side, m, n, ldx = shape
+    for k, (uplo, trans, diag) in enumerate(TRSM_CODES):
+        a, x, b = triangle_system(side, m, n, uplo, trans, diag, dtype, seed=k + 10 * m + n)
+        got = oracle.trsm(a, b, side, uplo, trans, diag, ldx=ldx)
+        assert np.array_equal(got, x), (side, m, n, ldx, uplo, trans, diag)
+
+
+def test_single_row_solve_takes_dot_with_eight_running_sums():
+    """`single` for a triangle read by rows (factor.rs:745-746) takes each item's `dot` (`sum_of_products`,
+    factor.rs:466-473): eight running sums over whole chunks of eight, folded in order from zero, then the remaining
+    terms one fused multiply-add each. A known answer that an in-order sum would miss: row 16 of a unit lower triangle
+    holds 1 at column 0, -2^53 at column 1 and 2^53 at column 8, zeros elsewhere, with x = 1 above it. Sum 0 takes
+    1 then 2^53 (2^53 + 1 rounds to even, 2^53) and sum 1 takes -2^53, so the dot is 0 and x_16 = b_16 - 0 = 0; in
+    order, 1 - 2^53 is exact and adding 2^53 gives 1."""
+    m = 17
+    a = np.eye(m)
+    a[16, 0], a[16, 1], a[16, 8] = 1.0, -2.0 ** 53, 2.0 ** 53
+    b = np.ones((m, 1)); b[16, 0] = 0.0
+    x = oracle.trsm(a, b, 'L', 'L', 'N', 'U')
+    assert np.array_equal(x[:16, 0], np.ones(16))
+    assert x[16, 0] == 0.0
+    # The other reading order of the same triangle (columns, through `axpy`) takes the terms in order: 0 - 1 + 2^53 ...
+    xt = oracle.trsm(np.ascontiguousarray(a.T), b, 'L', 'U', 'T', 'U')
+    assert xt[16, 0] == -1.0
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_triangular_solves_reproduce_lu_solve(dtype):
+    """lu_solve (factor.rs:633-641) is the two triangular solves of its factors' transposed view: the stored unit upper
+    triangle (L) then the stored lower one (U), both with code 112. The general entry reproduces it bit for bit."""
+    rng = np.random.default_rng(21)
+    for n, k in ((50, 1), (70, 5), (33, 40)):
+        a = rng.standard_normal((n, n)).astype(dtype)
+        b = rng.standard_normal((n, k)).astype(dtype)
+        f, perm, _ = oracle.lu(a)
+        y = oracle.trsm(f, b[perm], 'L', 'U', 'T', 'U')
+        x = oracle.trsm(f, y, 'L', 'L', 'T', 'N')
+        assert bits_equal(x, oracle.lu_solve(f, perm, b))
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_triangular_solves_within_the_rounding_bound(dtype):
+    """Random well-conditioned triangles (diagonal items 2 to 3 in magnitude): the residual |op(A) X - B| stays within
+    the substitution's rounding bound against SciPy's solve_triangular on every code and shape."""
+    rng = np.random.default_rng(4)
+    eps = np.finfo(dtype).eps
+    for side, m, n, ldx in TRSM_SHAPES:
+        order = m if side == 'L' else n
+        for uplo, trans, diag in TRSM_CODES:
+            a = (rng.standard_normal((order, order)) / np.sqrt(order)).astype(dtype)
+            a[np.diag_indices(order)] = rng.uniform(2, 3, order) * rng.choice([-1, 1], order)
+            b = rng.standard_normal((m, n)).astype(dtype)
+            x = oracle.trsm(a, b, side, uplo, trans, diag, ldx=ldx).astype(np.float64)
+            t = np.tril(a, -1) if uplo == 'L' else np.triu(a, 1)
+            t = (t + (np.eye(order) if diag == 'U' else np.diag(np.diag(a)))).astype(np.float64)
+            op = t.T if trans != 'N' else t
+            lower = (uplo == 'L') == (trans == 'N')  # the triangle op(A) has nonzeros in
+            if side == 'L': want = scipy.linalg.solve_triangular(op, b.astype(np.float64), lower=lower)
+            else: want = scipy.linalg.solve_triangular(op.T, b.T.astype(np.float64), lower=not lower).T
+            r = (op @ x - b) if side == 'L' else (x @ op - b)
+            scale = (np.abs(op) @ np.abs(x)) if side == 'L' else (np.abs(x) @ np.abs(op))
+            assert (np.abs(r) <= 4 * order * eps * scale + 1e-30).all()
+            assert np.abs(x - want).max() <= 100 * order * eps * max(np.abs(want).max(), 1)

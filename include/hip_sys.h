@@ -145,6 +145,14 @@ int hs_dev_dgetrs(int trans, int64_t n, int64_t k, const double* factors, int64_
                   int64_t ldb, void* stream);
 int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t ldf, const int64_t* perm, float* b,
                   int64_t ldb, void* stream);
+/* hs_dev_dgetrs that allocates nothing (no counterpart in the reference): for callers that cannot allocate device
+ * memory, such as code under HIP graph capture or a serving loop that forbids it. The same arguments, results and
+ * bits as hs_dev_dgetrs; its workspace is a static array of the library in each device's memory (4 MiB + 256 B, held
+ * from a device's first call until the process ends), which limits it to n <= 65536 (else -2), k <= 16 and n·k <=
+ * 2^18 (else -3). Every call on one device shares that device's array: calls that overlap on one device race, so a
+ * caller serialises them. Synchronises `stream` before returning. */
+int hs_dev_dgetrs_noalloc(int trans, int64_t n, int64_t k, const double* factors, int64_t ldf, const int64_t* perm,
+                          double* b, int64_t ldb, void* stream);
 /* The inverse from device factors (ld ldf) and perm into out (row-major, ld ldo), as hs_dlu_inverse computes it. */
 int hs_dev_dgetri(int64_t n, const double* factors, int64_t ldf, const int64_t* perm, double* out, int64_t ldo,
                   void* stream);

@@ -51,6 +51,7 @@ SIGNATURES = {
     'hs_dev_sgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]),
     'hs_dev_dgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_sgetrs': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    'hs_dev_dgetrs_noalloc': (_i32, [_i32, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_dgetri': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_sgetri': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_dsgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i32, _d, _vp, _vp, _vp]),

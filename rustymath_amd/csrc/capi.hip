@@ -702,6 +702,14 @@ int hs_dev_sgetrs(int trans, int64_t n, int64_t k, const float* factors, int64_t
     hipStream_t s = (hipStream_t)stream;
     return getrs<float>(trans != 0, n, k, factors, ldf, perm, b, ldb, s);
 }
+int hs_dev_dgetrs_noalloc(int trans, int64_t n, int64_t k, const double* factors, int64_t ldf, const int64_t* perm,
+                          double* b, int64_t ldb, void* stream) {
+    if (n < 0) return -2;
+    if (k < 0) return -3;
+    if (n > 0 && ldf < n) return -5;
+    if (n > 0 && k > 0 && ldb < k) return -8;
+    return getrs_noalloc(trans != 0, n, k, factors, ldf, perm, b, ldb, (hipStream_t)stream);
+}
 int hs_dev_dgetri(int64_t n, const double* factors, int64_t ldf, const int64_t* perm, double* out, int64_t ldo,
                   void* stream) {
     if (n < 0) return -1;

@@ -50,6 +50,11 @@ int trsm(bool right, bool stored_lower, bool ta, bool unit, int64_t m, int64_t n
 template <typename T>
 int trsv1_columns(bool lower, bool unit, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s);
 
+// getrs for FP64 with no allocation: its workspace is a static device array of the library (4 MiB and 256 bytes per
+// device), so n <= 65536, k <= 16 and n k <= 2^18; calls that overlap on one device race on that array.
+int getrs_noalloc(bool trans, int64_t n, int64_t k, const double* lu, int64_t lda, const int64_t* perm, double* b,
+                  int64_t ldb, hipStream_t s);
+
 // The inverse from factors and perm as getrf leaves them (`lu_inverse`, factor.rs:645-664): out (n×n, row-major, ld
 // ldo) = U⁻¹ L⁻¹ with its columns permuted back, in the reference's orders of operations.
 template <typename T>

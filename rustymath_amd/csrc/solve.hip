@@ -743,6 +743,50 @@ int getrs(bool trans, int64_t n, int64_t k, const T* lu, int64_t lda, const int6
 }
 
 namespace {
+// The workspace of getrs_noalloc, in the library's own device memory: the right-hand sides in solve order (at most
+// NOALLOC_ITEMS FP64 items) and the single-column passes' words (order at most NOALLOC_N), 4 MiB and 256 bytes in all.
+// Every device that runs getrs_noalloc holds its own copy from its first call until the process ends.
+constexpr int64_t NOALLOC_N = 65536, NOALLOC_K = 16, NOALLOC_ITEMS = int64_t(1) << 18;
+constexpr size_t NOALLOC_W = (size_t)NOALLOC_ITEMS * sizeof(double);
+constexpr size_t NOALLOC_BYTES = NOALLOC_W + 2 * (size_t)NOALLOC_N * 2 * 8 + 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// The address of each device's copy of `scratch`: host code names the array only through these (the array's own name
+// in host code is a host-side stand-in, not the device's memory).
+std::mutex g_scratch_mu;
+void* g_scratch_at[64];
+
+int scratch_of_current_device(void*& at) {
+    int dev = 0;
+    HS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) { set_error("device index above 63"); return hip_status(hipErrorInvalidDevice, "getrs_noalloc"); }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_at[dev]) {
+        void* p = nullptr;
+        HS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) { set_error("hipGetSymbolAddress gave no address"); return hip_status(hipErrorInvalidSymbol, "getrs_noalloc"); }
+        g_scratch_at[dev] = p;
+    }
+    at = g_scratch_at[dev];
+    return OK;
+}
+}  // namespace
+
+// getrs for FP64 with the library's static workspace instead of allocated memory: n <= 65536, k <= 16 and n k <= 2^18
+// (beyond them -2, -3 or -3 with a message). Every call on one device shares that device's workspace, so calls that
+// overlap on one device race; each call synchronises `s` before it returns.
+int getrs_noalloc(bool trans, int64_t n, int64_t k, const double* lu, int64_t lda, const int64_t* perm, double* b,
+                  int64_t ldb, hipStream_t s) {
+    if (n > NOALLOC_N) { set_error("getrs_noalloc: n above 65536"); return -2; }
+    if (k > NOALLOC_K || n * k > NOALLOC_ITEMS) { set_error("getrs_noalloc: k above 16 or n k above 2^18"); return -3; }
+    if (n == 0 || k == 0) return OK;
+    void* at = nullptr;
+    const int rc = scratch_of_current_device(at);
+    if (rc) return rc;
+    return getrs_ws<double>(trans, n, k, lu, lda, perm, b, ldb, s, (double*)at, (char*)at + NOALLOC_W);
+}
+
+namespace {
 template <typename T>
 __global__ void fill_identity(int64_t n, T* w) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -21,6 +21,30 @@ PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'hbm': 8.0e12}
 TRAFFIC = {('trail_f32', 16384): 395992297}
 
 
+def host_threads():
+    """Host threads for the CPU baseline: the cores this process may run on, capped by OMP_NUM_THREADS where the host
+    sets it (a shared GPU box grants each GPU a share of its cores); with the CPU model name, for the report."""
+    visible = len(os.sched_getaffinity(0))
+    cap = os.environ.get('OMP_NUM_THREADS')
+    threads = min(visible, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else visible
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            model = next((l.split(':', 1)[1].strip() for l in f if l.startswith('model name')), model)
+    except OSError:
+        pass
+    return threads, visible, model
+
+
+def gaussian_system(n, seed):
+    "The N(0, 1) variant of the mixed-precision workload: A (n×n) then b (n) iid N(0, 1) from torch's CUDA generator."
+    import torch
+    g = torch.Generator(device='cuda'); g.manual_seed(seed)
+    a = torch.randn(n, n, dtype=torch.float64, device='cuda', generator=g)
+    b = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+    return a, b
+
+
 def householder_matrix(n, seed, cond=1e6, reflectors=8):
     """A = U diag(s) Vᵀ with U and V each a product of `reflectors` Householder reflectors H = I - 2 v vᵀ / vᵀv (v iid
     N(0, 1)) and s log-spaced from 1 to 1/cond, built in FP64 on the GPU by rank-1 updates, never forming U or V."""
@@ -80,21 +104,40 @@ def batched_bench(reps=3):
     plan.close()
     t = min(times)
     big = ns >= 256
+    # The n >= 256 systems on their own, where the FP64 MFMA target applies: gathered from the same buffers.
+    a_off = np.concatenate([[0], np.cumsum(ns * ns)[:-1]]); b_off = np.concatenate([[0], np.cumsum(ns * ks)[:-1]])
+    idx = np.nonzero(big)[0]
+    sa0 = torch.cat([a0[int(a_off[i]):int(a_off[i] + ns[i] * ns[i])] for i in idx])
+    sb0 = torch.cat([b0[int(b_off[i]):int(b_off[i] + ns[i] * ks[i])] for i in idx])
+    sa, sb = sa0.clone(), sb0.clone()
+    splan = dv.BatchedSolve(sa, sb, ns[idx], ks[idx], panel_width=pw, wgs_per_cu=wg)
+    stimes = []
+    for r in range(reps + 1):
+        sa.copy_(sa0); sb.copy_(sb0)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(); splan.run(); e1.record(); e1.synchronize()
+        if r: stimes.append(e0.elapsed_time(e1) / 1e3)
+    splan.close()
+    st = min(stimes)
+    sflops = float((2 / 3 * ns[idx].astype(np.float64) ** 3 + 2.0 * ns[idx] * ns[idx] * ks[idx]).sum())
     return {'workload': f'{len(ns)} float64 systems, n uniform in [8, 512], k uniform in [1, 8], A = G + sqrt(n) I, '
                         'seed 3; one persistent launch replayed from a hipGraph',
             'solves_per_s': round(len(ns) / t, 1), 'ms': round(t * 1e3, 3), 'gflops': round(flops / t / 1e9, 2),
             'frac_fp64_peak': round(flops / t / PEAK['fp64_mfma'], 4), 'all_regular': ok,
             'tuned': {'panel_width': pw, 'wgs_per_cu': wg},
             'tuning_ms': {f'{k[0]}x{k[1]}': round(v, 3) for k, v in tuned.items()},
-            'systems_n_ge_256': int(big.sum())}
+            'systems_n_ge_256': int(big.sum()),
+            'n_ge_256_alone': {'ms': round(st * 1e3, 3), 'solves_per_s': round(len(idx) / st, 1),
+                               'gflops': round(sflops / st / 1e9, 2), 'frac_fp64_peak': round(sflops / st / PEAK['fp64_mfma'], 4)}}
 
 
-def batched_cpu_baseline(sample=800, threads=16):
+def batched_cpu_baseline(sample=800, threads=None):
     """The oracle's per-cell solve (rustymath's LuWork::solve restated) over a sample of the batched workload's
     distribution, one system per host thread as rustymath's stacks run, in solves/s."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
     import oracle
+    threads, visible, model = host_threads() if threads is None else (threads, threads, 'unknown')
     oracle.set_threads(1)
     rng = np.random.default_rng(3)
     ns = rng.integers(8, 513, size=sample); ks = rng.integers(1, 9, size=sample)
@@ -107,8 +150,85 @@ def batched_cpu_baseline(sample=800, threads=16):
     oracle.set_threads(threads)
     flops = float((2 / 3 * ns.astype(np.float64) ** 3 + 2.0 * ns * ns * ks).sum())
     return {'value': round(sample / t, 1), 'unit': 'solves/s', 'gflops': round(flops / t / 1e9, 2), 'cores': threads,
-            'kind': 'port', 'sample': f'{sample} systems of the same distribution (numpy seed 3), oracle solve, one '
-                                      f'system per thread on {threads} threads'}
+            'host_cores_visible': visible, 'cpu_model': model, 'kind': 'port',
+            'sample': f'{sample} systems of the same distribution (numpy seed 3), oracle solve, one system per thread '
+                      f'on {threads} threads'}
+
+
+def config1_bench(reps=5):
+    """The reference's CPU-runnable case (its first configuration): rm.matmul of two 1024×1024 float64 N(0, 1) matrices
+    (numpy seed 0, a then b) and rm.linalg.solve of a 1024×1024 N(0, 1) system with b 1024×1 (seed 1, A then b), through
+    this library's Python surface (host arrays in and out, the PCIe copies included) and device-resident (inputs already
+    in HBM), each the best of `reps` runs after one untimed; with the solve's backward error. Returns the timings and
+    the results, which the CPU leg compares with the oracle."""
+    import numpy as np
+    import torch
+    from rustymath_amd import linalg as la, device as dv
+    n = 1024
+    rng = np.random.default_rng(0); a = rng.standard_normal((n, n)); b = rng.standard_normal((n, n))
+    rng = np.random.default_rng(1); A = rng.standard_normal((n, n)); v = rng.standard_normal((n, 1))
+
+    def host_best(fn):
+        ts = []
+        for r in range(reps + 1):
+            t0 = time.perf_counter(); fn(); t1 = time.perf_counter()
+            if r: ts.append(t1 - t0)
+        return min(ts)
+
+    def dev_best(fn):
+        ts = []
+        for r in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(); fn(); e1.record(); e1.synchronize()
+            if r: ts.append(e0.elapsed_time(e1) / 1e3)
+        return min(ts)
+
+    c = la.matmul(a, b); x = la.solve(A, v)
+    t_mm_host = host_best(lambda: la.matmul(a, b)); t_solve_host = host_best(lambda: la.solve(A, v))
+    ad, bd = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(); cd = torch.empty_like(ad)
+    Ad, vd = torch.from_numpy(A).cuda(), torch.from_numpy(v).cuda()
+    work, xd = torch.empty_like(Ad), torch.empty_like(vd)
+
+    def dev_solve():
+        work.copy_(Ad); xd.copy_(vd); dv.gesv(work, xd)
+    t_mm_dev = dev_best(lambda: dv.gemm(ad, bd, cd)); t_solve_dev = dev_best(dev_solve)
+    mm_flops, solve_flops = 2.0 * n ** 3, 2 / 3 * n ** 3 + 2.0 * n * n
+    berr = float(np.linalg.norm(v - A @ x) / (np.linalg.norm(A) * np.linalg.norm(x)))
+    line = {'workload': 'matmul 1024x1024x1024 float64 N(0,1) (numpy seed 0); solve 1024x1024 float64 N(0,1) with b '
+                        '1024x1 (numpy seed 1, A then b); through linalg.matmul / linalg.solve (host arrays) and '
+                        'device-resident',
+            'matmul_host_ms': round(t_mm_host * 1e3, 3), 'matmul_host_gflops': round(mm_flops / t_mm_host / 1e9, 1),
+            'matmul_device_ms': round(t_mm_dev * 1e3, 3), 'matmul_device_gflops': round(mm_flops / t_mm_dev / 1e9, 1),
+            'solve_host_ms': round(t_solve_host * 1e3, 3), 'solve_host_gflops': round(solve_flops / t_solve_host / 1e9, 1),
+            'solve_device_ms': round(t_solve_dev * 1e3, 3),
+            'solve_device_gflops': round(solve_flops / t_solve_dev / 1e9, 1),
+            'backward_error': berr, 'backward_error_bound': 10 * n * float(np.finfo(np.float64).eps)}
+    return line, (a, b, c, A, v, x)
+
+
+def config1_cpu_baseline(line, data, reps=5):
+    """The CPU leg of the first configuration: the oracle (the C restatement of rustymath's CPU kernels) on the same
+    1024 inputs, on one host thread and on every host thread, best of `reps`; and whether the GPU's product and solve
+    equal the oracle's bit for bit."""
+    import oracle
+    a, b, c, A, v, x = data
+    threads, visible, model = host_threads()
+    out = {}
+    for t in (1, threads):
+        oracle.set_threads(t)
+        tm = min(_timed(lambda: oracle.matmul(a, b)) for _ in range(reps))
+        ts = min(_timed(lambda: oracle.solve(A, v)) for _ in range(reps))
+        out[f'{t}_thread' + ('s' if t > 1 else '')] = {'matmul_gflops': round(2.0 * 1024 ** 3 / tm / 1e9, 2),
+                                                       'solve_gflops': round((2 / 3 * 1024 ** 3 + 2 * 1024 ** 2) / ts / 1e9, 2)}
+    same = lambda p, q: p.shape == q.shape and p.tobytes() == q.tobytes()
+    line['bits_equal_oracle'] = {'matmul': same(c, oracle.matmul(a, b)), 'solve': same(x, oracle.solve(A, v))}
+    line['cpu_baseline'] = dict(out, kind='port', cores=threads, host_cores_visible=visible, cpu_model=model,
+                                sample='the same inputs; oracle matmul and solve, best of 5')
+    return line
+
+
+def _timed(fn):
+    t0 = time.perf_counter(); fn(); return time.perf_counter() - t0
 
 
 def adjoint_bench(reps=3):
@@ -132,8 +252,13 @@ def adjoint_bench(reps=3):
             (w * adjoint.solve(xs[0], adjoint.matmul(xs[1], xs[2]))).sum().backward()
             torch.cuda.synchronize()
             if r: best = min(best, time.perf_counter() - t0)
+        flops = max(batch, 1) * (2 / 3 * n ** 3 + 12.0 * n * n * 16)
         out[f'{max(batch, 1)}x{n}_ms'] = round(best * 1e3, 3)
+        out[f'{max(batch, 1)}x{n}_gflops'] = round(flops / best / 1e9, 1)
+        out[f'{max(batch, 1)}x{n}_frac_fp64_peak'] = round(flops / best / PEAK['fp64_mfma'], 4)
     out['workload'] = 'loss = sum(w * solve(A, C D)) + gradients in A, C, D; C n x n, D and w n x 16; float64'
+    out['flops'] = ('per system 2/3 n^3 (LU) + 12 n^2 k: the products C D, -Bbar X^T, Bbar D^T, C^T Bbar and the solve '
+                    'and transposed solve, 2 n^2 k each (k = 16)')
     return out
 
 
@@ -146,28 +271,29 @@ def precision_variants(a, b, x64, n, reps=2):
     from rustymath_amd import device as dv
     tol = 10 * n * torch.finfo(torch.float64).eps
     flops = 2 / 3 * n ** 3 + 2 * n * n
-    g = torch.Generator(device='cuda'); g.manual_seed(4)
-    ag = torch.randn(n, n, dtype=torch.float64, device='cuda', generator=g)
-    bg = torch.randn(n, dtype=torch.float64, device='cuda', generator=g)
+    ag, bg = gaussian_system(n, 4)
     fg, pg = dv.getrf(ag.clone())
     xg = bg.reshape(n, 1).clone(); dv.getrs(fg, pg, xg); xg = xg.reshape(n)
     del fg
     out = {}
     for name, (A, B, X) in (('cond1e6', (a, b, x64)), ('gaussian', (ag, bg, xg))):
-        for prec in ('fp32', 'bf16'):
+        for prec in ('fp32', 'bf16x3', 'bf16'):
             best, res = math.inf, None
             for _ in range(reps):
                 torch.cuda.synchronize(); t0 = time.perf_counter()
-                res = dv.dsgesv(A, B, tol=tol) if prec == 'fp32' else dv.dbgesv(A, B, tol=tol, max_iter=10)
+                if prec == 'fp32': res = dv.dsgesv(A, B, tol=tol)
+                else: res = dv.dbgesv(A, B, tol=tol, max_iter=10, terms=3 if prec == 'bf16x3' else 1)
                 torch.cuda.synchronize(); best = min(best, time.perf_counter() - t0)
             x = res[0]
             line = {'ms': round(best * 1e3, 3), 'gflops': round(flops / best / 1e9, 1), 'refinement_steps': res[1],
                     'backward_error': res[-1], 'forward_error_vs_fp64_lu': float((x - X).norm() / X.norm())}
-            if prec == 'bf16': line['gmres_steps'] = res[2]
+            if prec != 'fp32': line['gmres_steps'] = res[2]
             out[f'{name}_{prec}'] = line
     out['workload'] = (f'n={n}: the cond-1e6 matrix of the headline and an iid N(0,1) matrix (seed 4); fp32 = FP32 LU + '
-                       'classical FP64 refinement, bf16 = FP32 panels with BF16-input FP32-accumulate trailing products + '
-                       'FP64 GMRES-based refinement (restart 40, at most 10 refinement steps); tolerance 10 n eps64')
+                       'classical FP64 refinement; bf16x3 = FP32 panels with trailing products on split-BF16 operands '
+                       '(hi + lo pairs, three BF16 MFMA products, FP32 accumulation) + FP64 GMRES-based refinement; bf16 = '
+                       'the same with operands rounded once to BF16; restart 40, at most 10 refinement steps; tolerance '
+                       '10 n eps64')
     return out
 
 
@@ -222,11 +348,12 @@ def plain_fp64_bench(n=8192, k=16, reps=3):
             'backward_error': berr, 'backward_error_bound': 10 * n * float(torch.finfo(torch.float64).eps)}
 
 
-def cpu_baseline(n=6144, runs=3, threads=16):
+def cpu_baseline(n=6144, runs=3, threads=None):
     """The oracle's LU solve (the C restatement of rustymath's blocked CPU kernels) on a bounded sample: an FP64 N(0, 1)
-    system of order n on `threads` host threads, best of `runs`, in GFLOP/s (2/3 n³ + 2 n²)."""
+    system of order n on the host's threads (host_threads), best of `runs`, in GFLOP/s (2/3 n³ + 2 n²)."""
     import numpy as np
     import oracle
+    threads, visible, model = host_threads() if threads is None else (threads, threads, 'unknown')
     oracle.set_threads(threads)
     rng = np.random.default_rng(1)
     a = rng.standard_normal((n, n)); b = rng.standard_normal(n)
@@ -241,7 +368,8 @@ def cpu_baseline(n=6144, runs=3, threads=16):
     for _ in range(5):
         t0 = time.perf_counter(); oracle.solve(a1, b1); one = min(one, time.perf_counter() - t0)
     oracle.set_threads(threads)
-    return {'value': round((2 / 3 * n ** 3 + 2 * n * n) / best / 1e9, 2), 'unit': 'GFLOP/s', 'cores': threads, 'kind': 'port',
+    return {'value': round((2 / 3 * n ** 3 + 2 * n * n) / best / 1e9, 2), 'unit': 'GFLOP/s', 'cores': threads,
+            'host_cores_visible': visible, 'cpu_model': model, 'kind': 'port',
             'single_thread_n256_gflops': round((2 / 3 * 256 ** 3 + 2 * 256 * 256) / one / 1e9, 2),
             'sample': f'oracle (C restatement of rustymath lu_factor/lu_solve, AVX2 6x8 tiles) FP64 solve n={n}, 1 RHS, '
                       f'best of {runs}, {threads} threads'}
@@ -290,8 +418,15 @@ def main():
         x = b.reshape(n, 1).clone()
         dv.getrs(f, perm, x)
         return x.reshape(n)
-    x64 = fp64_step(); torch.cuda.synchronize()
-    t0 = time.perf_counter(); x64 = fp64_step(); torch.cuda.synchronize(); t64 = time.perf_counter() - t0
+    # The same warm-up and steps as the mixed solve: mean and best.
+    for _ in range(max(args.warmup, 1)):
+        x64 = fp64_step()
+    t64s = []
+    for _ in range(args.steps):
+        torch.cuda.synchronize(); t0 = time.perf_counter()
+        x64 = fp64_step()
+        torch.cuda.synchronize(); t64s.append(time.perf_counter() - t0)
+    t64, t64_best = sum(t64s) / len(t64s), min(t64s)
     r64 = ((b - a @ x64).norm() / (a.norm() * x64.norm())).item()
 
     # Low precision + refinement: the timed steps.
@@ -348,12 +483,13 @@ def main():
              'phases_ms': {k: round(ticks[q] / 1e5, 3) for k, q in (('reduce', 0), ('publish', 7), ('update', 1),
                                                                      ('exchange', 2), ('barrier', 3), ('pivot_update', 4),
                                                                      ('row_loads', 5), ('row_stores', 6))}}
-    batched = adj = plain = variants = None
+    batched = adj = plain = variants = cfg1 = None
     if not args.only_mixed:
         variants = precision_variants(a, b, x64, n)
         batched = batched_bench()
         adj = adjoint_bench()
         plain = plain_fp64_bench()
+        cfg1, cfg1_data = config1_bench()
 
     out = {
         'metric': METRIC, 'value': round(flops / t / 1e9, 2), 'unit': 'GFLOP/s', 'n_gpus': 1, 'steps': args.steps,
@@ -361,14 +497,17 @@ def main():
         'vs_baseline': None, 'dtype': 'f32 LU on MFMA + f64 refinement', 'data': 'synthetic',
         'config': {'workload': f'mixed precision solve, n={n}, A = U diag(s) V^T (8+8 Householder reflectors, '
                                f's = logspace(0, -6)), cond 1e6, b N(0,1), seed 3', 'n': n, 'rhs': 1},
-        'fp64_lu_solve_gflops': round(flops / t64 / 1e9, 2), 'speedup_vs_fp64_lu': round(t64 / t, 3),
+        'fp64_lu_solve_gflops': round(flops / t64 / 1e9, 2), 'fp64_lu_solve_ms': round(t64 * 1e3, 3),
+        'speedup_vs_fp64_lu': round(t64 / t, 3), 'best_ms_per_step': round(min(times) * 1e3, 3),
+        'fp64_lu_solve_best_ms': round(t64_best * 1e3, 3), 'speedup_vs_fp64_lu_best': round(t64_best / min(times), 3),
         'refinement_solves': iters, 'backward_error': berr, 'backward_error_fp64_lu': r64, 'tolerance': tol,
         'forward_error_vs_fp64_lu': fwd, 'roofline': roof, 'panel': panel, 'precision_variants': variants,
-        'batched': batched, 'adjoint': adj, 'plain_fp64': plain,
+        'batched': batched, 'adjoint': adj, 'plain_fp64': plain, 'config1': cfg1,
     }
     if not args.no_cpu and not args.only_mixed:
         out['cpu_baseline'] = cpu_baseline()
         out['batched']['cpu_baseline'] = batched_cpu_baseline()
+        out['config1'] = config1_cpu_baseline(cfg1, cfg1_data)
     print(json.dumps(out))
 
 

@@ -170,17 +170,19 @@ int hs_dev_spotrf(int64_t n, float* a, int64_t lda, int* ok, void* stream);
  * the number of corrections and the final backward error. A is read, never written. */
 int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, double tol,
                   int* iters, double* berr, void* stream);
-/* The BF16 factorization's trailing product on its own: c -= bf16(a) bf16(b), row-major FP32 operands (a m×k, b k×n,
- * c m×n) rounded to bfloat16 (nearest even), products summed in FP32 on MFMA. */
+/* The BF16 factorization's trailing product on its own: c -= a b for row-major FP32 operands (a m×k, b k×n, c m×n)
+ * taken as bfloat16, products summed in FP32 on MFMA: terms = 1 rounds each item to bfloat16 (nearest even); terms = 3
+ * splits each item into hi = bf16(x) and lo = bf16(x - hi) and adds a_lo b_hi, a_hi b_lo and a_hi b_hi. */
 int hs_dev_sgemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t lda, const float* b, int64_t ldb,
-                             float* c, int64_t ldc, void* stream);
+                             float* c, int64_t ldc, int terms, void* stream);
 /* The BF16 form of hs_dev_dsgesv (no counterpart in the reference): the factorization's trailing products take their
- * operands rounded to bfloat16 and sum in FP32 on MFMA (panels and rows of U in FP32), and each refinement step
- * solves A d = r by GMRES in FP64, left-preconditioned by those factors and restarted every `restart` (1..200)
- * steps, since classical refinement cannot converge from BF16 factors at cond(A) near 1e6. iters = refinement steps,
- * inner = GMRES steps (each one FP64 product with A and one pair of triangular solves). */
+ * operands as bfloat16 (`terms` as in hs_dev_sgemm_bf16_update: 1 = rounded once, 3 = split pairs) and sum in FP32
+ * on MFMA (panels and rows of U in FP32), and each refinement step solves A d = r by GMRES in FP64,
+ * left-preconditioned by those factors and restarted every `restart` (1..200) steps, since classical refinement cannot
+ * converge from such factors at cond(A) near 1e6. iters = refinement steps, inner = GMRES steps (each one FP64
+ * product with A and one pair of triangular solves). */
 int hs_dev_dbgesv(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, int restart,
-                  double tol, int* iters, int* inner, double* berr, void* stream);
+                  int terms, double tol, int* iters, int* inner, double* berr, void* stream);
 
 /* Batched uneven solves: the per-cell solves of a stack, which rustymath runs one `LuWork::solve` per Rayon
  * worker (src/linalg/stacked.rs:88-101 over core/src/parallel.rs:118-192 and core/src/linalg/factor.rs:207-226),

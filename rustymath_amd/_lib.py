@@ -55,8 +55,8 @@ SIGNATURES = {
     'hs_dev_dgetri': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_sgetri': (_i32, [_i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     'hs_dev_dsgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i32, _d, _vp, _vp, _vp]),
-    'hs_dev_sgemm_bf16_update': (_i32, [_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp]),
-    'hs_dev_dbgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i32, _i32, _d, _vp, _vp, _vp, _vp]),
+    'hs_dev_sgemm_bf16_update': (_i32, [_i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _i32, _vp]),
+    'hs_dev_dbgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i32, _i32, _i32, _d, _vp, _vp, _vp, _vp]),
     'hs_dev_dgesv_batched_plan': (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp]),
     'hs_dev_sgesv_batched_plan': (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp]),
     'hs_dev_gesv_batched_run': (_i32, [_i64, _i32, _vp]),

@@ -734,23 +734,25 @@ int hs_dev_dsgesv(int64_t n, const double* a, int64_t lda, const double* b, doub
 }
 
 int hs_dev_sgemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t lda, const float* b, int64_t ldb,
-                             float* c, int64_t ldc, void* stream) {
+                             float* c, int64_t ldc, int terms, void* stream) {
     if (m < 0) return -1;
     if (n < 0) return -2;
     if (k < 0) return -3;
     if (lda < (k > 1 ? k : 1)) return -5;
     if (ldb < (n > 1 ? n : 1)) return -7;
     if (ldc < (n > 1 ? n : 1)) return -9;
-    return gemm_bf16_update(m, n, k, a, lda, b, ldb, c, ldc, (hipStream_t)stream);
+    if (terms != 1 && terms != 3) return -10;
+    return gemm_bf16_update(m, n, k, a, lda, b, ldb, c, ldc, (hipStream_t)stream, terms);
 }
 
 int hs_dev_dbgesv(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, int restart,
-                  double tol, int* iters, int* inner, double* berr, void* stream) {
+                  int terms, double tol, int* iters, int* inner, double* berr, void* stream) {
     if (n < 0) return -1;
     if (lda != n) return -3;
     if (max_iter < 1) return -6;
     if (restart < 1 || restart > 200) return -7;
-    return gesv_gmres_ir(n, a, lda, b, x, max_iter, restart, tol, iters, inner, berr, (hipStream_t)stream);
+    if (terms != 1 && terms != 3) return -8;
+    return gesv_gmres_ir(n, a, lda, b, x, max_iter, restart, tol, iters, inner, berr, (hipStream_t)stream, terms);
 }
 
 int hs_dev_dgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,

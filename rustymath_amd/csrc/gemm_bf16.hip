@@ -1,7 +1,12 @@
-// The low-precision trailing product of the BF16 factorization: c (FP32) = c - a b with the items of a and b rounded
-// to bfloat16 (round to nearest even) and the products summed in FP32 on MFMA (v_mfma_f32_16x16x32_bf16). This form
-// has no counterpart in rustymath (which factors in FP64 or FP32 only); its results are not the reference's bits, and
-// the refinement that follows it recovers FP64 accuracy.
+// The low-precision trailing product of the BF16 factorization: c (FP32) = c - a b with the items of a and b taken as
+// bfloat16 and the products summed in FP32 on MFMA (v_mfma_f32_16x16x32_bf16). This form has no counterpart in
+// rustymath (which factors in FP64 or FP32 only); its results are not the reference's bits, and the refinement that
+// follows it recovers FP64 accuracy.
+//
+// TERMS = 1 rounds each item once to bfloat16 (nearest even): products carry a relative error up to 2^-8. TERMS = 3
+// splits each item x into hi = bf16(x) and lo = bf16(x - hi) (x - hi is exact in FP32), and sums a_lo b_hi,
+// a_hi b_lo and a_hi b_hi into the FP32 accumulator (the dropped a_lo b_lo and lo's rounding leave about 2^-16):
+// three BF16 products at the BF16 MFMA rate, close to FP32 operands in accuracy.
 //
 // Tiles of 128x128, 4 waves of 64x64 (4x4 MFMA tiles each), k in steps of 32: a's rows and b's columns are staged in
 // LDS as bfloat16, each row (column) holding its 32 items of the step contiguously, so that a lane reads its 8 items
@@ -23,11 +28,15 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {
     return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+__device__ __forceinline__ float from_bf16(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+
+template <int TERMS>
 __global__ __launch_bounds__(NT, 2) void gemm_bf16_kernel(int64_t m, int64_t n, int64_t k, const float* __restrict__ A,
                                                          int64_t lda, const float* __restrict__ B, int64_t ldb,
                                                          float* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
-    __shared__ __attribute__((aligned(16))) unsigned short As[TM][KP];  // rows of a, negated
-    __shared__ __attribute__((aligned(16))) unsigned short Bs[TN][KP];  // columns of b
+    constexpr int P = TERMS == 3 ? 2 : 1;  // parts per item: hi (and lo)
+    __shared__ __attribute__((aligned(16))) unsigned short As[P][TM][KP];  // rows of a, negated
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[P][TN][KP];  // columns of b
     const int total = tiles_m * tiles_n, per = (total + 7) / 8;
     const int id = (blockIdx.x % 8) * per + blockIdx.x / 8;  // each XCD a contiguous run of tiles
     if (id >= total) return;
@@ -69,22 +78,39 @@ __global__ __launch_bounds__(NT, 2) void gemm_bf16_kernel(int64_t m, int64_t n, 
         {
             const int r = t >> 1, p = (t & 1) * 16;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) As[r][p + q] = to_bf16(av[q]);
+            for (int q = 0; q < 16; ++q) {
+                const unsigned short hi = to_bf16(av[q]);
+                As[0][r][p + q] = hi;
+                if (TERMS == 3) As[P - 1][r][p + q] = to_bf16(av[q] - from_bf16(hi));
+            }
             const int pk = t >> 3, c = (t & 7) * 16;
 #pragma unroll
-            for (int q = 0; q < 16; ++q) Bs[c + q][pk] = to_bf16(bv[q]);
+            for (int q = 0; q < 16; ++q) {
+                const unsigned short hi = to_bf16(bv[q]);
+                Bs[0][c + q][pk] = hi;
+                if (TERMS == 3) Bs[P - 1][c + q][pk] = to_bf16(bv[q] - from_bf16(hi));
+            }
         }
         __syncthreads();
-        bf8 fa[4], fb[4];
+        bf8 fa[P][4], fb[P][4];
         const int kq = 8 * (lane >> 4), li = lane & 15;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) fa[i] = *reinterpret_cast<const bf8*>(&As[wm * 64 + i * 16 + li][kq]);
+        for (int h = 0; h < P; ++h) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) fb[j] = *reinterpret_cast<const bf8*>(&Bs[wn * 64 + j * 16 + li][kq]);
+            for (int i = 0; i < 4; ++i) fa[h][i] = *reinterpret_cast<const bf8*>(&As[h][wm * 64 + i * 16 + li][kq]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) fb[h][j] = *reinterpret_cast<const bf8*>(&Bs[h][wn * 64 + j * 16 + li][kq]);
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 4; ++j) {
+                if (TERMS == 3) {  // the small products first
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[P - 1][i], fb[0][j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][i], fb[P - 1][j], acc[i][j], 0, 0, 0);
+                }
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0][i], fb[0][j], acc[i][j], 0, 0, 0);
+            }
     }
 
 #pragma unroll
@@ -101,11 +127,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_bf16_kernel(int64_t m, int64_t n, 
 }  // namespace
 
 int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t lda, const float* b, int64_t ldb, float* c,
-                     int64_t ldc, hipStream_t s) {
+                     int64_t ldc, hipStream_t s, int terms) {
     if (m == 0 || n == 0 || k == 0) return OK;
     const int tiles_m = (int)((m + TM - 1) / TM), tiles_n = (int)((n + TN - 1) / TN);
     const unsigned grid = (unsigned)(((tiles_m * tiles_n + 7) / 8) * 8);
-    gemm_bf16_kernel<<<grid, NT, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, tiles_m, tiles_n);
+    if (terms == 3) gemm_bf16_kernel<3><<<grid, NT, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, tiles_m, tiles_n);
+    else gemm_bf16_kernel<1><<<grid, NT, 0, s>>>(m, n, k, a, lda, b, ldb, c, ldc, tiles_m, tiles_n);
     HS_HIP(hipGetLastError());
     return OK;
 }

@@ -15,21 +15,21 @@ template <typename T>
 int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda,
          const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s, bool in_order = false);
 
-// BF16-input, FP32-accumulate product for the low-precision factorization: c (f32) = c - a b with a, b
-// rounded to bfloat16 once (MFMA 16x16x32 bf16).
+// BF16-input, FP32-accumulate product for the low-precision factorization: c (f32) = c - a b with a, b rounded to
+// bfloat16 once (terms = 1) or split into bfloat16 pairs hi + lo with three products (terms = 3), on MFMA 16x16x32 bf16.
 int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t lda, const float* b, int64_t ldb,
-                     float* c, int64_t ldc, hipStream_t s);
+                     float* c, int64_t ldc, hipStream_t s, int terms);
 
 // LU with partial pivoting of the row-major n×n matrix `a` (ld `lda`) in place: L strictly below the
 // diagonal (unit diagonal implicit), U on and above; pivots[c] = the row swapped with row c (absolute,
 // 0-based, LAPACK ipiv style). The arithmetic is rustymath's lu_factor (core/src/linalg/factor.rs:617),
 // item for item. A zero pivot leaves its column undivided (factor.rs:606); the C ABI's `info` reports the first one
 // from the factors' diagonal afterwards. Orders above 16384 give ERR_UNSUPPORTED.
-// bf16_trailing (FP32 only): the trailing products after each 256-column block take their operands rounded to
-// bfloat16 (gemm_bf16_update); panels, row moves and the rows of U stay FP32. Not the reference's arithmetic: the
-// low-precision factors of the BF16 mixed solve.
+// bf16_terms (FP32 only, 0 = none): the trailing products after each 256-column block take their operands as bfloat16
+// (gemm_bf16_update with that many terms); panels, row moves and the rows of U stay FP32. Not the reference's
+// arithmetic: the low-precision factors of the BF16 mixed solve.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf16_trailing = false);
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms = 0);
 
 // Solves with factors from getrf: x = U⁻¹ L⁻¹ P b in place on b (n×k, ld ldb, row-major), with the
 // reference's substitution orders (factor.rs:633-641, :668-748). `trans` solves Aᵀ x = b instead
@@ -73,7 +73,7 @@ int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x,
 // FP64 GMRES-based refinement (GMRES on A d = r, left-preconditioned by the low-precision LU factors, restarted every
 // `restart` steps) until the backward error reaches tol (ir.hip). iters: refinement steps; inner: GMRES steps.
 int gesv_gmres_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, int restart,
-                  double tol, int* iters, int* inner, double* berr, hipStream_t s);
+                  double tol, int* iters, int* inner, double* berr, hipStream_t s, int terms);
 
 // Batched solves (batched.hip): a plan for `count` systems with orders n[i] <= 512 and k[i] <= 8 right-hand
 // sides, A_i row-major at a + a_off[i], B_i row-major n×k at b + b_off[i] (overwritten by X_i where regular),

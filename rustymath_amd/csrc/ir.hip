@@ -210,7 +210,7 @@ int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x,
 // two into FP32's range), classical Gram-Schmidt with one reorthogonalization, Givens rotations on the host. A step's
 // GMRES stops when its preconditioned residual has fallen by GMRES_TOL or after `restart` steps.
 int gesv_gmres_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x, int max_iter, int restart,
-                  double tol, int* iters, int* inner, double* berr, hipStream_t s) {
+                  double tol, int* iters, int* inner, double* berr, hipStream_t s, int terms) {
     constexpr double GMRES_TOL = 1e-4;
     if (iters) *iters = 0;
     if (inner) *inner = 0;
@@ -258,7 +258,7 @@ int gesv_gmres_ir(int64_t n, const double* a, int64_t lda, const double* b, doub
         return OK;
     };
     if (!rc) rc = cast_d2s(n, n, a, lda, a32, n, s);
-    if (!rc) rc = getrf<float>(n, a32, n, d_piv, s, true);
+    if (!rc) rc = getrf<float>(n, a32, n, d_piv, s, terms);
     if (!rc) {
         std::vector<int64_t> piv(n), perm(n);
         if (hipMemcpy(piv.data(), d_piv, sizeof(int64_t) * n, hipMemcpyDeviceToHost) != hipSuccess)

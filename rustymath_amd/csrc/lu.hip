@@ -794,7 +794,7 @@ int trail_profile(int on, double* ms, int64_t* launches, double* flops, double* 
 // the trailing matrix takes one product with k up to 256, which reads and writes it a quarter as often as per-panel
 // updates would. Every item still takes its terms in order of the pivot column, one fused multiply-add each.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf16_trailing) {
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms) {
     if (n == 0) return OK;
     if (n > (int64_t)MAX_WG * PT) {
         set_error("LU of order above 16384 is not supported: the panel kernel holds at most 32 x 512 rows");
@@ -928,9 +928,9 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
         }
         if (rc) break;
         if constexpr (std::is_same<T, float>::value) {
-            if (bf16_trailing) {
+            if (bf16_terms) {
                 rc = gemm_bf16_update(n - B1, n - B1, B1 - B0, a + B1 * lda + B0, lda, a + B0 * lda + B1, lda,
-                                      a + B1 * lda + B1, lda, s);
+                                      a + B1 * lda + B1, lda, s, bf16_terms);
                 continue;
             }
         }
@@ -977,8 +977,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, bool bf1
     return rc;
 }
 
-template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t, bool);
-template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t, bool);
+template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t, int);
+template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t, int);
 
 template <typename T>
 int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s) {

@@ -121,10 +121,11 @@ def dsgesv(a, b, max_iter=30, tol=None):
     return x, it.value, be.value
 
 
-def dbgesv(a, b, max_iter=30, restart=40, tol=None):
+def dbgesv(a, b, max_iter=30, restart=40, tol=None, terms=3):
     """Solves A x = b for the f64 matrix `a` (n×n, compact) and vector `b` with BF16 trailing products in the
-    factorization (FP32 panels, FP32 accumulation on MFMA) and FP64 GMRES-based refinement preconditioned by those
-    factors. Returns (x, refinement steps, GMRES steps, backward error); `tol` defaults to 10·n·eps64."""
+    factorization (FP32 panels, FP32 accumulation on MFMA; `terms` 3 splits each operand into a bfloat16 pair and takes
+    three products, 1 rounds it once) and FP64 GMRES-based refinement preconditioned by those factors. Returns
+    (x, refinement steps, GMRES steps, backward error); `tol` defaults to 10·n·eps64."""
     import ctypes
     _mat(a, 'a')
     n = a.shape[0]
@@ -134,21 +135,23 @@ def dbgesv(a, b, max_iter=30, restart=40, tol=None):
     if tol is None: tol = 10 * n * torch.finfo(torch.float64).eps
     it, inner, be = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0)
     _lib.check(_lib.load().hs_dev_dbgesv(n, a.data_ptr(), a.stride(0), b.data_ptr(), x.data_ptr(), max_iter, restart,
-                                         tol, ctypes.byref(it), ctypes.byref(inner), ctypes.byref(be), _stream()),
+                                         terms, tol, ctypes.byref(it), ctypes.byref(inner), ctypes.byref(be),
+                                         _stream()),
                'dbgesv')
     return x, it.value, inner.value, be.value
 
 
-def gemm_bf16_update(c, a, b):
-    """c -= bf16(a) bf16(b) in place for FP32 tensors (the BF16 factorization's trailing product): a and b rounded to
-    bfloat16 (nearest even), products summed in FP32 on MFMA."""
+def gemm_bf16_update(c, a, b, terms=1):
+    """c -= a b in place for FP32 tensors with the operands taken as bfloat16 (the BF16 factorization's trailing
+    product): rounded once to nearest even (terms 1) or split into hi + lo pairs with three products (terms 3), the
+    products summed in FP32 on MFMA."""
     for t, name in ((a, 'a'), (b, 'b'), (c, 'c')):
         _mat(t, name)
         if t.dtype != torch.float32: raise TypeError('gemm_bf16_update takes float32')
     m, k = a.shape
     if b.shape[0] != k or c.shape != (m, b.shape[1]): raise ValueError('gemm_bf16_update: shapes do not match')
     _lib.check(_lib.load().hs_dev_sgemm_bf16_update(m, b.shape[1], k, a.data_ptr(), a.stride(0), b.data_ptr(),
-                                                    b.stride(0), c.data_ptr(), c.stride(0), _stream()),
+                                                    b.stride(0), c.data_ptr(), c.stride(0), terms, _stream()),
                'gemm_bf16_update')
     return c
 

@@ -93,12 +93,14 @@ def test_mixed_precision_solve_rejects_bad_arguments_on_the_host():
     assert lib.hs_dev_dsgesv(-1, None, 1, None, None, 10, 0.0, None, None, None) == -1
     assert lib.hs_dev_dsgesv(4, None, 5, None, None, 10, 0.0, None, None, None) == -3
     assert lib.hs_dev_dsgesv(4, None, 4, None, None, 0, 0.0, None, None, None) == -6
-    assert lib.hs_dev_dbgesv(-1, None, 1, None, None, 10, 40, 0.0, None, None, None, None) == -1
-    assert lib.hs_dev_dbgesv(4, None, 5, None, None, 10, 40, 0.0, None, None, None, None) == -3
-    assert lib.hs_dev_dbgesv(4, None, 4, None, None, 0, 40, 0.0, None, None, None, None) == -6
-    assert lib.hs_dev_dbgesv(4, None, 4, None, None, 10, 0, 0.0, None, None, None, None) == -7
-    assert lib.hs_dev_sgemm_bf16_update(-1, 1, 1, None, 1, None, 1, None, 1, None) == -1
-    assert lib.hs_dev_sgemm_bf16_update(2, 3, 4, None, 3, None, 3, None, 3, None) == -5
+    assert lib.hs_dev_dbgesv(-1, None, 1, None, None, 10, 40, 3, 0.0, None, None, None, None) == -1
+    assert lib.hs_dev_dbgesv(4, None, 5, None, None, 10, 40, 3, 0.0, None, None, None, None) == -3
+    assert lib.hs_dev_dbgesv(4, None, 4, None, None, 0, 40, 3, 0.0, None, None, None, None) == -6
+    assert lib.hs_dev_dbgesv(4, None, 4, None, None, 10, 0, 3, 0.0, None, None, None, None) == -7
+    assert lib.hs_dev_dbgesv(4, None, 4, None, None, 10, 40, 2, 0.0, None, None, None, None) == -8
+    assert lib.hs_dev_sgemm_bf16_update(-1, 1, 1, None, 1, None, 1, None, 1, 1, None) == -1
+    assert lib.hs_dev_sgemm_bf16_update(2, 3, 4, None, 3, None, 3, None, 3, 1, None) == -5
+    assert lib.hs_dev_sgemm_bf16_update(2, 3, 4, None, 4, None, 3, None, 3, 2, None) == -10
 
 
 def test_triangular_solve_rejects_bad_codes_on_the_host():

@@ -148,6 +148,23 @@ def test_lu_at_the_benchmark_orders_matches_the_oracle_bit_for_bit(la, dtype, n)
     assert bits_equal(f.factors, want), f'max diff {np.abs(f.factors - want).max()}'
     b = random(n, 1, 13, dtype)[:, 0]
     assert bits_equal(f.solve(b), oracle.lu_solve(want, perm, b))
+    # Sixteen right-hand sides (the plain FP64 benchmark's): `backward`'s halving at full size (factor.rs:693-707).
+    B = np.random.default_rng(1).standard_normal((n, 16)).astype(dtype)
+    assert bits_equal(f.solve(B), oracle.lu_solve(want, perm, B))
+
+
+def test_the_cpu_runnable_configuration_equals_the_oracle(la):
+    """The reference's CPU-runnable configuration through the Python surface (py/src/linalg.rs:29-33, :241-247):
+    matmul of two 1024×1024 float64 N(0, 1) matrices (numpy seed 0, a then b) and solve of a 1024×1024 N(0, 1) system
+    with b 1024×1 (seed 1, A then b); bits equal to the oracle's, and the solve's backward error within 10·n·eps64."""
+    n = 1024
+    rng = np.random.default_rng(0); a = rng.standard_normal((n, n)); b = rng.standard_normal((n, n))
+    assert bits_equal(la.matmul(a, b), oracle.matmul(a, b))
+    rng = np.random.default_rng(1); A = rng.standard_normal((n, n)); v = rng.standard_normal((n, 1))
+    x = la.solve(A, v)
+    assert bits_equal(x, oracle.solve(A, v))
+    assert bits_equal(la.solve(A, v[:, 0]), x[:, 0])
+    assert backward_error(A, x, v) <= 10 * n * EPS[np.float64]
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])

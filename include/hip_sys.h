@@ -204,6 +204,19 @@ int hs_dev_dgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k,
 int hs_dev_sgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
                               const int64_t* b_off, const int64_t* p_off, float* a, float* b, int64_t* perm,
                               int* regular, int mode, int panel_width, int wgs_per_cu, int64_t* plan);
+/* The FP32-LU + FP64-refinement form of the batched solves (no counterpart in the reference, which has no refinement):
+ * the same systems and offsets; a and b (f64, device) are read, never written. Each system is factored in FP32 on
+ * MFMA (a32: Σ n² FP32 items at a's offsets, left holding float32(A_i)'s factors as a mode-0 plan leaves them, with
+ * perm and regular), solved, then refined on the GPU: X += the FP32 solve of the scaled residual, R = B - A X in
+ * FP64, until the normwise backward error ||R||_F / (||A||_F ||X||_F) is at most tol_per_n · n_i · 2^-52 or
+ * max_iter (0..100) corrections have run. r32 (Σ n k FP32 items at b's offsets) is workspace; x (f64, b's offsets)
+ * gets X_i, iters[i] the corrections applied and berr[i] the final backward error (NaN where regular[i] = 0, whose x
+ * is left as it was). One run is the whole sequence (one persistent launch per pass), replayed from a hipGraph with
+ * graph = 1; run and free it as any plan. */
+int hs_dev_dsgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
+                               const int64_t* b_off, const int64_t* p_off, const double* a, const double* b, double* x,
+                               float* a32, float* r32, int64_t* perm, int* regular, int* iters, double* berr,
+                               int max_iter, double tol_per_n, int panel_width, int wgs_per_cu, int64_t* plan);
 int hs_dev_gesv_batched_run(int64_t plan, int graph, void* stream);
 int hs_dev_gesv_batched_free(int64_t plan);
 

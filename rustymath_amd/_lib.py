@@ -59,6 +59,8 @@ SIGNATURES = {
     'hs_dev_dbgesv': (_i32, [_i64, _vp, _i64, _vp, _vp, _i32, _i32, _i32, _d, _vp, _vp, _vp, _vp]),
     'hs_dev_dgesv_batched_plan': (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp]),
     'hs_dev_sgesv_batched_plan': (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _vp]),
+    'hs_dev_dsgesv_batched_plan': (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          _i32, _d, _i32, _i32, _vp]),
     'hs_dev_gesv_batched_run': (_i32, [_i64, _i32, _vp]),
     'hs_dev_gesv_batched_free': (_i32, [_i64]),
     'hs_profile_panel': (_i32, [_i32, _vp]),

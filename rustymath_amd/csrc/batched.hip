@@ -165,11 +165,13 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
 
 // BW: the widest panel (16 or 32 columns, a tuning choice; the results do not depend on it). `mode`: factor and
 // solve (perm and regular written), or solve A X = B / Aᵀ X = B with the factors and perm of an earlier run.
+// Where `active` is not null, a system whose flag is 0 is skipped (the refinement's converged systems).
 template <typename T, int BW>
 __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc* __restrict__ desc, const int* __restrict__ order,
                                                           T* __restrict__ a_all, T* __restrict__ b_all,
                                                           long long* __restrict__ perm_all, int* __restrict__ regular_all,
-                                                          int* next, int mode, long long* prof) {
+                                                          const int* __restrict__ active, int* next, int mode,
+                                                          long long* prof) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Where `prof` is not null, thread 0 of workgroup 0 adds each phase's 100 MHz ticks into prof[0..4] and counts
     // its systems in prof[7]: setup, panel columns, U rows and trailing update, regularity + forward, backward.
@@ -201,8 +203,12 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
         __syncthreads();
         const int q = s_sys;
         if (q >= count) break;
-        if (timed) prof[7] += 1;
         const int sys = order[q];
+        if (active && !active[sys]) {
+            __syncthreads();  // every thread has read s_sys before thread 0 takes the next system
+            continue;
+        }
+        if (timed) prof[7] += 1;
         const Desc d = desc[sys];
         const int n = (int)d.n, k = (int)d.k;
         T* a = a_all + d.a_off;
@@ -531,6 +537,134 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
 
 __global__ void zero_int(int* p) { *p = 0; }
 
+// ---- The FP32-LU + FP64-refinement form (no counterpart in the reference, which has no refinement) ----
+// Per system, one workgroup of 256 threads: the FP32 kernel above factors float32(A_i) and solves, and each refinement
+// step here adds the correction in FP64, forms r = B - A X in FP64 from the original A, and decides on the GPU whether
+// the normwise backward error ||R||_F / (||A||_F ||X||_F) has reached the system's tolerance (tol_per_n · n · 2^-52);
+// a system that has not takes another FP32 solve of its residual, scaled by a power of two into FP32's range.
+constexpr int IRT = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    // A fixed order: within each wave by halving, then the four waves' sums in order.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// The power of two that brings a positive finite v to [1, 2), or 1.
+__device__ __forceinline__ double pow2_scale(double v) {
+    if (!(v > 0) || !(v <= 1.7976931348623157e308)) return 1.0;
+    return exp2(-floor(log2(v)));
+}
+
+__global__ __launch_bounds__(IRT) void ir_prep_kernel(int count, const Desc* __restrict__ desc, const double* __restrict__ a,
+                                                      const double* __restrict__ b, float* __restrict__ a32,
+                                                      float* __restrict__ r32, double* __restrict__ anorm,
+                                                      double* __restrict__ rscale) {
+    __shared__ double red[4];
+    const int sys = blockIdx.x;
+    if (sys >= count) return;
+    const Desc d = desc[sys];
+    const long long nn = d.n * d.n, nk = d.n * d.k;
+    double aa = 0, bb = 0;
+    for (long long i = threadIdx.x; i < nn; i += IRT) {
+        const double v = a[d.a_off + i];
+        a32[d.a_off + i] = (float)v;
+        aa = __builtin_elementwise_fma(v, v, aa);
+    }
+    for (long long i = threadIdx.x; i < nk; i += IRT) {
+        const double v = b[d.b_off + i];
+        bb = __builtin_elementwise_fma(v, v, bb);
+    }
+    aa = block_sum(aa, red);
+    bb = block_sum(bb, red);
+    const double sc = pow2_scale(sqrt(bb / (double)(nk > 0 ? nk : 1)));
+    for (long long i = threadIdx.x; i < nk; i += IRT) r32[d.b_off + i] = (float)(b[d.b_off + i] * sc);
+    if (threadIdx.x == 0) { anorm[sys] = sqrt(aa); rscale[sys] = sc; }
+}
+
+// One refinement step for every system still active (all regular systems at the first step, which takes X from the
+// first FP32 solve): X += correction, R = B - A X, the backward error, and the next scaled residual where needed.
+__global__ __launch_bounds__(IRT) void ir_step_kernel(int count, const Desc* __restrict__ desc, const double* __restrict__ a,
+                                                      const double* __restrict__ b, double* __restrict__ x,
+                                                      float* __restrict__ r32, double* __restrict__ rscale,
+                                                      const double* __restrict__ anorm, int* __restrict__ active,
+                                                      int* __restrict__ iters, double* __restrict__ berr,
+                                                      const int* __restrict__ regular, int first, int last,
+                                                      double tol_per_n) {
+    __shared__ double xs[BT][MAXRHS];
+    __shared__ double red[4];
+    const int sys = blockIdx.x;
+    if (sys >= count) return;
+    const Desc d = desc[sys];
+    const int n = (int)d.n, k = (int)d.k, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (first) {
+        if (tid == 0) { iters[sys] = 0; berr[sys] = regular[sys] && n > 0 ? 0.0 : __builtin_nan(""); active[sys] = 0; }
+        if (!regular[sys] || n == 0 || k == 0) return;
+    } else if (!active[sys]) {
+        return;
+    }
+    const double inv = 1.0 / rscale[sys];
+    for (int i = tid; i < n * k; i += IRT) {
+        const double c = (double)r32[d.b_off + i];
+        const double v = first ? c * inv : __builtin_elementwise_fma(c, inv, x[d.b_off + i]);
+        x[d.b_off + i] = v;
+        xs[i / k][i % k] = v;
+    }
+    __syncthreads();
+    // R = B - A X: one wave per row, lanes along the row. The residual goes to r32 as FP32 at once (its scaling by a
+    // power of two afterwards is exact), its squares into rr.
+    const double* A = a + d.a_off;
+    double rr = 0, xx = 0;
+    for (int row = wave; row < n; row += IRT / 64) {
+        double acc[MAXRHS];
+#pragma unroll
+        for (int c = 0; c < MAXRHS; ++c) acc[c] = 0;
+        for (int j = lane; j < n; j += 64) {
+            const double aij = A[(long long)row * n + j];
+#pragma unroll
+            for (int c = 0; c < MAXRHS; ++c)
+                if (c < k) acc[c] = __builtin_elementwise_fma(aij, xs[j][c], acc[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < MAXRHS; ++c) {
+            if (c < k) {
+                double v = acc[c];
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == 0) {
+                    const double r = b[d.b_off + (long long)row * k + c] - v;
+                    r32[d.b_off + (long long)row * k + c] = (float)r;
+                    rr = __builtin_elementwise_fma(r, r, rr);
+                }
+            }
+        }
+    }
+    for (int i = tid; i < n * k; i += IRT) {
+        const double v = xs[i / k][i % k];
+        xx = __builtin_elementwise_fma(v, v, xx);
+    }
+    rr = block_sum(rr, red);
+    xx = block_sum(xx, red);
+    const double be = sqrt(rr) / (anorm[sys] * sqrt(xx));
+    const double tol = tol_per_n * (double)n * 2.220446049250313e-16;
+    const bool more = !last && be > tol && be <= 1.7976931348623157e308;
+    if (tid == 0) {
+        berr[sys] = be;
+        active[sys] = more ? 1 : 0;
+        if (!first) iters[sys] += 1;
+    }
+    if (more) {
+        const double sc = pow2_scale(sqrt(rr / (double)(n * k)));
+        if (tid == 0) rscale[sys] = sc;
+        __syncthreads();  // every row's residual is in r32
+        for (int i = tid; i < n * k; i += IRT) r32[d.b_off + i] = (float)((double)r32[d.b_off + i] * sc);
+    }
+}
+
 // A plan: the systems' descriptors and their order on the device, and the launch captured in a hipGraph.
 struct Plan {
     int count = 0;
@@ -546,6 +680,18 @@ struct Plan {
     int* regular = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // The FP32-LU + FP64-refinement form (a, b above are then the FP32 factors and scaled residuals):
+    bool ir = false;
+    const double* a64 = nullptr;
+    const double* b64 = nullptr;
+    double* x64 = nullptr;
+    int* iters = nullptr;
+    double* berr = nullptr;
+    double* d_anorm = nullptr;
+    double* d_rscale = nullptr;
+    int* d_active = nullptr;
+    int max_iter = 0;
+    double tol_per_n = 10.0;
 };
 long long* g_bprof = nullptr;  // the batched kernel's phase ticks, while profiling is on
 bool g_bprof_on = false;
@@ -554,16 +700,40 @@ std::map<long long, Plan*> g_plans;
 long long g_next_plan = 1;
 
 template <typename T>
-int enqueue(Plan* p, hipStream_t s, long long* prof) {
+int enqueue_mode(Plan* p, int mode, const int* active, hipStream_t s, long long* prof) {
     zero_int<<<1, 1, 0, s>>>(p->d_next);
     if (p->bw == 16)
         gesv_batched_kernel<T, 16><<<p->grid, BT, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, (T*)p->b, p->perm,
-                                                         p->regular, p->d_next, p->mode, prof);
+                                                         p->regular, active, p->d_next, mode, prof);
     else
         gesv_batched_kernel<T, 32><<<p->grid, BT, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, (T*)p->b, p->perm,
-                                                         p->regular, p->d_next, p->mode, prof);
+                                                         p->regular, active, p->d_next, mode, prof);
     HS_HIP(hipGetLastError());
     return OK;
+}
+
+// The refinement form's sequence: cast and scale, the FP32 factorization and first solve, then max_iter rounds of a
+// refinement step and an FP32 solve of the systems still active, and a last step that only measures.
+int enqueue_ir(Plan* p, hipStream_t s, long long* prof) {
+    const unsigned g = (unsigned)(p->count > 0 ? p->count : 1);
+    ir_prep_kernel<<<g, IRT, 0, s>>>(p->count, p->d_desc, p->a64, p->b64, (float*)p->a, (float*)p->b, p->d_anorm,
+                                     p->d_rscale);
+    HS_HIP(hipGetLastError());
+    int rc = enqueue_mode<float>(p, FACTOR_SOLVE, nullptr, s, prof);
+    for (int it = 0; rc == OK && it <= p->max_iter; ++it) {
+        ir_step_kernel<<<g, IRT, 0, s>>>(p->count, p->d_desc, p->a64, p->b64, p->x64, (float*)p->b, p->d_rscale,
+                                         p->d_anorm, p->d_active, p->iters, p->berr, p->regular, it == 0,
+                                         it == p->max_iter, p->tol_per_n);
+        HS_HIP(hipGetLastError());
+        if (it < p->max_iter) rc = enqueue_mode<float>(p, SOLVE, p->d_active, s, prof);
+    }
+    return rc;
+}
+
+template <typename T>
+int enqueue(Plan* p, hipStream_t s, long long* prof) {
+    if (p->ir) return enqueue_ir(p, s, prof);
+    return enqueue_mode<T>(p, p->mode, nullptr, s, prof);
 }
 
 Plan* find_plan(long long handle) {
@@ -620,6 +790,32 @@ int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_
     return OK;
 }
 
+int batched_ir_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
+                    const int64_t* p_off, const double* a, const double* b, double* x, float* a32, float* r32,
+                    int64_t* perm, int* regular, int* iters, double* berr, int max_iter, double tol_per_n,
+                    int panel_width, int wgs_per_cu, long long* handle) {
+    if (max_iter < 0 || max_iter > 100) { set_error("batched refinement: max_iter must be in 0..100"); return -16; }
+    if (!(tol_per_n > 0)) { set_error("batched refinement: the tolerance factor must be > 0"); return -17; }
+    for (int64_t i = 0; i < count; ++i)
+        if (n[i] > 0 && (!a || !x || !iters || !berr)) { set_error("batched refinement needs a, x, iters and berr"); return -7; }
+    long long h = 0;
+    int rc = batched_plan(count, n, k, a_off, b_off, p_off, false, a32, r32, perm, regular, FACTOR_SOLVE, panel_width,
+                          wgs_per_cu, &h);
+    if (rc) return rc;
+    Plan* p = find_plan(h);
+    p->ir = true;
+    p->a64 = a; p->b64 = b; p->x64 = x; p->iters = iters; p->berr = berr;
+    p->max_iter = max_iter; p->tol_per_n = tol_per_n;
+    const size_t cnt = count > 0 ? (size_t)count : 1;
+    if (hipMalloc(&p->d_anorm, sizeof(double) * cnt) != hipSuccess || hipMalloc(&p->d_rscale, sizeof(double) * cnt) != hipSuccess ||
+        hipMalloc(&p->d_active, sizeof(int) * cnt) != hipSuccess) {
+        batched_free(h);
+        return hip_status(hipErrorOutOfMemory, "hipMalloc");
+    }
+    *handle = h;
+    return OK;
+}
+
 int batched_run(long long handle, bool use_graph, hipStream_t s) {
     Plan* p = find_plan(handle);
     if (!p) { set_error("unknown batched plan"); return -1; }
@@ -666,6 +862,9 @@ int batched_free(long long handle) {
     if (p->exec) hipGraphExecDestroy(p->exec);
     if (p->graph) hipGraphDestroy(p->graph);
     hipFree(p->d_desc); hipFree(p->d_order); hipFree(p->d_next);
+    if (p->d_anorm) hipFree(p->d_anorm);
+    if (p->d_rscale) hipFree(p->d_rscale);
+    if (p->d_active) hipFree(p->d_active);
     delete p;
     return OK;
 }

@@ -779,6 +779,19 @@ int hs_dev_sgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k,
     *plan = h;
     return rc;
 }
+int hs_dev_dsgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
+                               const int64_t* b_off, const int64_t* p_off, const double* a, const double* b, double* x,
+                               float* a32, float* r32, int64_t* perm, int* regular, int* iters, double* berr,
+                               int max_iter, double tol_per_n, int panel_width, int wgs_per_cu, int64_t* plan) {
+    if (count < 0) return -1;
+    if (count > 0 && (!n || !k || !a_off || !b_off || !p_off)) return -2;
+    if (!plan) return -20;
+    long long h = 0;
+    const int rc = batched_ir_plan(count, n, k, a_off, b_off, p_off, a, b, x, a32, r32, perm, regular, iters, berr, max_iter,
+                                   tol_per_n, panel_width, wgs_per_cu, &h);
+    *plan = h;
+    return rc;
+}
 int hs_dev_gesv_batched_run(int64_t plan, int graph, void* stream) {
     return batched_run(plan, graph != 0, (hipStream_t)stream);
 }

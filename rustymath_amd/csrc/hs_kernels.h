@@ -84,6 +84,15 @@ int gesv_gmres_ir(int64_t n, const double* a, int64_t lda, const double* b, doub
 int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
                  const int64_t* p_off, bool f64, void* a, void* b, int64_t* perm, int* regular, int mode,
                  int panel_width, int wgs_per_cu, long long* handle);
+// The FP32-LU + FP64-refinement form of a plan over the same systems: a (f64) and b (f64) are read, never written;
+// a32 (Σ n² FP32 items, a's offsets) gets float32(A_i)'s factors, r32 (Σ n k, b's offsets) is the FP32 solves'
+// workspace, x (f64, b's offsets) the refined solutions; per system, perm, regular, the corrections applied (iters)
+// and the final backward error (berr). A system stops once ||B - A X||_F / (||A||_F ||X||_F) <= tol_per_n n 2^-52 or
+// after max_iter corrections; its test runs on the GPU. Runs and frees as batched_run / batched_free.
+int batched_ir_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
+                    const int64_t* p_off, const double* a, const double* b, double* x, float* a32, float* r32,
+                    int64_t* perm, int* regular, int* iters, double* berr, int max_iter, double tol_per_n,
+                    int panel_width, int wgs_per_cu, long long* handle);
 int batched_run(long long handle, bool use_graph, hipStream_t s);
 int batched_free(long long handle);
 // The batched kernel's per-phase profile (workgroup 0, 100 MHz ticks): on = 1 clears and enables (runs then

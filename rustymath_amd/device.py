@@ -225,6 +225,72 @@ class BatchedSolve:
         return self.a[o:o + n * n].view(n, n), self.perm[po:po + n]
 
 
+class BatchedMixedSolve:
+    """The FP32-LU + FP64-refinement form of BatchedSolve: many float64 systems A_i X_i = B_i (n_i <= 512, k_i <= 8),
+    each factored in FP32 on MFMA (float32(A_i), bit for bit rustymath's f32 `lu_factor`) and refined in FP64 on the
+    GPU until ||B_i - A_i X_i||_F / (||A_i||_F ||X_i||_F) <= tol_per_n · n_i · 2^-52 or max_iter corrections have run;
+    the convergence test runs on the GPU. One run is one sequence of persistent launches, replayed from a hipGraph.
+
+    `a` and `b` are the flat float64 device buffers of BatchedSolve's layout; they are read, never written. After a run:
+    `x(i)` is X_i, `iters[i]` the corrections applied, `berr[i]` the final backward error, `regular[i]` the
+    reference's `regular` decision on the FP32 factors (NaN error and X left at zero where it is 0), and
+    `factors32(i)` float32(A_i)'s factors with perm_i (row perm_i[j] holds position j)."""
+
+    def __init__(self, a, b, ns, ks, max_iter=10, tol_per_n=10.0, panel_width=32, wgs_per_cu=1):
+        import ctypes
+        import numpy as np
+        if not (a.is_cuda and b.is_cuda): raise ValueError('batched buffers must be GPU tensors')
+        if a.dtype != torch.float64 or b.dtype != torch.float64: raise TypeError('a and b must be float64')
+        if not (a.is_contiguous() and b.is_contiguous()): raise ValueError('batched buffers must be contiguous')
+        ns = np.ascontiguousarray(ns, dtype=np.int64); ks = np.ascontiguousarray(ks, dtype=np.int64)
+        if ns.shape != ks.shape or ns.ndim != 1: raise ValueError('ns and ks: one order and one column count per system')
+        self.ns, self.ks = ns, ks
+        offsets = lambda sizes: np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if len(ns) else ns.copy()
+        self.a_off, self.b_off, self.p_off = offsets(ns * ns), offsets(ns * ks), offsets(ns)
+        if a.numel() < int((ns * ns).sum()) or b.numel() < int((ns * ks).sum()):
+            raise ValueError('the buffers are smaller than the systems they should hold')
+        dev = a.device
+        self.a, self.b = a, b
+        self.xbuf = torch.zeros(max(b.numel(), 1), dtype=torch.float64, device=dev)
+        self.a32 = torch.empty(max(a.numel(), 1), dtype=torch.float32, device=dev)
+        self.r32 = torch.empty(max(b.numel(), 1), dtype=torch.float32, device=dev)
+        self.perm = torch.empty(max(int(ns.sum()), 1), dtype=torch.int64, device=dev)
+        self.regular = torch.zeros(max(len(ns), 1), dtype=torch.int32, device=dev)
+        self.iters = torch.zeros(max(len(ns), 1), dtype=torch.int32, device=dev)
+        self.berr = torch.zeros(max(len(ns), 1), dtype=torch.float64, device=dev)
+        h = ctypes.c_int64(0)
+        p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+        _lib.check(_lib.load().hs_dev_dsgesv_batched_plan(
+            len(ns), p(ns), p(ks), p(self.a_off), p(self.b_off), p(self.p_off), a.data_ptr(), b.data_ptr(),
+            self.xbuf.data_ptr(), self.a32.data_ptr(), self.r32.data_ptr(), self.perm.data_ptr(),
+            self.regular.data_ptr(), self.iters.data_ptr(), self.berr.data_ptr(), max_iter, tol_per_n, panel_width,
+            wgs_per_cu, ctypes.byref(h)), 'dsgesv_batched_plan')
+        self.handle = h.value
+
+    def run(self, graph=True):
+        "Solves and refines every system on the current stream (asynchronous)."
+        if self.handle is None: raise ValueError('the plan is closed')
+        _lib.check(_lib.load().hs_dev_gesv_batched_run(self.handle, 1 if graph else 0, _stream()), 'gesv_batched_run')
+        return self
+
+    def close(self):
+        if self.handle is not None:
+            _lib.check(_lib.load().hs_dev_gesv_batched_free(self.handle), 'gesv_batched_free')
+            self.handle = None
+
+    def __del__(self):
+        try: self.close()
+        except Exception: pass
+
+    def x(self, i):
+        n, k, o = int(self.ns[i]), int(self.ks[i]), int(self.b_off[i])
+        return self.xbuf[o:o + n * k].view(n, k)
+
+    def factors32(self, i):
+        n, o, po = int(self.ns[i]), int(self.a_off[i]), int(self.p_off[i])
+        return self.a32[o:o + n * n].view(n, n), self.perm[po:po + n]
+
+
 def pack(mats, rhs, dtype=torch.float64, device='cuda'):
     "Flat device buffers for BatchedSolve from lists of host (numpy) square matrices and right-hand sides (n×k)."
     import numpy as np

@@ -219,3 +219,72 @@ def test_factors_are_reused_for_solves_and_transposed_solves(dv, dtype):
         scale = np.linalg.norm(mats[i].astype(np.float64)) * np.linalg.norm(y.cpu().numpy())
         assert r <= 10 * n * np.finfo(dtype).eps * scale
     for p in (plan, again, trans): p.close()
+
+
+def test_fp32_lu_with_fp64_refinement_on_the_full_workload(dv):
+    """The second form of the batched workload (10,000 float64 systems, n uniform in [8, 512], k in [1, 8],
+    A = G + sqrt(n) I, seed 3): FP32 LU on MFMA and FP64 refinement on the GPU. Every system is regular and ends with
+    a normwise backward error ||B - A X||_F / (||A||_F ||X||_F) at most 10 n eps64, recomputed here in FP64 from the
+    original float64 A and B; a graph replay gives the same bits; and the FP32 factors of a sample are the oracle's
+    FP32 `lu` of float32(A_i), bit for bit (rustymath's own f32 factorization)."""
+    import torch
+    from bench import batched_workload
+    a0, b0, ns, ks = batched_workload()
+    plan = dv.BatchedMixedSolve(a0, b0, ns, ks)
+    plan.run(graph=False)
+    torch.cuda.synchronize()
+    assert bool((plan.regular[:len(ns)] == 1).all())
+    berr = plan.berr[:len(ns)].cpu().numpy()
+    iters = plan.iters[:len(ns)].cpu().numpy()
+    eps = np.finfo(np.float64).eps
+    assert (berr <= 10 * ns * eps).all(), float((berr / (10 * ns * eps)).max())
+    assert 1 <= iters.max() <= 10
+    worst = 0.0
+    for n in np.unique(ns):
+        idx = np.nonzero(ns == n)[0]
+        for k in np.unique(ks[idx]):
+            sel = idx[ks[idx] == k]
+            A = torch.stack([a0[plan.a_off[i]:plan.a_off[i] + n * n].view(n, n) for i in sel])
+            B = torch.stack([b0[plan.b_off[i]:plan.b_off[i] + n * k].view(n, k) for i in sel])
+            X = torch.stack([plan.x(i) for i in sel])
+            r = (B - torch.bmm(A, X.contiguous())).flatten(1).norm(dim=1)
+            e = (r / (A.flatten(1).norm(dim=1) * X.flatten(1).norm(dim=1))).cpu().numpy()
+            assert (e <= 10 * n * eps).all(), (n, k, float(e.max()))
+            worst = max(worst, float((e / (10 * n * eps)).max()))
+    x1 = plan.xbuf.clone()
+    plan.run(graph=True); plan.run(graph=True)
+    torch.cuda.synchronize()
+    assert bits_equal(plan.xbuf.cpu().numpy(), x1.cpu().numpy())
+    for i in range(0, len(ns), 997):
+        n = int(ns[i])
+        f32, perm = plan.factors32(i)
+        a32 = a0[plan.a_off[i]:plan.a_off[i] + n * n].view(n, n).float().cpu().numpy()
+        wf, wperm, _ = oracle.lu(a32)
+        assert (perm.cpu().numpy() == wperm).all(), f'cell {i}: pivots'
+        assert bits_equal(f32.cpu().numpy()[wperm], wf.T), f'cell {i}: factors'
+    plan.close()
+    print(f'worst backward error / (10 n eps) = {worst:.3f}, corrections {np.bincount(iters)}')
+
+
+def test_refinement_stops_at_its_cap_and_skips_singular_systems(dv):
+    """max_iter = 0 applies no correction (the FP32 solve alone, its backward error reported); a singular system is
+    marked not regular, with a NaN error and X left at zero, while the others in the batch converge."""
+    import torch
+    rng = np.random.default_rng(9)
+    mats = [rng.standard_normal((n, n)) + np.sqrt(n) * np.eye(n) for n in (5, 100, 300)]
+    mats.append(np.ones((50, 50)))
+    rhs = [rng.standard_normal((m.shape[0], 3)) for m in mats]
+    a, b, ns, ks = dv.pack(mats, rhs)
+    p0 = dv.BatchedMixedSolve(a, b, ns, ks, max_iter=0)
+    p0.run(); torch.cuda.synchronize()
+    assert (p0.iters[:4].cpu().numpy() == 0).all()
+    assert (p0.berr[:3].cpu().numpy() > 10 * np.array(ns[:3]) * np.finfo(np.float64).eps).any()
+    p0.close()
+    p = dv.BatchedMixedSolve(a, b, ns, ks)
+    p.run(); torch.cuda.synchronize()
+    reg = p.regular[:4].cpu().numpy()
+    assert list(reg) == [1, 1, 1, 0]
+    berr = p.berr[:4].cpu().numpy()
+    assert (berr[:3] <= 10 * np.array(ns[:3]) * np.finfo(np.float64).eps).all() and np.isnan(berr[3])
+    assert not p.x(3).abs().sum().item()
+    p.close()

@@ -55,6 +55,17 @@ int hs_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, 
 int hs_sgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
              const float* b, int64_t ldb, float beta, float* c, int64_t ldc);
 
+/* `batch` products at once, c_i = alpha op(a_i) op(b_i) + beta c_i with cell i at a + i·sa, b + i·sb, c + i·sc (items;
+ * a stride of 0 repeats an operand; cells of c may not overlap, else -16): a stack's `Inner::inner` over its frames
+ * (src/linalg/stacked.rs over core/src/linalg.rs:118-122), each cell exactly as hs_dgemm computes it, in one launch.
+ * Strides are arguments 9, 12 and 16, batch 17. */
+int hs_dgemm_batched(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                     int64_t sa, const double* b, int64_t ldb, int64_t sb, double beta, double* c, int64_t ldc,
+                     int64_t sc, int64_t batch);
+int hs_sgemm_batched(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
+                     int64_t sa, const float* b, int64_t ldb, int64_t sb, float beta, float* c, int64_t ldc,
+                     int64_t sc, int64_t batch);
+
 /* Triangular solves in place on x (row-major m×n, ld ldx): op(A) X = B (side 141, A m×m) or X op(A) = B (side 142,
  * A n×n), A row-major with ld lda, of which only the triangle `uplo` names is read (121 upper, 122 lower; diag 131
  * non-unit, 132 unit diagonal taken as ones); trans 111 / 112 / 113 (113 is 112 for real items).
@@ -126,6 +137,13 @@ int hs_dev_dtrsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, 
                  int64_t ldx, void* stream);
 int hs_dev_strsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const float* a, int64_t lda, float* x,
                  int64_t ldx, void* stream);
+/* hs_dgemm_batched on device memory and `stream` (asynchronous, as hs_dev_dgemm). */
+int hs_dev_dgemm_batched(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a,
+                         int64_t lda, int64_t sa, const double* b, int64_t ldb, int64_t sb, double beta, double* c,
+                         int64_t ldc, int64_t sc, int64_t batch, void* stream);
+int hs_dev_sgemm_batched(int op_a, int op_b, int64_t m, int64_t n, int64_t k, float alpha, const float* a,
+                         int64_t lda, int64_t sa, const float* b, int64_t ldb, int64_t sb, float beta, float* c,
+                         int64_t ldc, int64_t sc, int64_t batch, void* stream);
 /* Factors the row-major a (ld lda) in place into column-major factors (same buffer, same ld) and writes
  * perm (device, n int64) and, where not NULL, *info (host) as hs_dlu_factor gives it. Synchronises `stream` before
  * returning. */

@@ -21,6 +21,14 @@ SIGNATURES = {
     'hs_last_error': (ctypes.c_char_p, []),
     'hs_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
     'hs_sgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64]),
+    'hs_dgemm_batched': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _i64, _vp, _i64, _i64, _d, _vp, _i64, _i64,
+                                _i64]),
+    'hs_sgemm_batched': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _i64, _vp, _i64, _i64, _f, _vp, _i64, _i64,
+                                _i64]),
+    'hs_dev_dgemm_batched': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _i64, _vp, _i64, _i64, _d, _vp, _i64,
+                                    _i64, _i64, _vp]),
+    'hs_dev_sgemm_batched': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _i64, _vp, _i64, _i64, _f, _vp, _i64,
+                                    _i64, _i64, _vp]),
     'hs_dtrsm': (_i32, [_i32, _i32, _i32, _i32, _i64, _i64, _vp, _i64, _vp, _i64]),
     'hs_strsm': (_i32, [_i32, _i32, _i32, _i32, _i64, _i64, _vp, _i64, _vp, _i64]),
     'hs_dev_dtrsm': (_i32, [_i32, _i32, _i32, _i32, _i64, _i64, _vp, _i64, _vp, _i64, _vp]),

@@ -7,6 +7,8 @@ the standard adjoints, built so that the backward pass reuses what the forward p
                     the inverse row permutation; no second factorization), and Ā = −B̄ Xᵀ, one MFMA product;
   C = matmul(A, B): Ā = C̄ Bᵀ and B̄ = Aᵀ C̄, MFMA products with transposed operands read in place.
 
+A stack's products (forward and backward) are one batched launch each.
+
 Forward results are the library's (bit for bit rustymath's `lu_factor`/`lu_solve` and product folds). Inputs are
 float64 or float32 CUDA tensors: A (n×n) with B (n×k) or (n,), or stacks of them (batch×n×n with batch×n×k or
 batch×n). A stack of systems with n <= 512 is factored and solved by the batched kernel, one workgroup per system;
@@ -100,10 +102,8 @@ class Solve(torch.autograd.Function):
         g = g.unsqueeze(-1) if ctx.vec else g
         gb = _solve_transposed(fs, perms, g, ctx.layout)
         ga = None
-        if ctx.needs_input_grad[0]:
-            ga = torch.empty(fs.shape, dtype=x.dtype, device=x.device)
-            for i in range(x.shape[0]):
-                dv.gemm(gb[i], x[i], ga[i], alpha=-1.0, beta=0.0, trans_b=True)
+        if ctx.needs_input_grad[0]:  # Ā = −B̄ Xᵀ for every system, one launch
+            ga = dv.gemm_batched(gb.contiguous(), x.contiguous(), alpha=-1.0, trans_b=True)
             ga = ga if ctx.stacked else ga[0]
         gb = gb.squeeze(-1) if ctx.vec else gb
         gb = gb if ctx.stacked else gb[0]
@@ -118,7 +118,7 @@ class Matmul(torch.autograd.Function):
         if a.dim() != b.dim() or a.dim() not in (2, 3): raise ValueError('matmul takes two matrices or two stacks')
         ctx.save_for_backward(a, b)
         if a.dim() == 2: return dv.gemm(a.contiguous(), b.contiguous())
-        return torch.stack([dv.gemm(a[i].contiguous(), b[i].contiguous()) for i in range(a.shape[0])])
+        return dv.gemm_batched(a.contiguous(), b.contiguous())
 
     @staticmethod
     def backward(ctx, gc):
@@ -129,10 +129,8 @@ class Matmul(torch.autograd.Function):
             if ctx.needs_input_grad[0]: ga = dv.gemm(gc, b.contiguous(), trans_b=True)
             if ctx.needs_input_grad[1]: gb = dv.gemm(a.contiguous(), gc, trans_a=True)
             return ga, gb
-        if ctx.needs_input_grad[0]:
-            ga = torch.stack([dv.gemm(gc[i], b[i].contiguous(), trans_b=True) for i in range(a.shape[0])])
-        if ctx.needs_input_grad[1]:
-            gb = torch.stack([dv.gemm(a[i].contiguous(), gc[i], trans_a=True) for i in range(a.shape[0])])
+        if ctx.needs_input_grad[0]: ga = dv.gemm_batched(gc, b.contiguous(), trans_b=True)
+        if ctx.needs_input_grad[1]: gb = dv.gemm_batched(a.contiguous(), gc, trans_a=True)
         return ga, gb
 
 

@@ -221,41 +221,56 @@ int dev_trsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, cons
     return trsm<T>(right, lower, ta, unit, m, n, a, lda, x, ldx, ldx == 1, (hipStream_t)stream);
 }
 
+// The items a row-major r×c matrix with leading dimension ld spans from its first item: what one 1-D copy moves.
+int64_t extent(int64_t r, int64_t c, int64_t ld) { return r > 0 && c > 0 ? (r - 1) * ld + c : 0; }
+
+// `batch` products c_i = alpha op(a_i) op(b_i) + beta c_i from host memory, cell i at a + i sa, b + i sb, c + i sc
+// (a stride of 0 repeats an operand): each operand's whole span goes to the device in one 1-D copy, leading dimensions
+// kept, and c's span comes back the same way.
+template <typename T>
+int host_gemm_batched(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, int64_t sa,
+                      const T* B, int64_t ldb, int64_t sb, T beta, T* C, int64_t ldc, int64_t sc, int64_t batch) {
+    bool ta, tb;
+    int e = check_gemm<T>(opa, opb, m, n, k, lda, ldb, ldc, ta, tb);
+    if (e) return e == -10 ? -11 : (e == -13 ? -15 : e);  // ldb and ldc sit after the strides here
+    if (batch < 0) return -17;
+    if (sa < 0 || sb < 0 || sc < 0) return sa < 0 ? -9 : (sb < 0 ? -12 : -16);
+    if (m == 0 || n == 0 || batch == 0) return OK;
+    const int64_t ar = ta ? k : m, ac = ta ? m : k, br = tb ? n : k, bc = tb ? k : n;
+    const int64_t ea = extent(ar, ac, lda), eb = extent(br, bc, ldb), ec = extent(m, n, ldc);
+    if (batch > 1 && sc < ec) { set_error("batched products: cells of c overlap"); return -16; }
+    const int64_t na = ea ? (batch - 1) * sa + ea : 0, nb = eb ? (batch - 1) * sb + eb : 0, nc = (batch - 1) * sc + ec;
+    hipStream_t s = thread_stream();
+    const bool product = k > 0 && alpha != T(0);
+    T* dA = product ? (T*)pool_get(sizeof(T) * (size_t)(na ? na : 1)) : nullptr;
+    T* dB = product ? (T*)pool_get(sizeof(T) * (size_t)(nb ? nb : 1)) : nullptr;
+    T* dC = (T*)pool_get(sizeof(T) * (size_t)nc);
+    int rc = OK;
+    if (!dC || (product && (!dA || !dB))) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && product && ((na && hipMemcpyAsync(dA, A, sizeof(T) * na, hipMemcpyHostToDevice, s) != hipSuccess) ||
+                           (nb && hipMemcpyAsync(dB, B, sizeof(T) * nb, hipMemcpyHostToDevice, s) != hipSuccess)))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    // c's span goes over where beta is not 0, and where it has items between rows or cells, which must come back as
+    // they were.
+    const bool gaps = (m > 1 && ldc > n) || (batch > 1 && sc > ec);
+    if (!rc && (beta != T(0) || gaps) && hipMemcpyAsync(dC, C, sizeof(T) * nc, hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) rc = gemm_batched<T>(ta, tb, m, n, k, alpha, dA, lda, sa, dB, ldb, sb, beta, dC, ldc, sc, batch, s);
+    if (!rc && (hipMemcpyAsync(C, dC, sizeof(T) * nc, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+        rc = hip_status(hipErrorUnknown, "copy back");
+    hipStreamSynchronize(s);
+    pool_put(dA); pool_put(dB); pool_put(dC);
+    return rc;
+}
+
 template <typename T>
 int host_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
               T beta, T* C, int64_t ldc) {
     bool ta, tb;
-    int e = check_gemm<T>(opa, opb, m, n, k, lda, ldb, ldc, ta, tb);
+    const int e = check_gemm<T>(opa, opb, m, n, k, lda, ldb, ldc, ta, tb);
     if (e) return e;
-    if (m == 0 || n == 0) return OK;
-    hipStream_t s = thread_stream();
-    const int64_t ar = ta ? k : m, ac = ta ? m : k, br = tb ? n : k, bc = tb ? k : n;
-    T *dA = nullptr, *dB = nullptr, *dC = (T*)pool_get(sizeof(T) * (size_t)(m * n));
-    if (!dC) return hip_status(hipErrorOutOfMemory, "pool_get");
-    int rc = OK;
-    if (k > 0 && alpha != T(0)) {
-        dA = (T*)pool_get(sizeof(T) * (size_t)(ar * ac ? ar * ac : 1));
-        dB = (T*)pool_get(sizeof(T) * (size_t)(br * bc ? br * bc : 1));
-        if (!dA || !dB) { rc = hip_status(hipErrorOutOfMemory, "pool_get"); goto done; }
-        if (hipMemcpy2DAsync(dA, ac * sizeof(T), A, lda * sizeof(T), ac * sizeof(T), ar, hipMemcpyHostToDevice, s) != hipSuccess ||
-            hipMemcpy2DAsync(dB, bc * sizeof(T), B, ldb * sizeof(T), bc * sizeof(T), br, hipMemcpyHostToDevice, s) != hipSuccess) {
-            rc = hip_status(hipErrorUnknown, "hipMemcpy2DAsync"); goto done;
-        }
-    }
-    if (beta != T(0)) {
-        if (hipMemcpy2DAsync(dC, n * sizeof(T), C, ldc * sizeof(T), n * sizeof(T), m, hipMemcpyHostToDevice, s) != hipSuccess) {
-            rc = hip_status(hipErrorUnknown, "hipMemcpy2DAsync"); goto done;
-        }
-    }
-    rc = gemm<T>(ta, tb, m, n, k, alpha, dA, ac, dB, bc, beta, dC, n, s);
-    if (rc) goto done;
-    if (hipMemcpy2DAsync(C, ldc * sizeof(T), dC, n * sizeof(T), n * sizeof(T), m, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        rc = hip_status(hipErrorUnknown, "copy back");
-done:
-    hipStreamSynchronize(s);
-    pool_put(dA); pool_put(dB); pool_put(dC);
-    return rc;
+    return host_gemm_batched<T>(opa, opb, m, n, k, alpha, A, lda, 0, B, ldb, 0, beta, C, ldc, 0, 1);
 }
 
 template <typename T>
@@ -534,6 +549,19 @@ int host_stack(int64_t count, int64_t n, int64_t k, const T* a, const T* b, T* x
 }
 
 template <typename T>
+int dev_gemm_batched(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, int64_t sa,
+                     const T* B, int64_t ldb, int64_t sb, T beta, T* C, int64_t ldc, int64_t sc, int64_t batch,
+                     void* stream) {
+    bool ta, tb;
+    int e = check_gemm<T>(opa, opb, m, n, k, lda, ldb, ldc, ta, tb);
+    if (e) return e == -10 ? -11 : (e == -13 ? -15 : e);  // ldb and ldc sit after the strides here
+    if (batch < 0) return -17;
+    if (sa < 0 || sb < 0 || sc < 0) return sa < 0 ? -9 : (sb < 0 ? -12 : -16);
+    if (batch > 1 && m > 0 && n > 0 && sc < extent(m, n, ldc)) { set_error("batched products: cells of c overlap"); return -16; }
+    return gemm_batched<T>(ta, tb, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, batch, (hipStream_t)stream);
+}
+
+template <typename T>
 int dev_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, const T* B, int64_t ldb,
              T beta, T* C, int64_t ldc, void* stream) {
     bool ta, tb;
@@ -572,6 +600,26 @@ int hs_dgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, double alpha, co
 int hs_sgemm(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda, const float* b,
              int64_t ldb, float beta, float* c, int64_t ldc) {
     return host_gemm<float>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+int hs_dgemm_batched(int opa, int opb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                     int64_t sa, const double* b, int64_t ldb, int64_t sb, double beta, double* c, int64_t ldc, int64_t sc,
+                     int64_t batch) {
+    return host_gemm_batched<double>(opa, opb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch);
+}
+int hs_sgemm_batched(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
+                     int64_t sa, const float* b, int64_t ldb, int64_t sb, float beta, float* c, int64_t ldc, int64_t sc,
+                     int64_t batch) {
+    return host_gemm_batched<float>(opa, opb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch);
+}
+int hs_dev_dgemm_batched(int opa, int opb, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,
+                         int64_t sa, const double* b, int64_t ldb, int64_t sb, double beta, double* c, int64_t ldc,
+                         int64_t sc, int64_t batch, void* stream) {
+    return dev_gemm_batched<double>(opa, opb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch, stream);
+}
+int hs_dev_sgemm_batched(int opa, int opb, int64_t m, int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
+                         int64_t sa, const float* b, int64_t ldb, int64_t sb, float beta, float* c, int64_t ldc,
+                         int64_t sc, int64_t batch, void* stream) {
+    return dev_gemm_batched<float>(opa, opb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch, stream);
 }
 int hs_dtrsm(int side, int uplo, int trans, int diag, int64_t m, int64_t n, const double* a, int64_t lda, double* x,
              int64_t ldx) {

@@ -11,6 +11,7 @@
 #include <hip/hip_ext.h>
 #include <mutex>
 #include <vector>
+#include <algorithm>
 
 namespace hs {
 
@@ -64,16 +65,21 @@ __device__ __forceinline__ bool tile_of(int w, int tiles_m, int tiles_n, int& tm
     return true;
 }
 
+// blockIdx.y is the cell of a batch: A, B and C advance by sa, sb and sc items per cell (0 repeats an operand).
 template <typename T, bool TA, bool TB>
 __global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, int64_t k, T alpha,
                                                          const T* __restrict__ A, int64_t lda,
                                                          const T* __restrict__ B, int64_t ldb, T beta,
-                                                         T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n) {
+                                                         T* __restrict__ C, int64_t ldc, int tiles_m, int tiles_n,
+                                                         int64_t sa, int64_t sb, int64_t sc) {
     constexpr int P = Pad<T>::v;
     __shared__ T As[2][BK][BM + P];
     __shared__ T Bs[2][BK][BN + P];
     int tm, tn;
     if (!tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn)) return;
+    A += blockIdx.y * sa;
+    B += blockIdx.y * sb;
+    C += blockIdx.y * sc;
     const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
     if (m0 + BM <= m && n0 + BN <= n)
         gemm_tile<T, TA, TB, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, m0, n0, As, Bs, threadIdx.x, true);
@@ -83,9 +89,10 @@ __global__ __launch_bounds__(THREADS, 2) void gemm_mfma(int64_t m, int64_t n, in
 
 // `scale` (core/src/linalg.rs:1000-1008): c = beta c, zeros for beta = 0.
 template <typename T>
-__global__ void scale_kernel(int64_t m, int64_t n, T beta, T* C, int64_t ldc) {
+__global__ void scale_kernel(int64_t m, int64_t n, T beta, T* C, int64_t ldc, int64_t sc) {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= m * n) return;
+    C += blockIdx.y * sc;
     int64_t i = idx / n, j = idx % n;
     T* x = &C[i * ldc + j];
     *x = beta == T(0) ? T(0) : beta * *x;
@@ -99,9 +106,13 @@ __global__ void scale_kernel(int64_t m, int64_t n, T beta, T* C, int64_t ldc) {
 // matrix slice's length from its start, which `narrow_sums` reads.
 template <typename T>
 __global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ mat, int64_t ld, int mtrans, int64_t mlen,
-                                 const T* __restrict__ vec, int64_t vstride, T alpha, T beta, T* C, int64_t step) {
+                                 const T* __restrict__ vec, int64_t vstride, T alpha, T beta, T* C, int64_t step,
+                                 int64_t smat, int64_t svec, int64_t sc) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rows) return;
+    mat += blockIdx.y * smat;
+    vec += blockIdx.y * svec;
+    C += blockIdx.y * sc;
     T y;
     if (!mtrans) {
         const T* row = mat + i * ld;
@@ -151,31 +162,36 @@ __global__ void by_vector_kernel(int64_t rows, int64_t k, const T* __restrict__ 
 
 }  // namespace
 
+namespace {
+// One launch of `batch` (<= 65535) cells.
 template <typename T>
-int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
-         T beta, T* c, int64_t ldc, hipStream_t s, bool in_order) {
-    if (m == 0 || n == 0) return OK;
+int gemm_cells(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, int64_t sa,
+               const T* b, int64_t ldb, int64_t sb, T beta, T* c, int64_t ldc, int64_t sc, int batch, hipStream_t s,
+               bool in_order) {
+    if (m == 0 || n == 0 || batch == 0) return OK;
     if (k == 0 || alpha == T(0)) {
         if (beta == T(1)) return OK;
         int64_t total = m * n;
-        scale_kernel<T><<<(unsigned)((total + 255) / 256), 256, 0, s>>>(m, n, beta, c, ldc);
+        scale_kernel<T><<<dim3((unsigned)((total + 255) / 256), (unsigned)batch), 256, 0, s>>>(m, n, beta, c, ldc, sc);
         HS_HIP(hipGetLastError());
         return OK;
     }
     if ((n == 1 || m == 1) && !in_order) {
         // As by_vector: for n = 1 the matrix is a and the vector b's column; for m = 1 the matrix is bᵀ.
-        const T* mat; int64_t ld, rows, step, vstride, mlen; int mtrans; const T* vec;
+        const T* mat; int64_t ld, rows, step, vstride, mlen, smat, svec; int mtrans; const T* vec;
         if (n == 1) {
             mat = a; ld = lda; mtrans = ta; rows = m; step = ldc;
             vec = b; vstride = tb ? 1 : ldb;
             mlen = ta ? (k - 1) * lda + m : (m - 1) * lda + k;
+            smat = sa; svec = sb;
         } else {
             mat = b; ld = ldb; mtrans = !tb; rows = n; step = 1;
             vec = a; vstride = ta ? lda : 1;
             mlen = tb ? (n - 1) * ldb + k : (k - 1) * ldb + n;
+            smat = sb; svec = sa;
         }
-        by_vector_kernel<T><<<(unsigned)((rows + 255) / 256), 256, 0, s>>>(rows, k, mat, ld, mtrans, mlen, vec, vstride,
-                                                                          alpha, beta, c, step);
+        by_vector_kernel<T><<<dim3((unsigned)((rows + 255) / 256), (unsigned)batch), 256, 0, s>>>(
+            rows, k, mat, ld, mtrans, mlen, vec, vstride, alpha, beta, c, step, smat, svec, sc);
         HS_HIP(hipGetLastError());
         return OK;
     }
@@ -187,15 +203,15 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
 #define HS_LAUNCH(TA_, TB_)                                                                                        \
     do {                                                                                                           \
         if (profiled)                                                                                              \
-            hipExtLaunchKernelGGL(gemm_mfma<T, TA_, TB_>, dim3(grid), dim3(THREADS), 0, s, rec.a, rec.b, 0, m, n, k, \
-                                  alpha, a, lda, b, ldb, beta, c, ldc, tiles_m, tiles_n);                          \
+            hipExtLaunchKernelGGL(gemm_mfma<T, TA_, TB_>, dim3(grid, batch), dim3(THREADS), 0, s, rec.a, rec.b, 0,  \
+                                  m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tiles_m, tiles_n, sa, sb, sc);     \
         else                                                                                                       \
-            gemm_mfma<T, TA_, TB_><<<grid, THREADS, 0, s>>>(m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, tiles_m,   \
-                                                            tiles_n);                                              \
+            gemm_mfma<T, TA_, TB_><<<dim3(grid, batch), THREADS, 0, s>>>(m, n, k, alpha, a, lda, b, ldb, beta, c,     \
+                                                                       ldc, tiles_m, tiles_n, sa, sb, sc);         \
     } while (0)
     // Algorithmic bytes: a and b read once, c written once (and read once where beta is not 0).
-    GemmProfile::Rec rec{nullptr, nullptr, 2.0 * (double)m * (double)n * (double)k,
-                         (double)sizeof(T) * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n)};
+    GemmProfile::Rec rec{nullptr, nullptr, 2.0 * batch * (double)m * (double)n * (double)k,
+                         (double)sizeof(T) * batch * ((double)m * k + (double)k * n + (beta != T(0) ? 2.0 : 1.0) * m * n)};
     bool profiled = false;
     {
         std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -217,6 +233,31 @@ int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a,
     }
     return OK;
 }
+}  // namespace
+
+template <typename T>
+int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, const T* b, int64_t ldb,
+         T beta, T* c, int64_t ldc, hipStream_t s, bool in_order) {
+    return gemm_cells<T>(ta, tb, m, n, k, alpha, a, lda, 0, b, ldb, 0, beta, c, ldc, 0, 1, s, in_order);
+}
+
+template <typename T>
+int gemm_batched(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, int64_t sa,
+                 const T* b, int64_t ldb, int64_t sb, T beta, T* c, int64_t ldc, int64_t sc, int64_t batch,
+                 hipStream_t s) {
+    for (int64_t c0 = 0; c0 < batch; c0 += 65535) {
+        const int cells = (int)std::min<int64_t>(65535, batch - c0);
+        const int rc = gemm_cells<T>(ta, tb, m, n, k, alpha, a + c0 * sa, lda, sa, b + c0 * sb, ldb, sb, beta,
+                                     c + c0 * sc, ldc, sc, cells, s, false);
+        if (rc) return rc;
+    }
+    return OK;
+}
+template int gemm_batched<double>(bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, int64_t,
+                                  const double*, int64_t, int64_t, double, double*, int64_t, int64_t, int64_t,
+                                  hipStream_t);
+template int gemm_batched<float>(bool, bool, int64_t, int64_t, int64_t, float, const float*, int64_t, int64_t,
+                                 const float*, int64_t, int64_t, float, float*, int64_t, int64_t, int64_t, hipStream_t);
 
 template int gemm<double>(bool, bool, int64_t, int64_t, int64_t, double, const double*, int64_t, const double*, int64_t,
                           double, double*, int64_t, hipStream_t, bool);

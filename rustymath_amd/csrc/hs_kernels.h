@@ -15,6 +15,13 @@ template <typename T>
 int gemm(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda,
          const T* b, int64_t ldb, T beta, T* c, int64_t ldc, hipStream_t s, bool in_order = false);
 
+// `gemm` for `batch` cells at once (a stack's products, one launch): cell i takes a + i sa, b + i sb and c + i sc
+// (a stride of 0 repeats an operand); every cell's routing and arithmetic are gemm's.
+template <typename T>
+int gemm_batched(bool ta, bool tb, int64_t m, int64_t n, int64_t k, T alpha, const T* a, int64_t lda, int64_t sa,
+                 const T* b, int64_t ldb, int64_t sb, T beta, T* c, int64_t ldc, int64_t sc, int64_t batch,
+                 hipStream_t s);
+
 // BF16-input, FP32-accumulate product for the low-precision factorization: c (f32) = c - a b with a, b rounded to
 // bfloat16 once (terms = 1) or split into bfloat16 pairs hi + lo with three products (terms = 3), on MFMA 16x16x32 bf16.
 int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t lda, const float* b, int64_t ldb,

@@ -41,6 +41,34 @@ def gemm(a, b, c=None, alpha=1.0, beta=0.0, trans_a=False, trans_b=False):
     return c
 
 
+def gemm_batched(a, b, c=None, alpha=1.0, beta=0.0, trans_a=False, trans_b=False):
+    """c_i = alpha op(a_i) op(b_i) + beta c_i for every cell of a stack in one launch, each cell exactly as `gemm`
+    computes it. a, b and c are 3-D (cells, rows, columns) with unit column stride; a 2-D a or b is used for every cell.
+    Returns c (a new stack where c is None)."""
+    def cells(x, name):
+        if not x.is_cuda or x.stride(-1) != 1 or x.dim() not in (2, 3):
+            raise ValueError(f'{name} must be a 2-D or 3-D GPU tensor with unit column stride')
+        return (x, 0) if x.dim() == 2 else (x, x.stride(0))
+    (a, sa), (b, sb) = cells(a, 'a'), cells(b, 'b')
+    batch = max(a.shape[0] if a.dim() == 3 else 1, b.shape[0] if b.dim() == 3 else 1)
+    for x in (a, b):
+        if x.dim() == 3 and x.shape[0] != batch: raise ValueError('a and b must hold as many cells')
+    ar, ac = a.shape[-2:]; br, bc = b.shape[-2:]
+    m, k = (ac, ar) if trans_a else (ar, ac)
+    kb, n = (bc, br) if trans_b else (br, bc)
+    if k != kb: raise ValueError(f'gemm contracts {k} items with {kb}')
+    if c is None:
+        c = torch.empty((batch, m, n), dtype=a.dtype, device=a.device)
+        beta = 0.0
+    if c.dim() != 3 or c.stride(-1) != 1 or tuple(c.shape) != (batch, m, n): raise ValueError('c must be cells x m x n')
+    f = getattr(_lib.load(), f'hs_dev_{_sfx(a.dtype)}gemm_batched')
+    ld = lambda x: max(x.stride(-2), 1)
+    _lib.check(f(_lib.TRANS if trans_a else _lib.NO_TRANS, _lib.TRANS if trans_b else _lib.NO_TRANS, m, n, k, alpha,
+                 a.data_ptr(), ld(a), sa, b.data_ptr(), ld(b), sb, beta, c.data_ptr(), ld(c), c.stride(0), batch,
+                 _stream()), 'gemm_batched')
+    return c
+
+
 def getrf(a, info=False):
     """Factors the square row-major tensor `a` in place into rustymath's column-major packed factors and returns
     (a, perm) with perm an int64 GPU tensor (core/src/linalg/factor.rs:617-630); with `info`, (a, perm, info) where

@@ -51,9 +51,39 @@ def _cells(x, frame, core):
 def _sfx(dt): return 'd' if dt == np.float64 else 's'
 
 
+def _operand(x, fx, frame, count, rows, cols):
+    """(array, op code, leading dimension, cell stride) through which the product reads the [*fx, rows, cols] array x
+    over `frame`, as `Mapped::matrix` does (core/src/linalg.rs:658-670): in place where every cell is positive-strided
+    with one unit stride (row by row, or column by column as a transpose) and the cells lie a fixed stride apart, one
+    matrix broadcast over every cell with stride 0; any other layout as a compact copy of its cells."""
+    item = x.itemsize
+    cells = None
+    if all(f == 1 for f in fx):
+        cells, stride = x.reshape(rows, cols), 0
+    elif tuple(fx) == tuple(frame):
+        try:
+            cells = x.view(); cells.shape = (count, rows, cols)  # a view, or AttributeError where a copy would be needed
+            stride = cells.strides[0] // item if count > 1 else 0
+            if cells.strides[0] % item or stride < 0: cells = None
+        except AttributeError:
+            cells = None
+    if cells is not None:
+        s0, s1 = cells.strides[-2], cells.strides[-1]
+        if s0 % item == 0 and s1 % item == 0:
+            s0, s1 = s0 // item, s1 // item
+            if (cols == 1 or s1 == 1) and (rows == 1 or s0 >= cols) and s0 >= 0:
+                return cells, _lib.NO_TRANS, (s0 if rows > 1 else max(cols, 1)), stride
+            if (rows == 1 or s0 == 1) and (cols == 1 or s1 >= rows) and s1 >= 0:
+                return cells, _lib.TRANS, (s1 if cols > 1 else max(rows, 1)), stride
+    dense = _cells(x, frame, (rows, cols))
+    return dense, _lib.NO_TRANS, max(cols, 1), rows * cols
+
+
 def matmul(x1, x2):
     """The matrix product of x1 and x2 (py/src/linalg.rs:241-247, the `+.×` inner product). A vector on the left
-    acts as one row and on the right as one column, and the result leaves that axis out."""
+    acts as one row and on the right as one column, and the result leaves that axis out. A stack's products run as one
+    batched launch; operands whose cells are positive-strided with one unit stride (transposed views included) are read
+    where they lie, as rustymath's `Mapped::matrix` reads them (core/src/linalg.rs:658-670)."""
     a, b = np.asarray(x1), np.asarray(x2)
     if np.issubdtype(np.result_type(a, b), np.integer) or np.result_type(a, b) == np.bool_:
         raise TypeError('integer and Boolean products (int64 `+.×`) are outside this library')
@@ -66,14 +96,14 @@ def matmul(x1, x2):
     kb, n = b2.shape[-2:]
     if k != kb: raise ValueError(f'matmul contracts {k} items with {kb}')
     fa, fb, frame = _pair_frames(a2.shape[:-2], b2.shape[:-2])
-    ac = _cells(a2.reshape(fa + (m, k)).astype(compute, copy=False), frame, (m, k))
-    bc = _cells(b2.reshape(fb + (k, n)).astype(compute, copy=False), frame, (k, n))
-    out = np.zeros((ac.shape[0], m, n), compute)
-    lib = _lib.load()
-    f = getattr(lib, 'hs_' + _sfx(compute) + 'gemm')
-    for i in range(ac.shape[0]):
-        _lib.check(f(_lib.NO_TRANS, _lib.NO_TRANS, m, n, k, 1.0, _ptr(ac[i]), max(k, 1), _ptr(bc[i]), max(n, 1), 0.0,
-                     _ptr(out[i]), max(n, 1)), 'matmul')
+    count = int(np.prod(frame, dtype=np.int64))
+    am, opa, lda, sa = _operand(a2.reshape(fa + (m, k)).astype(compute, copy=False), fa, frame, count, m, k)
+    bm, opb, ldb, sb = _operand(b2.reshape(fb + (k, n)).astype(compute, copy=False), fb, frame, count, k, n)
+    out = np.zeros((count, m, n), compute)
+    if count:
+        f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'gemm_batched')
+        _lib.check(f(opa, opb, m, n, k, 1.0, _ptr(am), lda, sa, _ptr(bm), ldb, sb, 0.0, _ptr(out), max(n, 1), m * n,
+                     count), 'matmul')
     out = out.reshape(tuple(frame) + (m, n))
     if av: out = out[..., 0, :]
     if bv: out = out[..., 0]

@@ -117,3 +117,16 @@ def test_triangular_solve_rejects_bad_codes_on_the_host():
     assert lib.hs_dtrsm(141, 121, 111, 131, 0, 3, None, 0, None, 3) == 0
     assert lib.hs_dev_dgetrs(0, 4, 1, None, 3, None, None, 1, None) == -5
     assert lib.hs_dev_sgetrs(0, 4, 2, None, 4, None, None, 1, None) == -8
+
+
+def test_batched_products_reject_bad_arguments_on_the_host():
+    from rustymath_amd import _lib
+    lib = _lib.load()
+    assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 4, -1) == -17
+    assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 1, 4, None, 2, 4, 0.0, None, 2, 4, 3) == -8
+    assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 1, 4, 0.0, None, 2, 4, 3) == -11
+    assert lib.hs_sgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 1, 4, 3) == -15
+    assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, -4, None, 2, 4, 0.0, None, 2, 4, 3) == -9
+    assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 3, 3) == -16
+    assert lib.hs_dev_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 1, 3, None) == -16
+    assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 4, 0) == 0

@@ -478,3 +478,57 @@ def test_profiles_account_for_a_factorization(la, dtype):
     assert v[3].value > 0 or v[2].value == 0
     want, wperm, _ = oracle.lu(a)
     assert bits_equal(f.cpu().numpy(), want) and (perm.cpu().numpy() == wperm).all()
+
+
+def test_matrix_layouts_are_read_in_place(la):
+    """tests/test_linalg.py:207-227 on the GPU: reversed, transposed and strided views, and stacks beside a single
+    matrix. Views that are positive-strided with one unit stride are read where they lie (as `Mapped::matrix` reads
+    them, core/src/linalg.rs:658-670), others through a compact copy; the products are NumPy's within rounding, and
+    where both extents exceed 1 (the MFMA product, whose every item is the in-order fold whatever the layout) the
+    oracle's product of compact copies bit for bit, cell by cell."""
+    rng = np.random.default_rng(4)
+    a = rng.normal(size=(64, 8, 8))[:, ::-1, ::-1]
+    b = rng.normal(size=(8, 5))[::-1]
+    got = la.matmul(a, b)
+    assert np.allclose(got, a @ b)
+    for i in range(64):
+        assert bits_equal(got[i], oracle.matmul(np.ascontiguousarray(a[i]), np.ascontiguousarray(b)))
+    s = rng.normal(size=(3, 8, 5))
+    for dtype in (np.float32, np.float64):
+        m, st = b.astype(dtype), s.astype(dtype)
+        for view in (m.T, m[::-1], m[:, ::2]):
+            dense = np.ascontiguousarray(view)
+            p = la.matmul(view.T, view)
+            assert np.allclose(p, dense.T @ dense, rtol=1e-5, atol=1e-6)
+            assert bits_equal(p, oracle.matmul(np.ascontiguousarray(dense.T), dense))
+        for p, q in [(la.matmul(m[:, 1], m), b[:, 1] @ b), (la.matmul(m.T, m[:, 1]), b.T @ b[:, 1]),
+                     (la.matmul(st.swapaxes(-1, -2), st), s.swapaxes(-1, -2) @ s),
+                     (la.matmul(st[:, ::-1], m.T), s[:, ::-1] @ b.T)]:
+            assert np.allclose(p, q, rtol=1e-5, atol=1e-5)
+        stack = la.matmul(st.swapaxes(-1, -2), st)
+        for i in range(3):
+            assert bits_equal(stack[i], oracle.matmul(np.ascontiguousarray(st[i].T), np.ascontiguousarray(st[i])))
+    for x in [a[:1], a[:3], np.eye(128)[None]]:  # tests/test_linalg.py:205-206
+        assert np.allclose(la.matmul(x, x), x @ x)
+
+
+def test_batched_products_equal_the_single_products():
+    """hs_dgemm_batched / hs_sgemm_batched: every cell of a stack, a repeated operand (stride 0) and transposed cells
+    give the bits of hs_dgemm on that cell."""
+    import ctypes
+    from rustymath_amd import _lib
+    rng = np.random.default_rng(8)
+    p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    for dtype, f in ((np.float64, 'hs_dgemm_batched'), (np.float32, 'hs_sgemm_batched')):
+        for m, k, n, ops in ((37, 130, 65, 'NN'), (128, 64, 1, 'TN'), (1, 50, 70, 'NT'), (200, 33, 129, 'TT')):
+            cnt = 5
+            a = rng.standard_normal((cnt,) + ((m, k) if ops[0] == 'N' else (k, m))).astype(dtype)
+            b = rng.standard_normal((k, n) if ops[1] == 'N' else (n, k)).astype(dtype)
+            c0 = rng.standard_normal((cnt, m, n)).astype(dtype)
+            c = c0.copy()
+            _lib.check(getattr(_lib.load(), f)(oracle.OPS[ops[0]], oracle.OPS[ops[1]], m, n, k, 0.5, p(a), a.shape[2],
+                                               a.shape[1] * a.shape[2], p(b), b.shape[1], 0, -1.0, p(c), n, m * n, cnt),
+                       'gemm_batched')
+            for i in range(cnt):
+                want = hs_gemm(a[i], b, 0.5, -1.0, c0[i].copy(), ops[0], ops[1])
+                assert bits_equal(c[i], want), (dtype, m, k, n, ops, i)

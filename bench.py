@@ -14,11 +14,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'FP64-accurate solve GFLOP/s: low-prec LU+IR vs plain FP64 LU; batched solves/s'
 PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'hbm': 8.0e12}
-# Measured HBM bytes per launch of the FP32 trailing-tile kernel (trail_rest<float>) in one mixed solve of order n: two
-# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --no-cpu --only-mixed`, each
-# scaled by what the same counter reads on a 1 GiB device copy (`bench.py --copy-probe 1024`: FETCH_SIZE counts half
-# the bytes read, WRITE_SIZE all the bytes written); the summary is profiles/trail_rest_f32_lu_n16384_traffic.md.
-TRAFFIC = {('trail_f32', 16384): 395992297}
+# Measured per launch in one mixed solve of order n (profiles/mixed_solve_n16384_counters.md): HBM bytes from two
+# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --steps 1 --warmup 0 --only-mixed --no-cpu`, each
+# scaled by what the same counter reads on a 1 GiB device copy (`bench.py --copy-probe 1024`: FETCH_SIZE counts half the
+# bytes read, WRITE_SIZE all the bytes written); and the MFMA units' busy fraction (SQ_VALU_MFMA_BUSY_CYCLES over the
+# dispatch's GRBM_GUI_ACTIVE cycles times the 1024 SIMDs, rocprofv3's MfmaUtil) from a third pass.
+TRAFFIC = {('panel_f32', 16384): 166152918, ('trail_f32', 16384): 405026994}
+MFMA_BUSY = {('panel_f32', 16384): 0.2279, ('trail_f32', 16384): 0.5802}
 
 
 def host_threads():
@@ -400,6 +402,8 @@ def main():
     ap.add_argument('--n', type=int, default=16384)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--only-mixed', action='store_true', help='the mixed-precision solve only (for profiling runs)')
+    ap.add_argument('--part', choices=['plain', 'batched', 'adjoint', 'config1'], default=None,
+                    help='one secondary line only, without the headline (for profiling runs)')
     ap.add_argument('--copy-probe', type=int, default=0, metavar='MIB',
                     help='only copy MIB MiB between two device buffers 5 times: the known-size kernel that scales the '
                          'HBM byte counters of a profiling run')
@@ -422,6 +426,11 @@ def main():
     torch.cuda.set_device(0)
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
+    if args.part:
+        parts = {'plain': plain_fp64_bench, 'batched': batched_bench, 'adjoint': adjoint_bench,
+                 'config1': lambda: config1_bench()[0]}
+        print(json.dumps({args.part: parts[args.part]()}))
+        return
     a, b = householder_matrix(n, seed=3)
     torch.cuda.synchronize()
     tol = 10 * n * torch.finfo(torch.float64).eps
@@ -457,26 +466,27 @@ def main():
         fn(0, *[ctypes.byref(x) for x in v])
         return v[0].value, v[1].value, v[2].value, v[3].value
     times, iters, berr, x = [], 0, 0.0, None
-    lib.hs_profile_trail(1, None, None, None, None)  # dispatch-bound events on the roofline kernel's launches
+    lu_prof = (ctypes.c_double * 8)()
+    lib.hs_profile_lu(1, None)  # dispatch-bound events on the panel and trail_rest launches
     for _ in range(args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         x, iters, berr = dv.dsgesv(a, b, tol=tol)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
-    tms, tl, tf, tb = prof(lib.hs_profile_trail)
+    lib.hs_profile_lu(0, lu_prof)
+    tms, tl, tf, tb, pms, pl, pfl, wfl = list(lu_prof)
+    tl, pl = int(tl), int(pl)
     t = sum(times) / len(times)
     fwd = ((x - x64).norm() / x64.norm()).item()
 
-    # The roofline of the MFMA kernel that carries the factorization's flops. Each 256-column block's trailing product
-    # runs in three parts: the next block's columns as one product launch (gemm_mfma), the tiles that the next block's
-    # panel launches take on their idle workgroups, and the tiles those leave, in trail_rest. trail_rest is the pure
-    # MFMA kernel with the most of them: timed live over the timed steps with HIP events that the library records at
-    # each launch's dispatch and end on its stream (hs_profile_trail), with the flops and algorithmic bytes of the tiles
-    # each launch computed, per launch averaged over the steps; `traffic` is its measured HBM bytes per launch
-    # (TRAFFIC). The product launches (hs_profile_gemm) are reported beside it, and the panel kernel, latency-bound
-    # (one pivot exchange per column), by its own phase counters (hs_profile_panel, workgroup 0), both over one more
-    # solve (the panel's counters slow it).
+    # The rooflines, from live timings of the timed steps: the library brackets every panel and trail_rest launch with
+    # HIP events recorded at the dispatch's start and end on its stream (hs_profile_lu). The dominant kernel by time is
+    # the panel kernel: its launches factor the 64-column panels on XCD 0 and, on the other XCDs, compute trailing tiles
+    # of the previous block's product (look-ahead workers), so its flops are the panels' own and the workers' tiles
+    # (2 rows cols k each), against the FP32 MFMA peak. trail_rest takes the tiles the workers left; the product
+    # launches (gemm_mfma, the next block's columns) are timed over one more solve (hs_profile_gemm), as are the panel's
+    # per-phase counters (workgroup 0; they slow it). `traffic` is each kernel's measured HBM bytes per launch (TRAFFIC).
     ticks = (ctypes.c_int64 * 8)()
     lib.hs_profile_gemm(1, None, None, None, None)
     lib.hs_profile_panel(1, None)
@@ -484,16 +494,27 @@ def main():
     gms, gl, gf, gb = prof(lib.hs_profile_gemm)
     lib.hs_profile_panel(0, ticks)
     factor_flops = 2 / 3 * n ** 3
-    achieved = tf / (tms / 1e3) / 1e12
-    roof = {'bound': 'mfma', 'achieved': round(achieved, 2), 'peak': PEAK['fp32_mfma'] / 1e12, 'unit': 'TFLOP/s',
-            'frac': round(achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': TRAFFIC.get(('trail_f32', n)),
-            'kernel': 'trail_rest<float> (the LU trailing products\' tiles left by the panel launches)', 'launches': tl,
-            'flop_per_launch': round(tf / tl), 'algorithmic_bytes_per_launch': round(tb / tl),
-            'avg_launch_us': round(tms / tl * 1e3, 2),
-            'share_of_factorization_flops': round(tf / args.steps / factor_flops, 4),
-            'product_launches': {'kernel': 'gemm_mfma<float>', 'launches': gl, 'ms': round(gms, 3),
-                                 'tflops': round(gf / (gms / 1e3) / 1e12, 2),
-                                 'share_of_factorization_flops': round(gf / factor_flops, 4)}}
+    panel_achieved = (pfl + wfl) / (pms / 1e3) / 1e12
+    trail_achieved = tf / (tms / 1e3) / 1e12
+    roof = {'bound': 'mfma', 'achieved': round(panel_achieved, 2), 'peak': PEAK['fp32_mfma'] / 1e12, 'unit': 'TFLOP/s',
+            'frac': round(panel_achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': TRAFFIC.get(('panel_f32', n)),
+            'kernel': 'panel_kernel<float> (the panels on XCD 0, trailing tiles of the previous block on XCDs 1-7)',
+            'launches': pl, 'flop_per_launch': round((pfl + wfl) / pl), 'avg_launch_us': round(pms / pl * 1e3, 2),
+            'mfma_busy_rocprof': MFMA_BUSY.get(('panel_f32', n)),
+            'share_of_step_time': round(pms / args.steps / (t * 1e3), 4),
+            'worker_flop_share': round(wfl / (pfl + wfl), 4),
+            'share_of_factorization_flops': round((pfl + wfl) / args.steps / factor_flops, 4),
+            'other_kernels': [
+                {'kernel': 'trail_rest<float> (the trailing tiles the panel launches left)', 'bound': 'mfma',
+                 'achieved': round(trail_achieved, 2), 'frac': round(trail_achieved * 1e12 / PEAK['fp32_mfma'], 4),
+                 'launches': tl, 'flop_per_launch': round(tf / tl), 'algorithmic_bytes_per_launch': round(tb / tl),
+                 'avg_launch_us': round(tms / tl * 1e3, 2), 'traffic': TRAFFIC.get(('trail_f32', n)),
+                 'mfma_busy_rocprof': MFMA_BUSY.get(('trail_f32', n)),
+                 'share_of_step_time': round(tms / args.steps / (t * 1e3), 4),
+                 'share_of_factorization_flops': round(tf / args.steps / factor_flops, 4)},
+                {'kernel': 'gemm_mfma<float> (the next block\'s columns)', 'launches': gl, 'ms': round(gms, 3),
+                 'achieved': round(gf / (gms / 1e3) / 1e12, 2), 'frac': round(gf / (gms / 1e3) / PEAK['fp32_mfma'], 4),
+                 'share_of_factorization_flops': round(gf / factor_flops, 4)}]}
     panel_s = (sum(ticks[:5]) + ticks[7]) / 1e8
     panel = {'kernel': 'panel_kernel<float>', 'bound': 'latency (one pivot exchange per column)',
              'column_loop_ms': round(panel_s * 1e3, 3), 'us_per_column': round(panel_s / n * 1e6, 3),

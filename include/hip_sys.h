@@ -124,6 +124,13 @@ int hs_scholesky(const float* a, int64_t n, float* l, int* ok);
 int hs_dsolve_stack(int64_t count, int64_t n, int64_t k, const double* a, const double* b, double* x, int* regular);
 int hs_ssolve_stack(int64_t count, int64_t n, int64_t k, const float* a, const float* b, float* x, int* regular);
 int hs_dlu_stack(int64_t count, int64_t n, const double* a, double* factors, int64_t* perm);
+/* `Stacked::lu_solve` (src/linalg/stacked.rs:189-204): x (count×n×k) from reference-layout factors (count×n×n,
+ * column-major cells) and perm (count×n, every item in [0, n), else -5), each cell as hs_*lu_solve gives it. Cells up
+ * to order 512 with at most 8 right-hand sides and permutations for perm run as one batched launch. */
+int hs_dlu_solve_stack(int64_t count, int64_t n, int64_t k, const double* factors, const int64_t* perm, const double* b,
+                       double* x);
+int hs_slu_solve_stack(int64_t count, int64_t n, int64_t k, const float* factors, const int64_t* perm, const float* b,
+                       float* x);
 int hs_slu_stack(int64_t count, int64_t n, const float* a, float* factors, int64_t* perm);
 
 /* Device-resident forms (no counterpart in the reference, whose API takes host slices): pointers are
@@ -246,13 +253,14 @@ int hs_profile_panel(int on, int64_t* out);
  * stream, on = 0 stops; gives the launches' summed time (ms), their count, their algorithmic flops (2 m n k) and
  * algorithmic bytes (a and b read once, c written once and, where beta is not 0, read once). Any output may be NULL. */
 int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops, double* bytes);
-/* Profiling of the LU's trailing-tile kernel (no counterpart in the reference): the FP32 and FP64 factorizations of
- * order above 512 run each block's trailing product partly on the panel launches' idle workgroups and the remaining
- * tiles in trail_rest. on = 1 clears and starts bracketing every trail_rest launch with HIP events on its stream,
- * on = 0 stops; gives the launches' summed time (ms), their count, the flops of the tiles they computed (2 rows cols k
- * each) and the algorithmic bytes (the tiles' items of c read and written once, the rows of a and columns of b they
- * touch read once). Any output may be NULL. */
-int hs_profile_trail(int on, double* ms, int64_t* launches, double* flops, double* bytes);
+/* Profiling of the LU (no counterpart in the reference): the FP32 and FP64 factorizations of order above 512 run each
+ * block's trailing product partly on the panel launches' idle workgroups (their workers) and the remaining tiles in
+ * trail_rest. on = 1 clears and starts bracketing every panel and trail_rest launch with HIP events on its stream;
+ * on = 0 stops. out (8 values, may be NULL) gets, since the last call: trail_rest's summed time (ms), launches, the
+ * flops of its tiles (2 rows cols k each) and their algorithmic bytes (the tiles' items of c read and written once,
+ * the rows of a and columns of b they touch read once); then the panel launches' summed time (ms), launches, their
+ * own factorization flops and the flops of the tiles their workers computed. */
+int hs_profile_lu(int on, double* out);
 /* Profiling of the batched solves: on = 1 clears and starts per-phase timers in workgroup 0 (100 MHz ticks: setup,
  * panel columns, U rows and trailing update, regularity check and L, U), so that runs launch directly rather than
  * replay their graph; on = 0 stops. `out` (8 values, may be NULL) gets the sums, out[7] the systems timed. */

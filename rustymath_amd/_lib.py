@@ -49,6 +49,8 @@ SIGNATURES = {
     'hs_ssolve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     'hs_dlu_stack': (_i32, [_i64, _i64, _vp, _vp, _vp]),
     'hs_slu_stack': (_i32, [_i64, _i64, _vp, _vp, _vp]),
+    'hs_dlu_solve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
+    'hs_slu_solve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     'hs_dsolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),
     'hs_ssolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),
     'hs_dev_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64, _vp]),
@@ -73,7 +75,7 @@ SIGNATURES = {
     'hs_dev_gesv_batched_free': (_i32, [_i64]),
     'hs_profile_panel': (_i32, [_i32, _vp]),
     'hs_profile_gemm': (_i32, [_i32, _vp, _vp, _vp, _vp]),
-    'hs_profile_trail': (_i32, [_i32, _vp, _vp, _vp, _vp]),
+    'hs_profile_lu': (_i32, [_i32, _vp]),
     'hs_profile_batched': (_i32, [_i32, _vp]),
 }
 

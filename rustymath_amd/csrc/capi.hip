@@ -548,6 +548,70 @@ int host_stack(int64_t count, int64_t n, int64_t k, const T* a, const T* b, T* x
     return rc;
 }
 
+// Row r of a cell's reference-layout factors (column-major, "row j holds column j") into the batched kernel's layout
+// (row-major, row perm[r] holding position r): the inverse of batched_to_reference.
+template <typename T>
+__global__ void reference_to_batched(int64_t count, int64_t n, const T* __restrict__ f, const int64_t* __restrict__ perm,
+                                     T* a) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count * n * n) return;
+    const int64_t cell = idx / (n * n), rem = idx % (n * n), c = rem / n, r = rem % n;
+    a[cell * n * n + perm[cell * n + r] * n + c] = f[idx];
+}
+
+// `Stacked::lu_solve` (src/linalg/stacked.rs:189-204): x_i = lu_solve(factors_i, perm_i, b_i) for `count` cells of order
+// n with k right-hand sides. Orders up to 512 with at most 8 right-hand sides, where every perm_i is a permutation, run
+// as one batched launch (the batched kernel's lu_solve mode, bit for bit each cell's lu_solve); others cell by cell.
+template <typename T>
+int host_lu_solve_stack(int64_t count, int64_t n, int64_t k, const T* factors, const int64_t* perm, const T* b, T* x) {
+    if (count < 0) return -1;
+    if (n < 0) return -2;
+    if (k < 0) return -3;
+    for (int64_t i = 0; i < count * n; ++i) if (perm[i] < 0 || perm[i] >= n) return -5;
+    if (count == 0 || n == 0 || k == 0) return OK;
+    bool perms = n <= 512 && k <= 8;
+    for (int64_t c = 0; c < count && perms; ++c) {
+        std::vector<char> seen((size_t)n, 0);
+        for (int64_t i = 0; i < n && perms; ++i) { perms = !seen[perm[c * n + i]]; seen[perm[c * n + i]] = 1; }
+    }
+    if (!perms) {
+        for (int64_t c = 0; c < count; ++c) {
+            const int rc = host_lu_solve<T>(factors + c * n * n, perm + c * n, n, b + c * n * k, k, x + c * n * k);
+            if (rc) return rc;
+        }
+        return OK;
+    }
+    hipStream_t s = thread_stream();
+    T* df = (T*)pool_get(sizeof(T) * (size_t)(count * n * n));
+    T* da = (T*)pool_get(sizeof(T) * (size_t)(count * n * n));
+    T* db = (T*)pool_get(sizeof(T) * (size_t)(count * n * k));
+    int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)(count * n));
+    int rc = OK;
+    if (!df || !da || !db || !dp) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && (hipMemcpyAsync(df, factors, sizeof(T) * count * n * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(db, b, sizeof(T) * count * n * k, hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(dp, perm, sizeof(int64_t) * count * n, hipMemcpyHostToDevice, s) != hipSuccess))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) {
+        reference_to_batched<T><<<(unsigned)((count * n * n + 255) / 256), 256, 0, s>>>(count, n, df, dp, da);
+        if (hipGetLastError() != hipSuccess) rc = hip_status(hipErrorUnknown, "reference_to_batched");
+    }
+    if (!rc) {
+        std::vector<int64_t> ns((size_t)count, n), ks((size_t)count, k), ao((size_t)count), bo((size_t)count), po((size_t)count);
+        for (int64_t i = 0; i < count; ++i) { ao[i] = i * n * n; bo[i] = i * n * k; po[i] = i * n; }
+        long long h = 0;
+        rc = batched_plan(count, ns.data(), ks.data(), ao.data(), bo.data(), po.data(), sizeof(T) == 8, da, db, dp, nullptr,
+                          1, 32, 1, &h);
+        if (!rc) rc = batched_run(h, false, s);
+        if (h) { const int rf = batched_free(h); if (!rc) rc = rf; }
+    }
+    if (!rc && hipMemcpyAsync(x, db, sizeof(T) * count * n * k, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = hip_status(hipErrorUnknown, "D2H");
+    hipStreamSynchronize(s);
+    pool_put(df); pool_put(da); pool_put(db); pool_put(dp);
+    return rc;
+}
+
 template <typename T>
 int dev_gemm_batched(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T* A, int64_t lda, int64_t sa,
                      const T* B, int64_t ldb, int64_t sb, T beta, T* C, int64_t ldc, int64_t sc, int64_t batch,
@@ -686,6 +750,14 @@ int hs_dlu_stack(int64_t count, int64_t n, const double* a, double* factors, int
 }
 int hs_slu_stack(int64_t count, int64_t n, const float* a, float* factors, int64_t* perm) {
     return host_stack<float>(count, n, 0, a, nullptr, nullptr, nullptr, factors, perm);
+}
+int hs_dlu_solve_stack(int64_t count, int64_t n, int64_t k, const double* factors, const int64_t* perm, const double* b,
+                       double* x) {
+    return host_lu_solve_stack<double>(count, n, k, factors, perm, b, x);
+}
+int hs_slu_solve_stack(int64_t count, int64_t n, int64_t k, const float* factors, const int64_t* perm, const float* b,
+                       float* x) {
+    return host_lu_solve_stack<float>(count, n, k, factors, perm, b, x);
 }
 int hs_dsolve(const double* a, int64_t n, const double* b, int64_t k, double* x, int* regular) {
     return host_solve<double>(a, n, b, k, x, regular);
@@ -850,8 +922,6 @@ int hs_profile_batched(int on, int64_t* out) { return batched_profile(on, (long 
 int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
     return gemm_profile(on, ms, launches, flops, bytes);
 }
-int hs_profile_trail(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
-    return trail_profile(on, ms, launches, flops, bytes);
-}
+int hs_profile_lu(int on, double* out) { return lu_profile(on, out); }
 
 }  // extern "C"

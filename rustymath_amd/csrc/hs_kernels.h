@@ -114,9 +114,8 @@ int gemm_profile(int on, double* ms, int64_t* launches, double* flops, double* b
 // The panel kernel's per-phase profile: on = 1 clears and enables, 0 disables; out[8] gets tick sums.
 int panel_profile(int on, long long* out);
 
-// The live profile of the LU's trailing-tile kernel (the tiles the panels' workers left): on = 1 clears and starts,
-// 0 stops; gives the summed event time (ms), the launches, and the flops and algorithmic bytes of the tiles computed.
-int trail_profile(int on, double* ms, int64_t* launches, double* flops, double* bytes);
+// The LU's live profile: trail_rest and the panel launches bracketed by HIP events (see hs_profile_lu in hip_sys.h).
+int lu_profile(int on, double* out);
 
 // Converts between the device layouts: transpose (square or rectangular), f64 <-> f32 casts.
 template <typename T>

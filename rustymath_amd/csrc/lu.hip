@@ -715,7 +715,26 @@ struct TrailProfile {
         size_t item;
     };
     std::vector<Rec> recs;
+    // The panel launches, timed the same way: their own flops (the panel's factorization) are counted per launch; the
+    // tiles their workers computed are each trail_rest launch's tiles before its first.
+    struct PanelRec {
+        hipEvent_t a, b;
+        int64_t rows;
+        int w;
+    };
+    std::vector<PanelRec> panels;
 } g_trail;
+
+// The flops of a panel's own factorization: per column c, the multipliers of the rows below it and their update of the
+// panel's columns right of c.
+double panel_flops(int64_t rows, int w) {
+    double f = 0;
+    for (int c = 0; c < w; ++c) {
+        const double below = (double)(rows - c - 1);
+        f += below > 0 ? below * (1.0 + 2.0 * (double)(w - c - 1)) : 0.0;
+    }
+    return f;
+}
 
 // Tile number -> (tm, tn) as tile::grouped numbers tiles, on the host.
 void grouped_host(int id, int tiles_m, int tiles_n, int& tm, int& tn) {
@@ -746,12 +765,14 @@ int panel_profile(int on, long long* out) {
     return OK;
 }
 
-// trail_rest's live profile (see TrailProfile): on = 1 clears and starts, 0 stops; gives the summed event time (ms),
-// the launches, and the flops (2 rows cols k per tile computed) and algorithmic bytes (each computed tile's items of c
-// read and written, and the rows of a and columns of b that the launch's tiles touch, read once) since the last call.
-int trail_profile(int on, double* ms, int64_t* launches, double* flops, double* bytes) {
+// The LU's live profile (see TrailProfile): on = 1 clears and starts, 0 stops. out[0..3]: trail_rest's summed event
+// time (ms), launches, the flops of the tiles it computed (2 rows cols k per tile) and their algorithmic bytes (each
+// computed tile's items of c read and written, the rows of a and columns of b the launch's tiles touch, read once);
+// out[4..7]: the panel launches' summed event time (ms), launches, their own factorization flops, and the flops of the
+// trailing tiles their workers computed.
+int lu_profile(int on, double* out) {
     std::lock_guard<std::mutex> lk(g_trail.mu);
-    double total = 0, fl = 0, by = 0;
+    double total = 0, fl = 0, by = 0, worker = 0;
     std::vector<int> first(g_trail.recs.size());
     if (!g_trail.recs.empty())
         HS_HIP(hipMemcpy(first.data(), g_trail.first, first.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -765,23 +786,36 @@ int trail_profile(int on, double* ms, int64_t* launches, double* flops, double* 
         total += e;
         std::vector<char> row_seen(R.tiles_m, 0), col_seen(R.tiles_n, 0);
         double c_items = 0, a_rows = 0, b_cols = 0;
-        for (int id = first[r]; id < R.tiles_m * R.tiles_n; ++id) {
+        for (int id = 0; id < R.tiles_m * R.tiles_n; ++id) {
             int tm, tn;
             grouped_host(id, R.tiles_m, R.tiles_n, tm, tn);
             const double rows = (double)std::min<int64_t>(tile::BM, R.m - (int64_t)tm * tile::BM);
             const double cols = (double)std::min<int64_t>(tile::BN, R.n - (int64_t)tn * tile::BN);
-            fl += 2.0 * rows * cols * (double)R.k;
+            const double f = 2.0 * rows * cols * (double)R.k;
+            if (id < first[r]) { worker += f; continue; }
+            fl += f;
             c_items += rows * cols;
             if (!row_seen[tm]) { row_seen[tm] = 1; a_rows += rows; }
             if (!col_seen[tn]) { col_seen[tn] = 1; b_cols += cols; }
         }
         by += (double)R.item * (2.0 * c_items + (a_rows + b_cols) * (double)R.k);
     }
+    double pms = 0, pfl = 0;
+    for (auto& P : g_trail.panels) {
+        float e = 0;
+        HS_HIP(hipEventSynchronize(P.b));
+        HS_HIP(hipEventElapsedTime(&e, P.a, P.b));
+        hipEventDestroy(P.a);
+        hipEventDestroy(P.b);
+        pms += e;
+        pfl += panel_flops(P.rows, P.w);
+    }
+    if (out) {
+        out[0] = total; out[1] = (double)first.size(); out[2] = fl; out[3] = by;
+        out[4] = pms; out[5] = (double)g_trail.panels.size(); out[6] = pfl; out[7] = worker;
+    }
     g_trail.recs.clear();
-    if (ms) *ms = total;
-    if (launches) *launches = (int64_t)first.size();
-    if (flops) *flops = fl;
-    if (bytes) *bytes = by;
+    g_trail.panels.clear();
     g_trail.on = on != 0;
     if (g_trail.on && !g_trail.first) HS_HIP(hipMalloc(&g_trail.first, TrailProfile::SLOTS * sizeof(int)));
     return OK;
@@ -887,9 +921,22 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
             const int nwg = (int)((rows + PT - 1) / PT);
             const unsigned grid = ahead ? 8u * MAX_WG : (nwg > 1 ? (unsigned)(8 * nwg) : 1u);
             long long* mv = moves + i * move_words;
-            panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, mv, d_leaf, scratch, ex,
-                                                 (unsigned)(i * PW + 1), nwg, status, st->on ? st->prof : nullptr,
-                                                 ahead ? job : none);
+            {
+                std::unique_lock<std::mutex> lk(g_trail.mu);
+                if (g_trail.on && g_trail.panels.size() < (size_t)TrailProfile::SLOTS) {
+                    TrailProfile::PanelRec P{nullptr, nullptr, rows, w};
+                    HS_HIP(hipEventCreate(&P.a));
+                    HS_HIP(hipEventCreate(&P.b));
+                    hipExtLaunchKernelGGL(panel_kernel<T>, dim3(grid), dim3(PT), 0, s, P.a, P.b, 0, a, lda, c0, rows, w,
+                                          pivots, mv, d_leaf, scratch, ex, (unsigned)(i * PW + 1), nwg, status,
+                                          st->on ? st->prof : nullptr, ahead ? job : none);
+                    g_trail.panels.push_back(P);
+                } else {
+                    panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, mv, d_leaf, scratch, ex,
+                                                         (unsigned)(i * PW + 1), nwg, status, st->on ? st->prof : nullptr,
+                                                         ahead ? job : none);
+                }
+            }
             HS_HIP(hipGetLastError());
             if ((rc = ahead ? swap(mv, B0, B1, c0, c1) : swap(mv, 0, n, c0, c1))) break;
             if (c1 < B1) {

@@ -200,9 +200,13 @@ def lu_solve(factors, p, b):
     pc = _cells(perm.reshape(ff + (n,)), frame, (n,))
     bc = _cells(b2.reshape(fb + (n, k)).astype(compute, copy=False), frame, (n, k))
     out = np.zeros_like(bc)
-    f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_solve')
-    for i in range(fc.shape[0]):
-        _lib.check(f(_ptr(fc[i]), _ptr(pc[i]), n, _ptr(bc[i]), k, _ptr(out[i])), 'lu_solve')
+    if fc.shape[0] > 1:  # a stack: one batched launch where the cells are small enough (stacked.rs:189-204)
+        f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_solve_stack')
+        _lib.check(f(fc.shape[0], n, k, _ptr(fc), _ptr(np.ascontiguousarray(pc)), _ptr(bc), _ptr(out)), 'lu_solve')
+    else:
+        f = getattr(_lib.load(), 'hs_' + _sfx(compute) + 'lu_solve')
+        for i in range(fc.shape[0]):
+            _lib.check(f(_ptr(fc[i]), _ptr(pc[i]), n, _ptr(bc[i]), k, _ptr(out[i])), 'lu_solve')
     out = out.reshape(tuple(frame) + (n, k))
     if vector: out = out[..., 0]
     return out.astype(store, copy=False)

@@ -532,3 +532,28 @@ def test_batched_products_equal_the_single_products():
             for i in range(cnt):
                 want = hs_gemm(a[i], b, 0.5, -1.0, c0[i].copy(), ops[0], ops[1])
                 assert bits_equal(c[i], want), (dtype, m, k, n, ops, i)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_lu_solve_on_stacks_matches_each_cell(la, dtype):
+    """`Stacked::lu_solve` (src/linalg/stacked.rs:189-204): a stack of factors and positions with a stack of right-hand
+    sides (one batched launch up to order 512 and 8 columns, cell by cell beyond, and for positions that are not a
+    permutation), each cell the oracle's lu_solve bit for bit; a vector right-hand side broadcast over the stack."""
+    rng = np.random.default_rng(12)
+    for n, k, count in ((40, 3, 7), (200, 8, 4), (513, 2, 2), (64, 9, 3)):
+        a = rng.standard_normal((count, n, n)).astype(dtype)
+        b = rng.standard_normal((count, n, k)).astype(dtype)
+        f = la.lu(a)
+        x = la.lu_solve(f.factors, f.p, b)
+        for i in range(count):
+            assert bits_equal(x[i], oracle.lu_solve(f.factors[i], f.p[i], b[i])), (n, k, i)
+    v = rng.standard_normal(40).astype(dtype)
+    a = rng.standard_normal((5, 40, 40)).astype(dtype)
+    f = la.lu(a)
+    xv = la.lu_solve(f.factors, f.p, v)
+    for i in range(5):
+        assert bits_equal(xv[i], oracle.lu_solve(f.factors[i], f.p[i], v))
+    p = f.p.copy(); p[:, 1] = p[:, 0]  # positions in range but not a permutation: the cell-by-cell path
+    xp = la.lu_solve(f.factors, p, v)
+    for i in range(5):
+        assert bits_equal(xp[i], oracle.lu_solve(f.factors[i], p[i], v))

@@ -130,3 +130,27 @@ def test_batched_products_reject_bad_arguments_on_the_host():
     assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 3, 3) == -16
     assert lib.hs_dev_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 1, 3, None) == -16
     assert lib.hs_dgemm_batched(111, 111, 2, 2, 2, 1.0, None, 2, 4, None, 2, 4, 0.0, None, 2, 4, 0) == 0
+
+
+def header_params():
+    "name -> number of parameters, for every entry point that include/hip_sys.h declares."
+    text = re.sub(r'/\*.*?\*/', '', open(os.path.join(ROOT, 'include', 'hip_sys.h')).read(), flags=re.S)
+    out = {}
+    for name, args in re.findall(r'^\s*(?:int|const char\*)\s+(hs_\w+)\(([^;]*?)\);', text, re.M | re.S):
+        args = ' '.join(args.split())
+        out[name] = 0 if args in ('', 'void') else len(args.split(','))
+    return out
+
+
+def test_bindings_match_the_header():
+    """The ctypes table (rustymath_amd/_lib.py) and the hip-sys crate (hip-sys/src/lib.rs, the Rust binding a
+    rustymath build would link) declare every entry point of the header with its number of parameters."""
+    from rustymath_amd import _lib
+    want = header_params()
+    assert len(want) >= 50
+    for name, count in want.items():
+        assert len(_lib.SIGNATURES[name][1]) == count, name
+    rs = open(os.path.join(ROOT, 'hip-sys', 'src', 'lib.rs')).read()
+    got = {m.group(1): (0 if not m.group(2).strip() else m.group(2).count(':'))
+           for m in re.finditer(r'pub fn (hs_\w+)\(([^)]*)\)', rs)}
+    assert got == want

@@ -9,7 +9,7 @@
  *     row of row i (core/src/linalg/factor.rs:132-134);
  *   - every entry point returns 0, or -i where argument i is invalid (LAPACK style), or > 0 for a HIP
  *     runtime failure (100000 = a device-side bounded wait ran out; 100001 = valid arguments beyond what this
- *     library handles, such as an LU above order 16384); hs_last_error() gives the text.
+ *     library handles, such as an LU of order 2^31 or more); hs_last_error() gives the text.
  *     A zero pivot is not an error (factor.rs:606): the factorizations report the first one in `info`; NaN propagates;
  *   - the caller owns every buffer it passes. Host-pointer entry points copy in and out through a
  *     device-memory pool the library keeps across calls (as LuWork keeps buffers, factor.rs:196-201).

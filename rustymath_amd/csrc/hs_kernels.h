@@ -31,7 +31,8 @@ int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t ld
 // diagonal (unit diagonal implicit), U on and above; pivots[c] = the row swapped with row c (absolute,
 // 0-based, LAPACK ipiv style). The arithmetic is rustymath's lu_factor (core/src/linalg/factor.rs:617),
 // item for item. A zero pivot leaves its column undivided (factor.rs:606); the C ABI's `info` reports the first one
-// from the factors' diagonal afterwards. Orders above 16384 give ERR_UNSUPPORTED.
+// from the factors' diagonal afterwards. Above 32 x 512 rows the panels are factored in memory, column by
+// column, without look-ahead (the same arithmetic).
 // bf16_terms (FP32 only, 0 = none): the trailing products after each 256-column block take their operands as bfloat16
 // (gemm_bf16_update with that many terms); panels, row moves and the rows of U stay FP32. Not the reference's
 // arithmetic: the low-precision factors of the BF16 mixed solve.

@@ -674,6 +674,115 @@ __global__ void cast_kernel(int64_t rows, int64_t cols, const S* __restrict__ sr
     dst[i * ldd + j] = (D)src[i * lds + j];
 }
 
+// ---- Panels taller than the panel kernel holds (more than 32 x 512 rows) ----
+// The panel stays in memory and is factored one column at a time in three launches: a pivot search over the column in
+// blocks (each block's best key and first position), the pick (the same rule over the blocks' bests: larger magnitude,
+// then smaller position, NaN never chosen over a number) with the swap of the two rows within the panel's columns, and
+// the multipliers with their update of the panel's columns to the right, one row per thread. Every item takes the same
+// operations in the same order as in the panel kernel: l = x · (1/pivot) (a quotient for a subnormal pivot, x itself for
+// a zero one, whose update then skips the rest of its leaf), one fused multiply-add per later column.
+constexpr int TALL_T = 256, TALL_G = 256;
+static_assert(TALL_T == TALL_G, "the pick reduces one block best per thread");
+
+template <typename T>
+__global__ __launch_bounds__(TALL_T) void tall_colmax(const T* __restrict__ a, int64_t lda, int64_t c0, int64_t rows, int c,
+                                                      unsigned long long* __restrict__ part_kv, int* __restrict__ part_kp) {
+    __shared__ unsigned long long skv[TALL_T];
+    __shared__ int skp[TALL_T];
+    unsigned long long bkv = 0ull;
+    int bkp = 0x7fffffff;
+    for (int64_t r = c + (int64_t)blockIdx.x * TALL_T + threadIdx.x; r < rows; r += (int64_t)TALL_G * TALL_T) {
+        const unsigned long long kv = mag_key(a[(c0 + r) * lda + c0 + c]);
+        if (kv > bkv || (kv == bkv && (int)r < bkp)) { bkv = kv; bkp = (int)r; }
+    }
+    skv[threadIdx.x] = bkv;
+    skp[threadIdx.x] = bkp;
+    __syncthreads();
+    for (int h = TALL_T / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            const unsigned long long ov = skv[threadIdx.x + h];
+            const int op = skp[threadIdx.x + h];
+            if (ov > skv[threadIdx.x] || (ov == skv[threadIdx.x] && op < skp[threadIdx.x])) {
+                skv[threadIdx.x] = ov;
+                skp[threadIdx.x] = op;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part_kv[blockIdx.x] = skv[0]; part_kp[blockIdx.x] = skp[0]; }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TALL_T) void tall_pick_swap(T* __restrict__ a, int64_t lda, int64_t c0, int c, int w,
+                                                         const unsigned long long* __restrict__ part_kv,
+                                                         const int* __restrict__ part_kp, int64_t* __restrict__ pivots,
+                                                         int* __restrict__ pick) {
+    __shared__ unsigned long long skv[TALL_T];
+    __shared__ int skp[TALL_T];
+    skv[threadIdx.x] = part_kv[threadIdx.x];
+    skp[threadIdx.x] = part_kp[threadIdx.x];
+    __syncthreads();
+    for (int h = TALL_T / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            const unsigned long long ov = skv[threadIdx.x + h];
+            const int op = skp[threadIdx.x + h];
+            if (ov > skv[threadIdx.x] || (ov == skv[threadIdx.x] && op < skp[threadIdx.x])) {
+                skv[threadIdx.x] = ov;
+                skp[threadIdx.x] = op;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = skp[0];
+    if (threadIdx.x == 0) { pivots[c0 + c] = c0 + p; *pick = p; }
+    if (p != c && (int)threadIdx.x < w) {
+        T* rc = a + (c0 + c) * lda + c0;
+        T* rp = a + (c0 + (int64_t)p) * lda + c0;
+        const T t = rc[threadIdx.x];
+        rc[threadIdx.x] = rp[threadIdx.x];
+        rp[threadIdx.x] = t;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(TALL_T) void tall_update(T* __restrict__ a, int64_t lda, int64_t c0, int64_t rows, int c, int w,
+                                                      const int* __restrict__ leaf_end) {
+    const int64_t r = c + 1 + (int64_t)blockIdx.x * TALL_T + threadIdx.x;
+    if (r >= rows) return;
+    const T* prow = a + (c0 + c) * lda + c0;
+    T* row = a + (c0 + r) * lda + c0;
+    const T pivot = prow[c];
+    const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
+    const bool zero = pivot == T(0);
+    const int skip = zero ? leaf_end[c0 + c] - (int)c0 - c : 0;
+    const T x = row[c];
+    const T l = zero ? x : (fabs(pivot) >= tiny ? x * (T(1) / pivot) : x / pivot);
+    row[c] = l;
+    for (int j = c + 1; j < w; ++j)
+        if (j - c >= skip) row[j] = __builtin_elementwise_fma(l, -prow[j], row[j]);
+}
+
+// The panel's net row permutation as the move list that swap_kernel applies (count, then pairs dst, src): the swaps of
+// columns [c0, c0 + w) in order, followed on the at most 2 w positions they touch.
+__global__ void tall_moves(const int64_t* __restrict__ pivots, int64_t c0, int w, long long* __restrict__ moves) {
+    if (threadIdx.x != 0) return;
+    long long pos[2 * PW], orig[2 * PW];
+    int cnt = 0;
+    auto find = [&](long long x) {
+        for (int i = 0; i < cnt; ++i) if (pos[i] == x) return i;
+        pos[cnt] = x; orig[cnt] = x;
+        return cnt++;
+    };
+    for (int c = 0; c < w; ++c) {
+        const int i = find(c0 + c), j = find(pivots[c0 + c]);
+        const long long t = orig[i]; orig[i] = orig[j]; orig[j] = t;
+    }
+    long long k = 0;
+    for (int i = 0; i < cnt; ++i)
+        if (orig[i] != pos[i]) { moves[1 + 2 * k] = pos[i]; moves[2 + 2 * k] = orig[i]; ++k; }
+    moves[0] = k;
+}
+
 // The reference's leaves: lu_panel halves [c0, c1) at (c0 + c1) / 2 until 16 columns or fewer remain.
 void leaves(int64_t c0, int64_t c1, std::vector<int64_t>& ends) {
     if (c1 - c0 > 16) { int64_t mid = (c0 + c1) / 2; leaves(c0, mid, ends); leaves(mid, c1, ends); return; }
@@ -830,10 +939,9 @@ int lu_profile(int on, double* out) {
 template <typename T>
 int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms) {
     if (n == 0) return OK;
-    if (n > (int64_t)MAX_WG * PT) {
-        set_error("LU of order above 16384 is not supported: the panel kernel holds at most 32 x 512 rows");
-        return ERR_UNSUPPORTED;
-    }
+    if (n >= (int64_t)1 << 31) { set_error("LU of order 2^31 or more is not supported"); return ERR_UNSUPPORTED; }
+    // Above 32 x 512 rows the panels are factored in memory, column by column (tall_*), without look-ahead.
+    const bool tall = n > (int64_t)MAX_WG * PT;
     // The panel launches' workgroups wait on one another: one factorization with such panels at a time (see
     // resident_mutex). Every path below synchronises the stream before it returns.
     std::unique_lock<std::mutex> resident(resident_mutex(), std::defer_lock);
@@ -869,7 +977,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     const size_t np = panels.size();
     const size_t move_words = 1 + 4 * PW;
     // + status and stop mark, then one tile counter per block (for the product run beside that block's panels)
-    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 2 + (blocks.size() + 1) / 2;
+    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 2 + (blocks.size() + 1) / 2 + (TALL_G * 12 + 8) / 8;
     int* d_leaf = (int*)pool_get(n * sizeof(int));
     T* scratch = (T*)pool_get(sizeof(T) * (size_t)n * PW);
     char* xbuf = (char*)pool_get(clear_words * 8);
@@ -882,6 +990,10 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     int* status = (int*)(moves + np * move_words);
     unsigned* trail_stop = (unsigned*)(status + 2);
     int* trail_next = status + 4;  // blocks.size() counters
+    // The tall panels' block bests and pick, after the counters.
+    unsigned long long* tall_kv = (unsigned long long*)xbuf + (clear_words - (TALL_G * 12 + 8) / 8);
+    int* tall_kp = (int*)(tall_kv + TALL_G);
+    int* tall_pick = tall_kp + TALL_G;
     HS_HIP(hipMemcpyAsync(d_leaf, leaf_end.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
     fill_words<<<(unsigned)((clear_words + 255) / 256), 256, 0, s>>>((unsigned long long*)xbuf, (int64_t)clear_words, 0ull);
     HS_HIP(hipGetLastError());
@@ -921,7 +1033,15 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
             const int nwg = (int)((rows + PT - 1) / PT);
             const unsigned grid = ahead ? 8u * MAX_WG : (nwg > 1 ? (unsigned)(8 * nwg) : 1u);
             long long* mv = moves + i * move_words;
-            {
+            if (tall) {
+                for (int c = 0; c < w; ++c) {
+                    tall_colmax<T><<<TALL_G, TALL_T, 0, s>>>(a, lda, c0, rows, c, tall_kv, tall_kp);
+                    tall_pick_swap<T><<<1, TALL_T, 0, s>>>(a, lda, c0, c, w, tall_kv, tall_kp, pivots, tall_pick);
+                    tall_update<T><<<(unsigned)((rows - c - 1 + TALL_T - 1) / TALL_T + 1), TALL_T, 0, s>>>(a, lda, c0, rows,
+                                                                                                      c, w, d_leaf);
+                }
+                tall_moves<<<1, 64, 0, s>>>(pivots, c0, w, mv);
+            } else {
                 std::unique_lock<std::mutex> lk(g_trail.mu);
                 if (g_trail.on && g_trail.panels.size() < (size_t)TrailProfile::SLOTS) {
                     TrailProfile::PanelRec P{nullptr, nullptr, rows, w};
@@ -982,7 +1102,7 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
             }
         }
         const int64_t B2 = panels[blocks[b + 1].second - 1].second;
-        if (B2 >= n || n - B1 < 2) {
+        if (tall || B2 >= n || n - B1 < 2) {
             rc = gemm<T>(false, false, n - B1, n - B1, B1 - B0, T(-1), a + B1 * lda + B0, lda, a + B0 * lda + B1, lda, T(1),
                          a + B1 * lda + B1, lda, s);
             continue;

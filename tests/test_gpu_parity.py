@@ -557,3 +557,21 @@ def test_lu_solve_on_stacks_matches_each_cell(la, dtype):
     xp = la.lu_solve(f.factors, p, v)
     for i in range(5):
         assert bits_equal(xp[i], oracle.lu_solve(f.factors[i], p[i], v))
+
+
+def test_lu_above_the_panel_kernels_height_matches_the_oracle(la):
+    """Order 16448, above the 32 × 512 rows the panel kernel holds: the panels are factored in memory column by column
+    (pivot search in blocks, pick and swap, multipliers and their update), without look-ahead. Pivots and factors equal
+    the oracle's item for item, two zero columns included (one inside a leaf, one in the last panel: each leaves its
+    column undivided and skips the rest of its leaf, factor.rs:606). The reference's lu_factor has no order limit
+    (core/src/linalg/factor.rs:617-630)."""
+    n = 16448
+    a = random(n, n, 23, np.float64)
+    a[:, 5] = 0.0
+    a[:, 16400] = 0.0
+    print(f'tall LU n={n}: GPU', flush=True)
+    f = la.lu(a)
+    print('tall LU: oracle', flush=True)
+    want, perm, _ = oracle.lu(a)
+    assert (f.p == perm).all()
+    assert bits_equal(f.factors, want), f'max diff {np.nanmax(np.abs(f.factors - want))}'

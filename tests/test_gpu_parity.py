@@ -460,22 +460,27 @@ def test_profiles_account_for_a_factorization(la, dtype):
     lib = _lib.load()
     n = 2100
     a = random(n, n, 7, dtype)
-    v = [ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)]
+    v = (ctypes.c_double * 8)()
     g = [ctypes.c_double(0), ctypes.c_int64(0), ctypes.c_double(0), ctypes.c_double(0)]
     ticks = (ctypes.c_int64 * 8)()
-    assert lib.hs_profile_trail(1, None, None, None, None) == 0
+    assert lib.hs_profile_lu(1, None) == 0
     assert lib.hs_profile_gemm(1, None, None, None, None) == 0
     assert lib.hs_profile_panel(1, None) == 0
     f, perm = dv.getrf(torch.from_numpy(a.copy()).cuda())
     torch.cuda.synchronize()
-    assert lib.hs_profile_trail(0, *[ctypes.byref(x) for x in v]) == 0
+    assert lib.hs_profile_lu(0, v) == 0
     assert lib.hs_profile_gemm(0, *[ctypes.byref(x) for x in g]) == 0
     assert lib.hs_profile_panel(0, ticks) == 0
+    trail_ms, trail_n, trail_fl, trail_by, panel_ms, panel_n, panel_fl, worker_fl = list(v)
     # one trail_rest per block (at most 256 columns of whole panels) with a product beside the next block's panels
-    assert 0 < v[1].value < n // 128
-    assert v[0].value > 0 and g[0].value > 0 and g[1].value > 0 and sum(ticks) > 0
-    assert 0 <= v[2].value and v[2].value + g[2].value <= 2 / 3 * n ** 3
-    assert v[3].value > 0 or v[2].value == 0
+    assert 0 < trail_n < n // 128 and trail_ms > 0 and (trail_by > 0 or trail_fl == 0)
+    # every panel launch timed: the panels are whole reference leaves of at most 64 columns
+    assert n // 64 <= panel_n <= n // 16 and panel_ms > 0 and panel_fl > 0
+    assert g[0].value > 0 and g[1].value > 0 and sum(ticks) > 0
+    # the trailing tiles (workers' and trail_rest's), the product launches and the panels' own work stay within the
+    # factorization's flops (the row-of-U solves are not counted)
+    assert 0 <= trail_fl and 0 <= worker_fl
+    assert trail_fl + worker_fl + g[2].value + panel_fl <= 2 / 3 * n ** 3
     want, wperm, _ = oracle.lu(a)
     assert bits_equal(f.cpu().numpy(), want) and (perm.cpu().numpy() == wperm).all()
 

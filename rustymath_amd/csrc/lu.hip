@@ -832,6 +832,12 @@ struct TrailProfile {
         int w;
     };
     std::vector<PanelRec> panels;
+    // Events are kept for reuse once read, so that a profiled factorization creates none after the first.
+    std::vector<hipEvent_t> spare;
+    hipError_t event(hipEvent_t* e) {
+        if (!spare.empty()) { *e = spare.back(); spare.pop_back(); return hipSuccess; }
+        return hipEventCreate(e);
+    }
 } g_trail;
 
 // The flops of a panel's own factorization: per column c, the multipliers of the rows below it and their update of the
@@ -890,8 +896,8 @@ int lu_profile(int on, double* out) {
         float e = 0;
         HS_HIP(hipEventSynchronize(R.b));
         HS_HIP(hipEventElapsedTime(&e, R.a, R.b));
-        hipEventDestroy(R.a);
-        hipEventDestroy(R.b);
+        g_trail.spare.push_back(R.a);
+        g_trail.spare.push_back(R.b);
         total += e;
         std::vector<char> row_seen(R.tiles_m, 0), col_seen(R.tiles_n, 0);
         double c_items = 0, a_rows = 0, b_cols = 0;
@@ -914,8 +920,8 @@ int lu_profile(int on, double* out) {
         float e = 0;
         HS_HIP(hipEventSynchronize(P.b));
         HS_HIP(hipEventElapsedTime(&e, P.a, P.b));
-        hipEventDestroy(P.a);
-        hipEventDestroy(P.b);
+        g_trail.spare.push_back(P.a);
+        g_trail.spare.push_back(P.b);
         pms += e;
         pfl += panel_flops(P.rows, P.w);
     }
@@ -1045,8 +1051,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
                 std::unique_lock<std::mutex> lk(g_trail.mu);
                 if (g_trail.on && g_trail.panels.size() < (size_t)TrailProfile::SLOTS) {
                     TrailProfile::PanelRec P{nullptr, nullptr, rows, w};
-                    HS_HIP(hipEventCreate(&P.a));
-                    HS_HIP(hipEventCreate(&P.b));
+                    HS_HIP(g_trail.event(&P.a));
+                    HS_HIP(g_trail.event(&P.b));
                     hipExtLaunchKernelGGL(panel_kernel<T>, dim3(grid), dim3(PT), 0, s, P.a, P.b, 0, a, lda, c0, rows, w,
                                           pivots, mv, d_leaf, scratch, ex, (unsigned)(i * PW + 1), nwg, status,
                                           st->on ? st->prof : nullptr, ahead ? job : none);
@@ -1071,8 +1077,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
             std::unique_lock<std::mutex> lk(g_trail.mu);
             if (g_trail.on && g_trail.recs.size() < (size_t)TrailProfile::SLOTS) {
                 TrailProfile::Rec R{nullptr, nullptr, job.m, job.n, job.k, job.tiles_m, job.tiles_n, sizeof(T)};
-                HS_HIP(hipEventCreate(&R.a));
-                HS_HIP(hipEventCreate(&R.b));
+                HS_HIP(g_trail.event(&R.a));
+                HS_HIP(g_trail.event(&R.b));
                 job.log = g_trail.first + g_trail.recs.size();
                 hipExtLaunchKernelGGL(trail_rest<T>, dim3(grid), dim3(256), 0, s, R.a, R.b, 0, job);
                 g_trail.recs.push_back(R);

@@ -220,7 +220,8 @@ int hs_dev_dbgesv(int64_t n, const double* a, int64_t lda, const double* b, doub
  * mode 1 / 2: solves A_i X = B_i (the reference's `lu_solve`, factor.rs:633-641) or A_iᵀ X = B_i (Uᵀ, Lᵀ, then
  *   the inverse permutation; no counterpart in the reference, for adjoints) in place on B_i with the factors
  *   and perm that a mode-0 run left; `regular` may be NULL.
- * panel_width (16 or 32) and wgs_per_cu (1 or 2) are tuning knobs that leave the results unchanged. A run with
+ * panel_width (16 or 32) and wgs_per_cu (1: a workgroup of 512 threads per CU; 2: two of 256 threads, so two
+ * systems per CU) are tuning knobs that leave the results unchanged. A run with
  * graph = 1 replays a hipGraph captured at the plan's first run. The plan keeps the pointers; runs that overlap
  * on one plan race. */
 int hs_dev_dgesv_batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off,
@@ -262,8 +263,9 @@ int hs_profile_gemm(int on, double* ms, int64_t* launches, double* flops, double
  * own factorization flops and the flops of the tiles their workers computed. */
 int hs_profile_lu(int on, double* out);
 /* Profiling of the batched solves: on = 1 clears and starts per-phase timers in workgroup 0 (100 MHz ticks: setup,
- * panel columns, U rows and trailing update, regularity check and L, U), so that runs launch directly rather than
- * replay their graph; on = 0 stops. `out` (8 values, may be NULL) gets the sums, out[7] the systems timed. */
+ * panel columns, trailing update, regularity check and L, U, then in out[5] the panels' rows of U), so that runs
+ * launch directly rather than replay their graph; on = 0 stops. `out` (8 values, may be NULL) gets the sums, out[7]
+ * the systems timed. */
 int hs_profile_batched(int on, int64_t* out);
 
 #ifdef __cplusplus

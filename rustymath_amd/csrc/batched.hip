@@ -36,9 +36,8 @@ namespace hs {
 
 namespace {
 
-constexpr int BT = 512;             // threads of a workgroup = the largest order
-constexpr int WAVES = BT / 64;
-constexpr int MAXRHS = 8;           // right-hand sides per system (one wave each in the substitutions)
+constexpr int NMAX = 512;           // the largest order
+constexpr int MAXRHS = 8;           // right-hand sides per system
 constexpr int NS = 32;              // `backward`'s leaf height (factor.rs:693-707)
 constexpr int SB = 32;              // substitution block height
 
@@ -77,9 +76,11 @@ struct Mat {
 // multipliers (columns [c0, c0 + w)) and U12 its rows of U, in 32×32 blocks (2×2 MFMA tiles per wave). With at least
 // as many 32-row blocks as waves, each wave takes whole rows of blocks and keeps its rows' L21 in registers across
 // them (the update reads memory for c and U12 only); otherwise the blocks are dealt out one by one.
-template <typename T, int BW>
+// Where `mx` is not null, it also takes the largest |c| of the items read (NaN ignored): at the first panel, these are
+// the matrix's own items.
+template <typename T, int BW, int WAVES>
 __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, int w, int c1, const short* perm,
-                                                int wave, int lane) {
+                                                int wave, int lane, T* mx) {
     typedef typename Mfma<T>::acc V;
     const int m = n - c1, nb = (m + 31) / 32, last = n - 1;
     const int lr = lane & 15, lk = lane >> 4;
@@ -125,6 +126,14 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
             for (int tj = 0; tj < 2; ++tj)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) acc[ti][tj][e] = A.ld(cidx[ti][tj][e]);
+        if (mx) {
+#pragma unroll
+            for (int ti = 0; ti < 2; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) { const T v = fabs(acc[ti][tj][e]); *mx = v > *mx ? v : *mx; }
+        }
         T bv[BW / 4][2];
 #pragma unroll
         for (int s = 0; s < BW / 4; ++s) {
@@ -163,38 +172,42 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
     }
 }
 
-// BW: the widest panel (16 or 32 columns, a tuning choice; the results do not depend on it). `mode`: factor and
-// solve (perm and regular written), or solve A X = B / Aᵀ X = B with the factors and perm of an earlier run.
-// Where `active` is not null, a system whose flag is 0 is skipped (the refinement's converged systems).
-template <typename T, int BW>
-__global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc* __restrict__ desc, const int* __restrict__ order,
+// BW: the widest panel (16 or 32 columns); BT: the workgroup's threads (512: one row of a panel per thread, one
+// workgroup per CU; 256: up to two rows per thread, two workgroups and so two systems per CU, one system's
+// latency-bound pivot steps beside the other's products). Both are tuning choices; the results depend on neither.
+// `mode`: factor and solve (perm and regular written), or solve A X = B / Aᵀ X = B with the factors and perm of an
+// earlier run. Where `active` is not null, a system whose flag is 0 is skipped (the refinement's converged systems).
+template <typename T, int BW, int BT>
+__global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const Desc* __restrict__ desc, const int* __restrict__ order,
                                                           T* __restrict__ a_all, T* __restrict__ b_all,
                                                           long long* __restrict__ perm_all, int* __restrict__ regular_all,
                                                           const int* __restrict__ active, int* next, int mode,
                                                           long long* prof) {
+    constexpr int WAVES = BT / 64, R = NMAX / BT;  // R: a panel's rows per thread
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // Where `prof` is not null, thread 0 of workgroup 0 adds each phase's 100 MHz ticks into prof[0..4] and counts
-    // its systems in prof[7]: setup, panel columns, U rows and trailing update, regularity + forward, backward.
+    // Where `prof` is not null, thread 0 of workgroup 0 adds each phase's 100 MHz ticks into prof[0..5] and counts
+    // its systems in prof[7]: setup, panel columns, trailing update, regularity + forward, backward, U rows.
     const bool timed = prof != nullptr && blockIdx.x == 0 && tid == 0;
     long long tl = timed ? (long long)__builtin_amdgcn_s_memrealtime() : 0;
 #define HS_BPHASE(q) do { if (timed) { long long t_ = __builtin_amdgcn_s_memrealtime(); prof[q] += t_ - tl; tl = t_; } } while (0)
     __shared__ int s_sys;
     __shared__ unsigned long long s_max;
-    __shared__ short perm[BT];       // position -> row
-    __shared__ short leaf_end[BT];   // end of the reference leaf of each column
-    __shared__ short panel_end[BT];  // for a panel's first column, the panel's end
+    __shared__ short perm[NMAX];       // position -> row
+    __shared__ short leaf_end[NMAX];   // end of the reference leaf of each column
+    __shared__ short panel_end[NMAX];  // for a panel's first column, the panel's end
     __shared__ unsigned long long s_wkv[2][WAVES];   // each wave's best candidate, double-buffered by column
     __shared__ int s_wkp[2][WAVES];
     // Work area: the panel's candidate rows [2][WAVES][BW], its rows of U [BW][BW] and unit-lower triangle
-    // [BW][BW + 1]; or the substitutions' x [BT][MAXRHS].
+    // [BW][BW + 1]; or the substitutions' x [NMAX][MAXRHS].
     constexpr int PANEL_ITEMS = 2 * WAVES * BW + BW * BW + BW * (BW + 1);
-    constexpr int WORK = PANEL_ITEMS > BT * MAXRHS ? PANEL_ITEMS : BT * MAXRHS;
+    constexpr int WORK = PANEL_ITEMS > NMAX * MAXRHS ? PANEL_ITEMS : NMAX * MAXRHS;
     __shared__ T s_work[WORK];
     T (*s_cand)[WAVES][BW] = reinterpret_cast<T (*)[WAVES][BW]>(s_work);
     T (*s_u11)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * BW);
     T (*s_l11)[BW + 1] = reinterpret_cast<T (*)[BW + 1]>(s_work + 2 * WAVES * BW + BW * BW);
     T (*s_x)[MAXRHS] = reinterpret_cast<T (*)[MAXRHS]>(s_work);
     static_assert(BW == 16 || BW == 32, "panels hold whole 16-column leaves");
+    static_assert(BT * R == NMAX && (R == 1 || R == 2), "a workgroup of 512 or 256 threads");
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
     const T eps = sizeof(T) == 8 ? T(2.220446049250313e-16) : T(1.1920929e-07f);
 
@@ -218,33 +231,12 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
         bool solve;
 
         if (mode == FACTOR_SOLVE) {
-            if (tid < n) {
-                perm[tid] = (short)tid;
-                // The reference leaf holding column tid: lu_panel's halving while wider than 16 (factor.rs:592-616).
+            for (int i = tid; i < n; i += BT) {
+                perm[i] = (short)i;
+                // The reference leaf holding column i: lu_panel's halving while wider than 16 (factor.rs:592-616).
                 int l0 = 0, l1 = n;
-                while (l1 - l0 > 16) { const int mid = (l0 + l1) / 2; if (tid < mid) l1 = mid; else l0 = mid; }
-                leaf_end[tid] = (short)l1;
-            }
-            {
-                // `greatest` magnitude of the matrix, NaN ignored (factor.rs:488), for `regular`.
-                T mq[8];
-#pragma unroll
-                for (int t = 0; t < 8; ++t) mq[t] = T(0);
-                const int nn = n * n;
-                for (int i = tid; i < nn; i += 8 * BT) {
-#pragma unroll
-                    for (int t = 0; t < 8; ++t) {
-                        const int j = i + t * BT;
-                        const T v = j < nn ? fabs(a[j]) : T(0);
-                        mq[t] = v > mq[t] ? v : mq[t];
-                    }
-                }
-                T m = mq[0];
-#pragma unroll
-                for (int t = 1; t < 8; ++t) m = mq[t] > m ? mq[t] : m;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) { const T o = __shfl_xor(m, off); m = o > m ? o : m; }
-                if (lane == 0) atomicMax(&s_max, (unsigned long long)__builtin_bit_cast(typename Bits<T>::U, m));
+                while (l1 - l0 > 16) { const int mid = (l0 + l1) / 2; if (i < mid) l1 = mid; else l0 = mid; }
+                leaf_end[i] = (short)l1;
             }
             __syncthreads();
             if (tid == 0) {
@@ -259,25 +251,54 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
             HS_BPHASE(0);
 
             // ---- lu_factor ----
+            T big = T(0);  // this thread's share of the matrix's greatest magnitude
             for (int c0 = 0; c0 < n;) {
                 const int c1 = panel_end[c0], w = c1 - c0, rows = n - c0;
-                const bool has = tid < rows;
-                const int row = perm[c0 + (has ? tid : 0)];
-                int pos = tid;
-                T v[BW];
+                // Slot r of a thread holds the row at panel position tid + r·BT.
+                bool has[R];
+                int row[R], pos[R];
+                T v[R][BW];
 #pragma unroll
-                for (int j = 0; j < BW; ++j) v[j] = A.ld(A.at(has && j < w, row * n + c0 + j));
+                for (int r = 0; r < R; ++r) {
+                    const int t = tid + r * BT;
+                    has[r] = t < rows;
+                    row[r] = perm[c0 + (has[r] ? t : 0)];
+                    pos[r] = t;
+#pragma unroll
+                    for (int j = 0; j < BW; ++j) v[r][j] = A.ld(A.at(has[r] && j < w, row[r] * n + c0 + j));
+                }
+                // `greatest` magnitude of the matrix, NaN ignored (factor.rs:488), for `regular`: the first panel's
+                // loads, its rows of U right of it and its trailing update read every item of A once, as it was.
+                if (c0 == 0) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+#pragma unroll
+                        for (int j = 0; j < BW; ++j) { const T x = fabs(v[r][j]); big = x > big ? x : big; }
+                }
 #pragma clang loop unroll(disable)
                 for (int c = 0; c < w; ++c) {
                     const int buf = c & 1;
-                    // First greatest magnitude at positions >= c, NaN ignored; all NaN -> position c (factor.rs:476-490).
-                    const bool cnd = has && pos >= c;
-                    unsigned long long kv = cnd ? mag_key(v[0]) : 0ull;
-                    int kp = cnd ? pos : 0x7fffffff, hl = 0;
+                    // First greatest magnitude at positions >= c, NaN ignored; all NaN -> position c (factor.rs:476-490):
+                    // the thread's best slot (larger key, then smaller position), then the wave's.
+                    unsigned long long kv = 0ull;
+                    int kp = 0x7fffffff, kr = 0;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const bool cnd = has[r] && pos[r] >= c;
+                        const unsigned long long k2 = cnd ? mag_key(v[r][0]) : 0ull;
+                        const int p2 = cnd ? pos[r] : 0x7fffffff;
+                        if (k2 > kv || (k2 == kv && p2 < kp)) { kv = k2; kp = p2; kr = r; }
+                    }
+                    int hl = 0;
                     wave_best(kv, kp, hl);
                     if (lane == hl) {
+                        if (kr == 0) {
 #pragma unroll
-                        for (int j = 0; j < BW; ++j) s_cand[buf][wave][j] = v[j];
+                            for (int j = 0; j < BW; ++j) s_cand[buf][wave][j] = v[0][j];
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < BW; ++j) s_cand[buf][wave][j] = v[R - 1][j];
+                        }
                     }
                     if (lane == 0) { s_wkv[buf][wave] = kv; s_wkp[buf][wave] = kp; }
                     __syncthreads();
@@ -293,31 +314,65 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     asm volatile("" : "+v"(mine));
                     if (tid < BW) s_u11[c][tid] = piv[tid];
                     const T pivot = lane_value(mine, 0);
-                    if (has && pos == p) pos = c;
-                    else if (has && pos == c) pos = p;
-                    if (has && pos > c) {
-                        // divide (factor.rs:506-508), nothing for a zero pivot, whose leaf then skips its update (:606).
-                        T l = v[0];
-                        if (pivot != T(0)) {
-                            l = fabs(pivot) >= tiny ? l * (T(1) / pivot) : l / pivot;
+                    // Rows at positions > c take the column's multiplier and update. Every slot computes (those of
+                    // pivoted rows or past the panel's rows are never read again), so the update has no branch.
+                    // divide (factor.rs:506-508); a zero pivot leaves the column undivided and skips the rest of its
+                    // own leaf (:606): the shared pass then takes +0 for both the multiplier and the pivot row's
+                    // items, and v + (+0)·(−0) = v + (−0) is v exactly, so it only shifts the items; the items of the
+                    // leaf's later columns take their updates after it, in place.
+                    const bool nz = pivot != T(0);
+                    T l[R], lm[R];
+                    bool upd[R];
 #pragma unroll
-                            for (int j0 = 1; j0 < BW; j0 += 16) {
-                                if (j0 < w - c) {  // items past the panel are never used
+                    for (int r = 0; r < R; ++r) {
+                        if (has[r] && pos[r] == p) pos[r] = c;
+                        else if (has[r] && pos[r] == c) pos[r] = p;
+                        upd[r] = has[r] && pos[r] > c;
+                        l[r] = v[r][0];
+                    }
+                    if (nz) {
+                        if (fabs(pivot) >= tiny) {
+                            const T rp = T(1) / pivot;
 #pragma unroll
-                                    for (int j = j0; j < j0 + 16 && j < BW; ++j)
-                                        v[j - 1] = __builtin_elementwise_fma(l, -lane_value(mine, j), v[j]);
-                                }
-                            }
+                            for (int r = 0; r < R; ++r) l[r] = l[r] * rp;
                         } else {
-                            const int skip = leaf_end[c0 + c] - c0 - c;
 #pragma unroll
-                            for (int j = 1; j < BW; ++j) v[j - 1] = j >= skip ? __builtin_elementwise_fma(l, -piv[j], v[j]) : v[j];
+                            for (int r = 0; r < R; ++r) l[r] = l[r] / pivot;
                         }
-                        v[BW - 1] = T(0);
-                        a[row * n + c0 + c] = l;
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) lm[r] = nz ? l[r] : T(0);
+#pragma unroll
+                    for (int j0 = 1; j0 < BW; j0 += 16) {
+                        if (j0 < w - c) {  // items past the panel are never used
+#pragma unroll
+                            for (int j = j0; j < j0 + 16 && j < BW; ++j) {
+                                const T uj = nz ? lane_value(mine, j) : T(0);
+#pragma unroll
+                                for (int r = 0; r < R; ++r) v[r][j - 1] = __builtin_elementwise_fma(lm[r], -uj, v[r][j]);
+                            }
+                        }
+                    }
+                    if (!nz) {
+                        const int skip = leaf_end[c0 + c] - c0 - c;
+#pragma unroll
+                        for (int j = 1; j < BW; ++j) {
+                            if (j >= skip) {
+                                const T uj = piv[j];
+#pragma unroll
+                                for (int r = 0; r < R; ++r) v[r][j - 1] = __builtin_elementwise_fma(l[r], -uj, v[r][j - 1]);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        v[r][BW - 1] = T(0);
+                        if (upd[r]) a[row[r] * n + c0 + c] = l[r];
                     }
                 }
-                if (has) perm[c0 + pos] = (short)row;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (has[r]) perm[c0 + pos[r]] = (short)row[r];
                 __syncthreads();
                 HS_BPHASE(1);
                 // The panel's rows of U, from LDS to their rows.
@@ -326,29 +381,51 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     if (cc < w && j < w - cc) a[perm[c0 + cc] * n + c0 + cc + j] = s_u11[cc][j];
                 }
                 if (c1 < n) {
-                    for (int i = tid; i < BW * BW; i += BT) {
-                        const int r = i / BW, cq = i % BW;
-                        s_l11[r][cq] = (r < w && cq < r) ? a[perm[c0 + r] * n + c0 + cq] : T(0);
+                    // The panel's unit-lower triangle into LDS, every load issued before any is stored.
+                    constexpr int LF = (BW * BW + BT - 1) / BT;
+                    T lv[LF];
+#pragma unroll
+                    for (int t = 0; t < LF; ++t) {
+                        const int i = tid + t * BT, r = i / BW, cq = i % BW;
+                        const bool ok = i < BW * BW && r < w && cq < r;
+                        lv[t] = A.ld(A.at(ok, perm[c0 + (ok ? r : 0)] * n + c0 + cq));
                     }
-                    // U12: the panel's unit-lower solve for each column right of it, in order, one column per thread.
-                    // u[j] holds row qq + j of the column: step qq finishes row qq and shifts the rest down.
-                    const int col = c1 + tid;
-                    const bool live = col < n;
-                    const int colc = live ? col : n - 1;
+                    // U12: the panel's unit-lower solve for each column right of it, in order, one column per thread
+                    // (two in turn at 256 threads); the first column's loads are in flight across the barrier.
                     T u[BW];
+                    int col = c1 + tid;
 #pragma unroll
-                    for (int r = 0; r < BW; ++r) u[r] = A.ld(A.at(live && r < w, perm[c0 + (r < w ? r : 0)] * n + colc));
-                    __syncthreads();
-#pragma clang loop unroll(disable)
-                    for (int qq = 0; qq < w; ++qq) {
-                        const T x = u[0];
-                        if (live) a[perm[c0 + qq] * n + col] = x;
+                    for (int j = 0; j < BW; ++j) u[j] = A.ld(A.at(col < n && j < w, perm[c0 + (j < w ? j : 0)] * n + col));
 #pragma unroll
-                        for (int j = 1; j < BW; ++j) u[j - 1] = __builtin_elementwise_fma(-s_l11[qq + j < BW ? qq + j : BW - 1][qq], x, u[j]);
-                        u[BW - 1] = T(0);
+                    for (int t = 0; t < LF; ++t) {
+                        const int i = tid + t * BT;
+                        if (i < BW * BW) s_l11[i / BW][i % BW] = lv[t];
                     }
                     __syncthreads();
-                    trailing_update<T, BW>(A, n, c0, w, c1, perm, wave, lane);
+                    while (col < n) {
+                        if (c0 == 0) {
+#pragma unroll
+                            for (int j = 0; j < BW; ++j) { const T x = fabs(u[j]); big = x > big ? x : big; }
+                        }
+                        // u[j] holds row qq + j of the column: step qq finishes row qq and shifts the rest down.
+#pragma clang loop unroll(disable)
+                        for (int qq = 0; qq < w; ++qq) {
+                            const T x = u[0];
+                            a[perm[c0 + qq] * n + col] = x;
+#pragma unroll
+                            for (int j = 1; j < BW; ++j)
+                                u[j - 1] = __builtin_elementwise_fma(-s_l11[qq + j < BW ? qq + j : BW - 1][qq], x, u[j]);
+                            u[BW - 1] = T(0);
+                        }
+                        col += BT;
+                        if (col < n) {
+#pragma unroll
+                            for (int j = 0; j < BW; ++j) u[j] = A.ld(A.at(j < w, perm[c0 + (j < w ? j : 0)] * n + col));
+                        }
+                    }
+                    __syncthreads();
+                    HS_BPHASE(5);
+                    trailing_update<T, BW, WAVES>(A, n, c0, w, c1, perm, wave, lane, c0 == 0 ? &big : nullptr);
                 }
                 __syncthreads();
                 HS_BPHASE(2);
@@ -356,14 +433,20 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
             }
 
             // ---- regular (factor.rs:221-226): no |u_ii| at or below eps · n · greatest ----
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) { const T o = __shfl_xor(big, off); big = o > big ? o : big; }
+            if (lane == 0) atomicMax(&s_max, (unsigned long long)__builtin_bit_cast(typename Bits<T>::U, big));
+            __syncthreads();
             const T largest = __builtin_bit_cast(T, (typename Bits<T>::U)s_max);
             const T negligible = eps * T(n) * largest;
-            const int ok = __syncthreads_and(!(tid < n) || !(fabs(a[perm[tid] * n + tid]) <= negligible));
-            if (tid < n) gperm[tid] = perm[tid];
+            int mine_ok = 1;
+            for (int i = tid; i < n; i += BT) mine_ok &= !(fabs(a[perm[i] * n + i]) <= negligible);
+            const int ok = __syncthreads_and(mine_ok);
+            for (int i = tid; i < n; i += BT) gperm[i] = perm[i];
             if (tid == 0) regular_all[sys] = ok && n > 0;
             solve = ok && n > 0 && k > 0;
         } else {
-            if (tid < n) perm[tid] = (short)gperm[tid];
+            for (int i = tid; i < n; i += BT) perm[i] = (short)gperm[i];
             __syncthreads();
             solve = n > 0 && k > 0;
         }
@@ -380,40 +463,43 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                 s_x[r][c] = c < k ? b[(tr ? r : perm[r]) * k + c] : T(0);
             }
             __syncthreads();
-            // Forward: L (unit) or Uᵀ (a reciprocal multiply at each row's turn), increasing order.
+            // Forward: L (unit) or Uᵀ (a reciprocal multiply at each row's turn), increasing order. A diagonal block
+            // per right-hand side, one wave each (in turn where there are more columns than waves).
             for (int p0 = 0; p0 < n; p0 += SB) {
                 const int h = n - p0 < SB ? n - p0 : SB;
-                if (wave < k && lane < SB) {
-                    const bool act = lane < h;
-                    const int r = p0 + (act ? lane : 0);
-                    T x = act ? s_x[r][wave] : T(0);
-                    T mj[SB];
+                for (int cc = wave; cc < k; cc += WAVES) {
+                    if (lane < SB) {
+                        const bool act = lane < h;
+                        const int r = p0 + (act ? lane : 0);
+                        T x = act ? s_x[r][cc] : T(0);
+                        T mj[SB];
 #pragma unroll
-                    for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j < lane, r, p0 + (j < h ? j : 0)));
-                    const T dgl = ent(r, r);
-                    const T dg = (tr && act) ? dgl : T(1);
+                        for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j < lane, r, p0 + (j < h ? j : 0)));
+                        const T dgl = ent(r, r);
+                        const T dg = (tr && act) ? dgl : T(1);
 #pragma unroll
-                    for (int j = 0; j < SB; ++j) {
-                        if (j < h) {
-                            if (tr && lane == j) x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
-                            const T xj = lane_value(x, j);
-                            if (act && lane > j) x = __builtin_elementwise_fma(xj, -mj[j], x);
+                        for (int j = 0; j < SB; ++j) {
+                            if (j < h) {
+                                if (tr && lane == j) x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
+                                const T xj = lane_value(x, j);
+                                if (act && lane > j) x = __builtin_elementwise_fma(xj, -mj[j], x);
+                            }
                         }
+                        if (act) s_x[r][cc] = x;
                     }
-                    if (act) s_x[r][wave] = x;
                 }
                 __syncthreads();
-                if (tid >= p0 + h && tid < n) {
+                for (int i = p0 + h + tid; i < n; i += BT) {
                     T e[SB];
 #pragma unroll
-                    for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, tid, p0 + (j < h ? j : 0)));
+                    for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, i, p0 + (j < h ? j : 0)));
 #pragma unroll
                     for (int c = 0; c < MAXRHS; ++c) {
                         if (c < k) {
-                            T acc = s_x[tid][c];
+                            T acc = s_x[i][c];
 #pragma unroll
                             for (int j = 0; j < SB; ++j) if (j < h) acc = __builtin_elementwise_fma(s_x[p0 + j][c], e[j], acc);
-                            s_x[tid][c] = acc;
+                            s_x[i][c] = acc;
                         }
                     }
                 }
@@ -425,38 +511,40 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                 // Lᵀ (unit). Diagonal blocks sit right-aligned in the wave: lane SB - 1 holds the block's last row.
                 for (int q1 = n; q1 > 0; q1 -= SB) {
                     const int q0 = q1 - SB > 0 ? q1 - SB : 0, h = q1 - q0, off = SB - h;
-                    if (wave < k && lane < SB) {
-                        const bool act = lane >= off;
-                        const int r = act ? q0 + lane - off : q0;
-                        T x = act ? s_x[r][wave] : T(0);
-                        T mj[SB];
+                    for (int cc = wave; cc < k; cc += WAVES) {
+                        if (lane < SB) {
+                            const bool act = lane >= off;
+                            const int r = act ? q0 + lane - off : q0;
+                            T x = act ? s_x[r][cc] : T(0);
+                            T mj[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j > lane, r, j >= off ? q0 + j - off : q0));
-                        const T dgl = ent(r, r);
-                        const T dg = (!tr && act) ? dgl : T(1);
+                            for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j > lane, r, j >= off ? q0 + j - off : q0));
+                            const T dgl = ent(r, r);
+                            const T dg = (!tr && act) ? dgl : T(1);
 #pragma unroll
-                        for (int s = 0; s < SB; ++s) {
-                            const int L = SB - 1 - s;
-                            if (s < h) {
-                                if (!tr && lane == L) x = x / dg;
-                                const T xl = lane_value(x, L);
-                                if (act && lane < L) x = __builtin_elementwise_fma(xl, -mj[L], x);
+                            for (int s = 0; s < SB; ++s) {
+                                const int L = SB - 1 - s;
+                                if (s < h) {
+                                    if (!tr && lane == L) x = x / dg;
+                                    const T xl = lane_value(x, L);
+                                    if (act && lane < L) x = __builtin_elementwise_fma(xl, -mj[L], x);
+                                }
                             }
+                            if (act) s_x[r][cc] = x;
                         }
-                        if (act) s_x[r][wave] = x;
                     }
                     __syncthreads();
-                    if (tid < q0) {
+                    for (int i = tid; i < q0; i += BT) {
                         T e[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, tid, q0 + (j < h ? j : 0)));
+                        for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, i, q0 + (j < h ? j : 0)));
 #pragma unroll
                         for (int c = 0; c < MAXRHS; ++c) {
                             if (c < k) {
-                                T acc = s_x[tid][c];
+                                T acc = s_x[i][c];
 #pragma unroll
                                 for (int j = SB - 1; j >= 0; --j) if (j < h) acc = __builtin_elementwise_fma(s_x[q0 + j][c], e[j], acc);
-                                s_x[tid][c] = acc;
+                                s_x[i][c] = acc;
                             }
                         }
                     }
@@ -470,48 +558,50 @@ __global__ __launch_bounds__(BT) void gesv_batched_kernel(int count, const Desc*
                     int r0 = 0, r1 = n;
                     while (r1 - r0 > NS) { const int mid = (r0 + r1) / 2; if (end <= mid) r1 = mid; else r0 = mid; }
                     const int h = r1 - r0, off = SB - h;
-                    if (wave < k && lane < SB) {
-                        const bool act = lane >= off;
-                        const int r = act ? r0 + lane - off : r0;
-                        T x = act ? s_x[r][wave] : T(0);
-                        T mj[SB];
+                    for (int cc = wave; cc < k; cc += WAVES) {
+                        if (lane < SB) {
+                            const bool act = lane >= off;
+                            const int r = act ? r0 + lane - off : r0;
+                            T x = act ? s_x[r][cc] : T(0);
+                            T mj[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) mj[j] = A.ld(A.at(act && j > lane, perm[r] * n + (j >= off ? r0 + j - off : r0)));
-                        const T dgl = a[perm[r] * n + r];
-                        const T dg = act ? dgl : T(1);
-                        T xs[SB];
+                            for (int j = 0; j < SB; ++j) mj[j] = A.ld(A.at(act && j > lane, perm[r] * n + (j >= off ? r0 + j - off : r0)));
+                            const T dgl = a[perm[r] * n + r];
+                            const T dg = act ? dgl : T(1);
+                            T xs[SB];
 #pragma unroll
-                        for (int s = 0; s < SB; ++s) {
-                            const int L = SB - 1 - s;
-                            if (s < h) {
-                                if (lane == L) {
+                            for (int s = 0; s < SB; ++s) {
+                                const int L = SB - 1 - s;
+                                if (s < h) {
+                                    if (lane == L) {
 #pragma unroll
-                                    for (int j = L + 1; j < SB; ++j) x = __builtin_elementwise_fma(xs[j], -mj[j], x);
-                                    x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
+                                        for (int j = L + 1; j < SB; ++j) x = __builtin_elementwise_fma(xs[j], -mj[j], x);
+                                        x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
+                                    }
+                                    xs[L] = lane_value(x, L);
                                 }
-                                xs[L] = lane_value(x, L);
                             }
+                            if (act) s_x[r][cc] = x;
                         }
-                        if (act) s_x[r][wave] = x;
                     }
                     __syncthreads();
                     int a0 = 0, a1 = n;
                     while (a1 - a0 > NS) {
                         const int mid = (a0 + a1) / 2;
                         if (mid == r0) {
-                            if (tid >= a0 && tid < mid) {
+                            for (int i = a0 + tid; i < mid; i += BT) {
                                 for (int j0 = mid; j0 < a1; j0 += SB) {
                                     T e[SB];
 #pragma unroll
-                                    for (int j = 0; j < SB; ++j) e[j] = -A.ld(A.at(j0 + j < a1, perm[tid] * n + j0 + j));
+                                    for (int j = 0; j < SB; ++j) e[j] = -A.ld(A.at(j0 + j < a1, perm[i] * n + j0 + j));
 #pragma unroll
                                     for (int c = 0; c < MAXRHS; ++c) {
                                         if (c < k) {
-                                            T acc = s_x[tid][c];
+                                            T acc = s_x[i][c];
 #pragma unroll
                                             for (int j = 0; j < SB; ++j)
                                                 if (j0 + j < a1) acc = __builtin_elementwise_fma(s_x[j0 + j][c], e[j], acc);
-                                            s_x[tid][c] = acc;
+                                            s_x[i][c] = acc;
                                         }
                                     }
                                 }
@@ -595,7 +685,7 @@ __global__ __launch_bounds__(IRT) void ir_step_kernel(int count, const Desc* __r
                                                       int* __restrict__ iters, double* __restrict__ berr,
                                                       const int* __restrict__ regular, int first, int last,
                                                       double tol_per_n) {
-    __shared__ double xs[BT][MAXRHS];
+    __shared__ double xs[NMAX][MAXRHS];
     __shared__ double red[4];
     const int sys = blockIdx.x;
     if (sys >= count) return;
@@ -675,6 +765,7 @@ struct Plan {
     int* d_next = nullptr;
     int grid = 0;
     int bw = 32;
+    int wgs = 1;
     void *a = nullptr, *b = nullptr;
     long long* perm = nullptr;
     int* regular = nullptr;
@@ -702,12 +793,13 @@ long long g_next_plan = 1;
 template <typename T>
 int enqueue_mode(Plan* p, int mode, const int* active, hipStream_t s, long long* prof) {
     zero_int<<<1, 1, 0, s>>>(p->d_next);
-    if (p->bw == 16)
-        gesv_batched_kernel<T, 16><<<p->grid, BT, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, (T*)p->b, p->perm,
-                                                         p->regular, active, p->d_next, mode, prof);
-    else
-        gesv_batched_kernel<T, 32><<<p->grid, BT, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, (T*)p->b, p->perm,
-                                                         p->regular, active, p->d_next, mode, prof);
+    // One workgroup of 512 threads per CU, or two of 256.
+    const dim3 grid(p->grid);
+#define HS_BLAUNCH(bw, bt) gesv_batched_kernel<T, bw, bt><<<grid, bt, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, \
+        (T*)p->b, p->perm, p->regular, active, p->d_next, mode, prof)
+    if (p->bw == 16) { if (p->wgs == 2) HS_BLAUNCH(16, 256); else HS_BLAUNCH(16, 512); }
+    else { if (p->wgs == 2) HS_BLAUNCH(32, 256); else HS_BLAUNCH(32, 512); }
+#undef HS_BLAUNCH
     HS_HIP(hipGetLastError());
     return OK;
 }
@@ -753,7 +845,7 @@ int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_
     std::vector<Desc> desc(count);
     std::vector<int> order(count);
     for (int64_t i = 0; i < count; ++i) {
-        if (n[i] < 0 || n[i] > BT) { set_error("batched systems need 0 <= n <= 512"); return -2; }
+        if (n[i] < 0 || n[i] > NMAX) { set_error("batched systems need 0 <= n <= 512"); return -2; }
         if (k[i] < 0 || k[i] > MAXRHS) { set_error("batched systems need 0 <= k <= 8"); return -3; }
         if (a_off[i] < 0 || b_off[i] < 0 || p_off[i] < 0) { set_error("batched offsets must be >= 0"); return -4; }
         desc[i] = Desc{n[i], k[i], a_off[i], b_off[i], p_off[i]};
@@ -773,6 +865,7 @@ int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_
     p->regular = regular;
     p->grid = device_cus() * wgs_per_cu;
     p->bw = panel_width;
+    p->wgs = wgs_per_cu;
     const size_t cnt = count > 0 ? (size_t)count : 1;
     if (hipMalloc(&p->d_desc, sizeof(Desc) * cnt) != hipSuccess || hipMalloc(&p->d_order, sizeof(int) * cnt) != hipSuccess ||
         hipMalloc(&p->d_next, sizeof(int)) != hipSuccess) {

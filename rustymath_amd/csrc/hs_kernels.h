@@ -88,7 +88,7 @@ int gesv_gmres_ir(int64_t n, const double* a, int64_t lda, const double* b, doub
 // perm_i at perm + p_off[i]; regular[i] = 1 where the LU solve ran, 0 where rustymath would take least
 // squares. Host arrays describe device buffers. Mode 0 factors and solves; modes 1 and 2 solve A X = B and
 // Aᵀ X = B with the factors and perm that a mode-0 run left. Tuning knobs: panel width (16 or 32) and
-// workgroups per CU (1 or 2). A run replays the plan's hipGraph (or launches directly) on `s`.
+// workgroups per CU (1 of 512 threads, or 2 of 256). A run replays the plan's hipGraph (or launches directly) on `s`.
 int batched_plan(int64_t count, const int64_t* n, const int64_t* k, const int64_t* a_off, const int64_t* b_off,
                  const int64_t* p_off, bool f64, void* a, void* b, int64_t* perm, int* regular, int mode,
                  int panel_width, int wgs_per_cu, long long* handle);

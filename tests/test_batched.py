@@ -58,13 +58,18 @@ def check_cell(a, b, got, what):
     else: assert bits_equal(x, b), f'{what}: b must stay as it was'
 
 
+# Each (panel width, workgroups per CU) form: 512 threads with one panel row each, or 256 with up to two.
+FORMS = [(32, 1), (16, 1), (32, 2), (16, 2)]
+
+
+@pytest.mark.parametrize('form', FORMS)
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
-def test_every_order_matches_the_oracle(dv, dtype):
+def test_every_order_matches_the_oracle(dv, dtype, form):
     "Orders across the leaf (16) and backward-leaf (32) boundaries up to 512, 0 to 8 right-hand sides, N(0, 1) entries."
     rng = np.random.default_rng(11)
     mats = [rng.standard_normal((n, n)).astype(dtype) for n in EDGE_NS]
     rhs = [rng.standard_normal((n, i % 9)).astype(dtype) for i, n in enumerate(EDGE_NS)]
-    got = solve_batch(dv, mats, rhs, dtype)
+    got = solve_batch(dv, mats, rhs, dtype, panel_width=form[0], wgs_per_cu=form[1])
     for i, (a, b) in enumerate(zip(mats, rhs)):
         check_cell(a, b, got[i], f'n={a.shape[0]} k={b.shape[1]}')
 
@@ -86,11 +91,16 @@ def special_cells(dtype):
     a = np.eye(33, dtype=dtype)[::-1].copy(); cells.append(('reversed identity', a))
     a = np.triu(rng.standard_normal((50, 50))).astype(dtype); cells.append(('upper triangular', a))
     a = (rng.standard_normal((60, 5)) @ rng.standard_normal((5, 60))).astype(dtype); cells.append(('rank 5', a))
+    # Above order 256, where the 256-thread form holds two rows of a panel per thread.
+    a = (rng.integers(-2, 3, size=(300, 300))).astype(dtype); cells.append(('small integers, order 300', a))
+    a = rng.standard_normal((400, 400)).astype(dtype); a[:, 100] = 0; a[:, 260] = 0; a[:, 261] = 0
+    cells.append(('zero columns, order 400', a))
     return cells
 
 
+@pytest.mark.parametrize('form', FORMS)
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
-def test_special_matrices_match_the_oracle(dv, dtype):
+def test_special_matrices_match_the_oracle(dv, dtype, form):
     rng = np.random.default_rng(6)
     cells = special_cells(dtype)
     mats = [a for _, a in cells]
@@ -99,7 +109,7 @@ def test_special_matrices_match_the_oracle(dv, dtype):
         b = rng.standard_normal((a.shape[0], 1 + i % 8))
         if 'subnormal' in cells[i][0]: b = b * np.ldexp(1.0, -1060 if dtype == np.float64 else -140)
         rhs.append(b.astype(dtype))
-    got = solve_batch(dv, mats, rhs, dtype)
+    got = solve_batch(dv, mats, rhs, dtype, panel_width=form[0], wgs_per_cu=form[1])
     for i, (name, a) in enumerate(cells):
         check_cell(a, rhs[i], got[i], name)
 

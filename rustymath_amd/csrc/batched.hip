@@ -195,7 +195,9 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
     __shared__ short perm[NMAX];       // position -> row
     __shared__ short leaf_end[NMAX];   // end of the reference leaf of each column
     __shared__ short panel_end[NMAX];  // for a panel's first column, the panel's end
-    __shared__ unsigned long long s_wkv[2][WAVES];   // each wave's best candidate, double-buffered by column
+    // Magnitude keys: an FP32 item's fits in 32 bits, so its wave maximum takes one DPP pass.
+    typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type Key;
+    __shared__ Key s_wkv[2][WAVES];   // each wave's best candidate, double-buffered by column
     __shared__ int s_wkp[2][WAVES];
     // Work area: the panel's candidate rows [2][WAVES][BW], its rows of U [BW][BW] and unit-lower triangle
     // [BW][BW + 1]; or the substitutions' x [NMAX][MAXRHS].
@@ -204,7 +206,9 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
     __shared__ T s_work[WORK];
     T (*s_cand)[WAVES][BW] = reinterpret_cast<T (*)[WAVES][BW]>(s_work);
     T (*s_u11)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * BW);
-    T (*s_l11)[BW + 1] = reinterpret_cast<T (*)[BW + 1]>(s_work + 2 * WAVES * BW + BW * BW);
+    // s_l11n[q][j] = −L11[q + 1 + j][q]: column q of the unit-lower triangle below its diagonal, negated and shifted
+    // to the row start, so that step q of the U12 solve reads one row from one scalar offset.
+    T (*s_l11n)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * BW + BW * BW);
     T (*s_x)[MAXRHS] = reinterpret_cast<T (*)[MAXRHS]>(s_work);
     static_assert(BW == 16 || BW == 32, "panels hold whole 16-column leaves");
     static_assert(BT * R == NMAX && (R == 1 || R == 2), "a workgroup of 512 or 256 threads");
@@ -280,12 +284,12 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     const int buf = c & 1;
                     // First greatest magnitude at positions >= c, NaN ignored; all NaN -> position c (factor.rs:476-490):
                     // the thread's best slot (larger key, then smaller position), then the wave's.
-                    unsigned long long kv = 0ull;
+                    Key kv = 0;
                     int kp = 0x7fffffff, kr = 0;
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
                         const bool cnd = has[r] && pos[r] >= c;
-                        const unsigned long long k2 = cnd ? mag_key(v[r][0]) : 0ull;
+                        const Key k2 = cnd ? (Key)mag_key(v[r][0]) : Key(0);
                         const int p2 = cnd ? pos[r] : 0x7fffffff;
                         if (k2 > kv || (k2 == kv && p2 < kp)) { kv = k2; kp = p2; kr = r; }
                     }
@@ -302,11 +306,19 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     }
                     if (lane == 0) { s_wkv[buf][wave] = kv; s_wkp[buf][wave] = kp; }
                     __syncthreads();
-                    // The waves' bests: lane u of every wave reads wave u's, and a wave reduction picks the winner
-                    // (rather than every thread reading every wave's key from LDS).
-                    unsigned long long bkv = lane < WAVES ? s_wkv[buf][lane] : 0ull;
-                    int p = lane < WAVES ? s_wkp[buf][lane] : 0x7fffffff, win = 0;
-                    wave_best(bkv, p, win);
+                    // The waves' bests: every lane reads them all from LDS (one broadcast each) and keeps the first
+                    // greatest key, then the smallest position; no cross-lane steps.
+                    Key bkv = s_wkv[buf][0];
+                    int p = s_wkp[buf][0], win = 0;
+#pragma unroll
+                    for (int u = 1; u < WAVES; ++u) {
+                        const Key ku = s_wkv[buf][u];
+                        const int pu = s_wkp[buf][u];
+                        const bool better = ku > bkv || (ku == bkv && pu < p);
+                        bkv = better ? ku : bkv;
+                        p = better ? pu : p;
+                        win = better ? u : win;
+                    }
                     const T* piv = s_cand[buf][win];
                     // The pivot row as scalar operands: lane j reads item j (every lane active), then reads go lane
                     // to scalar.
@@ -398,8 +410,8 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     for (int j = 0; j < BW; ++j) u[j] = A.ld(A.at(col < n && j < w, perm[c0 + (j < w ? j : 0)] * n + col));
 #pragma unroll
                     for (int t = 0; t < LF; ++t) {
-                        const int i = tid + t * BT;
-                        if (i < BW * BW) s_l11[i / BW][i % BW] = lv[t];
+                        const int i = tid + t * BT, r = i / BW, cq = i % BW;
+                        if (i < BW * BW && r > cq) s_l11n[cq][r - cq - 1] = -lv[t];
                     }
                     __syncthreads();
                     while (col < n) {
@@ -407,15 +419,20 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
 #pragma unroll
                             for (int j = 0; j < BW; ++j) { const T x = fabs(u[j]); big = x > big ? x : big; }
                         }
-                        // u[j] holds row qq + j of the column: step qq finishes row qq and shifts the rest down.
-#pragma clang loop unroll(disable)
-                        for (int qq = 0; qq < w; ++qq) {
-                            const T x = u[0];
-                            a[perm[c0 + qq] * n + col] = x;
+                        // u[j] holds row qq + j of the column: step qq finishes row qq and shifts the rest down. In
+                        // the P-th quarter of the steps only the rows below it (j < BW − P·BW/4) are live.
 #pragma unroll
-                            for (int j = 1; j < BW; ++j)
-                                u[j - 1] = __builtin_elementwise_fma(-s_l11[qq + j < BW ? qq + j : BW - 1][qq], x, u[j]);
-                            u[BW - 1] = T(0);
+                        for (int P = 0; P < 4; ++P) {
+                            constexpr int QW = BW / 4;
+                            const int q1 = (P + 1) * QW < w ? (P + 1) * QW : w;
+#pragma clang loop unroll(disable)
+                            for (int qq = P * QW; qq < q1; ++qq) {
+                                const T x = u[0];
+                                a[perm[c0 + qq] * n + col] = x;
+                                const T* lq = s_l11n[qq];
+#pragma unroll
+                                for (int j = 1; j < BW - P * QW; ++j) u[j - 1] = __builtin_elementwise_fma(lq[j - 1], x, u[j]);
+                            }
                         }
                         col += BT;
                         if (col < n) {
@@ -467,6 +484,11 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
             // per right-hand side, one wave each (in turn where there are more columns than waves).
             for (int p0 = 0; p0 < n; p0 += SB) {
                 const int h = n - p0 < SB ? n - p0 : SB;
+                // The block's terms for this thread's first row below it, in flight across the diagonal block.
+                const int i0 = p0 + h + tid, i0c = i0 < n ? i0 : n - 1;
+                T e0[SB];
+#pragma unroll
+                for (int j = 0; j < SB; ++j) e0[j] = A.ld(ent_at(j < h && i0 < n, i0c, p0 + (j < h ? j : 0)));
                 for (int cc = wave; cc < k; cc += WAVES) {
                     if (lane < SB) {
                         const bool act = lane < h;
@@ -489,10 +511,10 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     }
                 }
                 __syncthreads();
-                for (int i = p0 + h + tid; i < n; i += BT) {
+                for (int i = i0; i < n; i += BT) {
                     T e[SB];
 #pragma unroll
-                    for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, i, p0 + (j < h ? j : 0)));
+                    for (int j = 0; j < SB; ++j) e[j] = -(i == i0 ? e0[j] : A.ld(ent_at(j < h, i, p0 + (j < h ? j : 0))));
 #pragma unroll
                     for (int c = 0; c < MAXRHS; ++c) {
                         if (c < k) {
@@ -511,6 +533,11 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                 // Lᵀ (unit). Diagonal blocks sit right-aligned in the wave: lane SB - 1 holds the block's last row.
                 for (int q1 = n; q1 > 0; q1 -= SB) {
                     const int q0 = q1 - SB > 0 ? q1 - SB : 0, h = q1 - q0, off = SB - h;
+                    // The block's terms for this thread's first row above it, in flight across the diagonal block.
+                    const int i0c = tid < q0 ? tid : 0;
+                    T e0[SB];
+#pragma unroll
+                    for (int j = 0; j < SB; ++j) e0[j] = A.ld(ent_at(j < h && tid < q0, i0c, q0 + (j < h ? j : 0)));
                     for (int cc = wave; cc < k; cc += WAVES) {
                         if (lane < SB) {
                             const bool act = lane >= off;
@@ -537,7 +564,7 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     for (int i = tid; i < q0; i += BT) {
                         T e[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) e[j] = -A.ld(ent_at(j < h, i, q0 + (j < h ? j : 0)));
+                        for (int j = 0; j < SB; ++j) e[j] = -(i == tid ? e0[j] : A.ld(ent_at(j < h, i, q0 + (j < h ? j : 0))));
 #pragma unroll
                         for (int c = 0; c < MAXRHS; ++c) {
                             if (c < k) {

@@ -35,8 +35,12 @@ __device__ __forceinline__ unsigned dpp_max_u32(unsigned x) {
     return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
 }
 __device__ __forceinline__ unsigned long long dpp_max_u64(unsigned long long x) {
-    // The larger high word first, then the largest low word among the lanes that hold it.
+    // The larger high word first, then the largest low word among the lanes that hold it; where one lane alone holds
+    // the largest high word, its key is the maximum and the second pass is skipped.
     const unsigned hi = (unsigned)(x >> 32), bhi = dpp_max_u32(hi);
+    const unsigned long long tied = __ballot(hi == bhi);
+    if (tied != 0ull && (tied & (tied - 1)) == 0ull)
+        return ((unsigned long long)bhi << 32) | (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, (int)__builtin_ctzll(tied));
     const unsigned blo = dpp_max_u32(hi == bhi ? (unsigned)x : 0u);
     return ((unsigned long long)bhi << 32) | blo;
 }

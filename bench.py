@@ -487,7 +487,7 @@ def main():
 
     # The rooflines, from live timings of the timed steps: the library brackets every panel and trail_rest launch with
     # HIP events recorded at the dispatch's start and end on its stream (hs_profile_lu). The dominant kernel by time is
-    # the panel kernel: its launches factor the 64-column panels on XCD 0 and, on the other XCDs, compute trailing tiles
+    # the panel kernel: its launches factor the 64-column panels on one XCD and, on the other XCDs, compute trailing tiles
     # of the previous block's product (look-ahead workers), so its flops are the panels' own and the workers' tiles
     # (2 rows cols k each), against the FP32 MFMA peak. trail_rest takes the tiles the workers left; the product
     # launches (gemm_mfma, the next block's columns) are timed over one more solve (hs_profile_gemm), as are the panel's
@@ -503,7 +503,7 @@ def main():
     trail_achieved = tf / (tms / 1e3) / 1e12
     roof = {'bound': 'mfma', 'achieved': round(panel_achieved, 2), 'peak': PEAK['fp32_mfma'] / 1e12, 'unit': 'TFLOP/s',
             'frac': round(panel_achieved * 1e12 / PEAK['fp32_mfma'], 4), 'traffic': TRAFFIC.get(('panel_f32', n)),
-            'kernel': 'panel_kernel<float> (the panels on XCD 0, trailing tiles of the previous block on XCDs 1-7)',
+            'kernel': 'panel_kernel<float> (the panels on one XCD, trailing tiles of the previous block on the other seven)',
             'launches': pl, 'flop_per_launch': round((pfl + wfl) / pl), 'avg_launch_us': round(pms / pl * 1e3, 2),
             'mfma_busy_rocprof': MFMA_BUSY.get(('panel_f32', n)),
             'share_of_step_time': round(pms / (t * 1e3), 4),

@@ -21,6 +21,7 @@ pub const HS_RIGHT: c_int = 142;
 extern "C" {
     pub fn hs_init() -> c_int;
     pub fn hs_finalize() -> c_int;
+    pub fn hs_set_exchange(mode: c_int, local: *mut c_int) -> c_int;
     pub fn hs_last_error() -> *const c_char;
     pub fn hs_dgemm(op_a: c_int, op_b: c_int, m: i64, n: i64, k: i64, alpha: f64, a: *const f64, lda: i64,
                     b: *const f64, ldb: i64, beta: f64, c: *mut f64, ldc: i64) -> c_int;

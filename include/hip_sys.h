@@ -39,9 +39,17 @@ extern "C" {
 #define HS_LEFT 141
 #define HS_RIGHT 142
 
-/* Library set-up and tear-down; hs_finalize frees the device-memory pool. */
+/* Library set-up and tear-down; hs_finalize frees the device-memory pool. hs_init also probes the current device's
+ * workgroup placement (see hs_set_exchange). */
 int hs_init(void);
 int hs_finalize(void);
+/* How the workgroups of an LU panel (above order 512) and of a single-column substitution exchange pivots and rows
+ * (no counterpart in the reference, whose LU runs on CPU threads). mode 0, the default: through one XCD's L2 where a
+ * probe of the current device (at hs_init, or at its first factorization or substitution) finds launch indices
+ * congruent mod 8 on one XCD, as round-robin dispatch places them; otherwise, and always with mode 1, every word is
+ * published at device scope, which is slower and independent of placement. Results never depend on the mode.
+ * `local` (may be NULL) gets 1 where the current device now exchanges through one L2, 0 otherwise. */
+int hs_set_exchange(int mode, int* local);
 /* The calling thread's last error text. */
 const char* hs_last_error(void);
 

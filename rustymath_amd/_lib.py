@@ -18,6 +18,7 @@ _d, _f = ctypes.c_double, ctypes.c_float
 SIGNATURES = {
     'hs_init': (_i32, []),
     'hs_finalize': (_i32, []),
+    'hs_set_exchange': (_i32, [_i32, _vp]),
     'hs_last_error': (ctypes.c_char_p, []),
     'hs_dgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _d, _vp, _i64, _vp, _i64, _d, _vp, _i64]),
     'hs_sgemm': (_i32, [_i32, _i32, _i64, _i64, _i64, _f, _vp, _i64, _vp, _i64, _f, _vp, _i64]),

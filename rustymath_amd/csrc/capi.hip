@@ -636,17 +636,66 @@ int dev_gemm(int opa, int opb, int64_t m, int64_t n, int64_t k, T alpha, const T
 }
 
 }  // namespace
+namespace {
+__device__ int g_xcc[256];  // the XCD each of 256 launch indices ran on, written by the probe
+__global__ void xcc_probe_kernel() {
+    if (threadIdx.x == 0) {
+        unsigned x;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID, 0, 4)" : "=s"(x));
+        g_xcc[blockIdx.x] = (int)x;
+    }
+}
+std::mutex g_xmu;
+int g_xforce = 0;              // 1: device scope always
+signed char g_xlocal[64];      // per device: 0 unknown, 1 one L2, 2 device scope
+}  // namespace
+
+bool exchange_local(bool may_probe) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return false;
+    std::lock_guard<std::mutex> lk(g_xmu);
+    if (g_xforce) return false;
+    if (g_xlocal[dev] == 0 && may_probe) {
+        // Launch index i on the XCD of index i mod 8 (round-robin dispatch; which XCD index 0 takes varies from
+        // launch to launch): the panel's workgroups (indices 0, 8, 16, ...) share one L2, and so do the
+        // substitutions' runs of blocks.
+        int xcc[256];
+        bool ok = false;
+        xcc_probe_kernel<<<256, 64, 0, 0>>>();
+        if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+            hipMemcpyFromSymbol(xcc, HIP_SYMBOL(g_xcc), sizeof(xcc)) == hipSuccess) {
+            ok = true;
+            for (int i = 8; i < 256; ++i) ok = ok && xcc[i] == xcc[i % 8];
+        }
+        g_xlocal[dev] = ok ? 1 : 2;
+    }
+    return g_xlocal[dev] == 1;
+}
+
 }  // namespace hs
 
 using namespace hs;
 
 extern "C" {
 
+int hs_set_exchange(int mode, int* local) {
+    if (mode < 0 || mode > 1) { set_error("exchange mode must be 0 (probe) or 1 (device scope)"); return -1; }
+    {
+        std::lock_guard<std::mutex> lk(hs::g_xmu);
+        hs::g_xforce = mode;
+    }
+    const bool l = hs::exchange_local(true);
+    if (local) *local = l ? 1 : 0;
+    return OK;
+}
+
 int hs_init(void) {
     int n = 0;
     HS_HIP(hipGetDeviceCount(&n));
     if (n <= 0) { set_error("no HIP device"); return 1; }
-    return thread_stream() ? OK : hip_status(hipErrorUnknown, "hipStreamCreate");
+    if (!thread_stream()) return hip_status(hipErrorUnknown, "hipStreamCreate");
+    hs::exchange_local(true);
+    return OK;
 }
 
 int hs_finalize(void) {

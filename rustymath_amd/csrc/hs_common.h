@@ -32,6 +32,10 @@ void pool_release_all();
 
 // Number of compute units of the current device (for grid sizing of persistent kernels).
 int device_cus();
+// The workgroup exchanges of the LU panels and single-column substitutions go through one XCD's L2 where launch
+// indices congruent mod 8 share an XCD (probed once per device: at hs_init or the first call that may probe) and
+// the caller has not asked for device scope; otherwise their words are published at device scope.
+bool exchange_local(bool may_probe);
 
 // Held by every launch sequence whose workgroups wait on one another and so must all be resident at once (the LU's
 // panel launches, the single-column solves' runs): one such sequence at a time in the process, so that two of them on

@@ -49,29 +49,31 @@ struct Exchange {
     unsigned long long row[2][MAX_WG][ROW_WORDS];
 };
 
-// The workgroups of a panel all run on XCD 0 and share its L2. A word is published with a store that carries no cache
+// The workgroups of a panel (launch indices 0, 8, 16, ...) run on one XCD under round-robin dispatch (which XCD
+// varies from launch to launch) and share its L2. A word is published with a store that carries no cache
 // bits (the wavefront-scope atomic store): it passes the writer's write-through L1 into that L2 and stays there, where
 // the readers' agent-scope loads (which miss every L1) find it. An agent-scope store would also write the line
 // through to memory first: measured on one XCD, a round trip between two workgroups takes 0.46 us this way against
-// 1.1 us with agent-scope stores. Workgroups on different XCDs would not see each other's words; their waits then run
-// out and the factorization reports it, so a wrong placement cannot pass unnoticed.
-__device__ __forceinline__ void xput(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+// 1.1 us with agent-scope stores. Where the host has not found launch indices 0, 8, 16, ... on one XCD (or the caller
+// asked for it), `agent` is set and every word goes out at agent scope, which any XCD's loads see.
+__device__ __forceinline__ void xput(unsigned long long* p, unsigned long long v, bool agent) {
+    if (agent) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 __device__ __forceinline__ unsigned long long xget(const unsigned long long* p) {
     return __hip_atomic_load(const_cast<unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename T>
-__device__ __forceinline__ void put_key(unsigned long long* k, unsigned tag, unsigned long long kv, int kp) {
+__device__ __forceinline__ void put_key(unsigned long long* k, unsigned tag, unsigned long long kv, int kp, bool agent) {
     const unsigned long long pp = (unsigned long long)(32767 - (kp < 32767 ? kp : 32767));
     if constexpr (sizeof(T) == 4) {
-        xput(&k[0], ((unsigned long long)tag << 46) | (kv << 15) | pp);
+        xput(&k[0], ((unsigned long long)tag << 46) | (kv << 15) | pp, agent);
     } else {
         const unsigned long long t = (unsigned long long)tag << 32;
-        xput(&k[0], t | (kv >> 32));
-        xput(&k[1], t | (kv & 0xffffffffull));
-        xput(&k[2], t | pp);
+        xput(&k[0], t | (kv >> 32), agent);
+        xput(&k[1], t | (kv & 0xffffffffull), agent);
+        xput(&k[2], t | pp, agent);
     }
 }
 // Reads a key; false while any of its words is older than `tag`.
@@ -92,14 +94,14 @@ __device__ __forceinline__ bool get_key(const unsigned long long* k, unsigned ta
     }
 }
 template <typename T>
-__device__ __forceinline__ void put_item(unsigned long long* r, int j, unsigned tag, T v) {
+__device__ __forceinline__ void put_item(unsigned long long* r, int j, unsigned tag, T v, bool agent) {
     const unsigned long long t = (unsigned long long)tag << 32;
     if constexpr (sizeof(T) == 4) {
-        xput(&r[j], t | __builtin_bit_cast(unsigned, v));
+        xput(&r[j], t | __builtin_bit_cast(unsigned, v), agent);
     } else {
         const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-        xput(&r[2 * j], t | (u & 0xffffffffull));
-        xput(&r[2 * j + 1], t | (u >> 32));
+        xput(&r[2 * j], t | (u & 0xffffffffull), agent);
+        xput(&r[2 * j + 1], t | (u >> 32), agent);
     }
 }
 template <typename T>
@@ -155,7 +157,7 @@ struct WorkLds {
 template <typename T>
 constexpr size_t lds_bytes() { return sizeof(WorkLds<T>) > sizeof(PanelLds<T>) ? sizeof(WorkLds<T>) : sizeof(PanelLds<T>); }
 
-// A worker of the panel kernel (a workgroup of 512 threads outside XCD 0): takes the next tile of the trailing
+// A worker of the panel kernel (a workgroup of 512 threads off the panel's XCD): takes the next tile of the trailing
 // product, all eight waves on it, until the tiles run out or the panel's columns are done, then returns.
 template <typename T>
 __device__ void trail_work(const TrailJob<T>& J, unsigned stop_tag, WorkLds<T>& W) {
@@ -246,17 +248,19 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                                                    int64_t* __restrict__ pivots, long long* __restrict__ moves,
                                                    const int* __restrict__ leaf_end, T* __restrict__ scratch,
                                                    Exchange* __restrict__ ex, unsigned tag0, int nwg, int* status,
-                                                   long long* prof, TrailJob<T> job) {
+                                                   long long* prof, TrailJob<T> job, int local) {
     __shared__ __attribute__((aligned(16))) unsigned char lds[lds_bytes<T>()];
     // After a wait that ran out in an earlier launch the factorization is void: the launch returns rather than wait
     // again. The panel's workgroups test the word once their rows are in (its load then adds no latency of its own).
     const int void_run = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (blockIdx.x & 7) {  // outside XCD 0: a worker of the trailing product, if there is one
+    if (blockIdx.x & 7) {  // off the panel's XCD: a worker of the trailing product, if there is one
         if (job.C && !void_run) trail_work<T>(job, tag0, *reinterpret_cast<WorkLds<T>*>(lds));
         return;
     }
     const int g = (int)(blockIdx.x >> 3);
     if (g >= nwg) return;
+    // local = 0: words go out at agent scope, and where the workgroups run does not matter.
+    const bool agent = !local;
     PanelLds<T>& L = *reinterpret_cast<PanelLds<T>*>(lds);
     auto& s_stage = L.s_stage;
     auto& s_kv = L.s_kv;
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         bkp = lane < WAVES ? bkp : 0x7fffffff;
         wave_best(bk, bkp, bl);
         const unsigned long long bkv = bk;
-        if (nwg > 1 && tid == 0) put_key<T>(ex->key[buf][g], tag, bkv, bkp);
+        if (nwg > 1 && tid == 0) put_key<T>(ex->key[buf][g], tag, bkv, bkp, agent);
 
         // The workgroup's best row goes out before the rest of the update runs: the holding lane writes its items,
         // still owing the previous column's update, to its wave's LDS row, and lane j applies that update to item j
@@ -369,7 +373,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                     item = lane == 0 ? h[PW] : (lane == PW - 1 ? T(0) : (j >= skip_prev ? u : h[j]));
                 }
                 if (nwg > 1) {
-                    if (lane < rem) put_item<T>(ex->row[buf][g], lane, tag, item);
+                    if (lane < rem) put_item<T>(ex->row[buf][g], lane, tag, item, agent);
                 } else {
                     s_piv[buf][lane] = item;
                     if (lane == 0) s_rcp[buf] = T(1) / item;
@@ -952,6 +956,9 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     // resident_mutex). Every path below synchronises the stream before it returns.
     std::unique_lock<std::mutex> resident(resident_mutex(), std::defer_lock);
     if (n > PT) resident.lock();
+    // The panels' exchange: through one L2 where the placement probe found launch indices 0, 8, 16, ... on one XCD
+    // (local = 1), else at agent scope.
+    const int local = n > PT && exchange_local(true) ? 1 : 0;
     PanelProfile* st;
     int e = panel_prof(st);
     if (e) return e;
@@ -1022,8 +1029,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     };
     // Look-ahead. After block b, the trailing product runs in two parts: the next block's columns at once (on the whole
     // chip), and the columns right of the next block (`job`) by the workers of the next block's panel launches, which
-    // fill XCDs 1-7 while the panel's workgroups run on XCD 0; trail_rest takes the tiles left when the block's panels
-    // are done. Meanwhile a panel's row moves reach only its own block's columns; the other columns take them after the
+    // fill the other seven XCDs while the panel's workgroups run on one; trail_rest takes the tiles left when the
+    // block's panels are done. Meanwhile a panel's row moves reach only its own block's columns; the other columns take them after the
     // product (row moves and the product touch every item in the same order either way). Every tile keeps its terms in
     // order of p, so every item takes the same operations as with one product.
     TrailJob<T> job{};
@@ -1055,12 +1062,12 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
                     HS_HIP(g_trail.event(&P.b));
                     hipExtLaunchKernelGGL(panel_kernel<T>, dim3(grid), dim3(PT), 0, s, P.a, P.b, 0, a, lda, c0, rows, w,
                                           pivots, mv, d_leaf, scratch, ex, (unsigned)(i * PW + 1), nwg, status,
-                                          st->on ? st->prof : nullptr, ahead ? job : none);
+                                          st->on ? st->prof : nullptr, ahead ? job : none, local);
                     g_trail.panels.push_back(P);
                 } else {
                     panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, mv, d_leaf, scratch, ex,
                                                          (unsigned)(i * PW + 1), nwg, status, st->on ? st->prof : nullptr,
-                                                         ahead ? job : none);
+                                                         ahead ? job : none, local);
                 }
             }
             HS_HIP(hipGetLastError());

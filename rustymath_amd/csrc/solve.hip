@@ -279,7 +279,9 @@ int trsv1_lu(bool trans, int64_t n, const T* f, int64_t ld, T* x, hipStream_t s,
     // Runs of RUN1 blocks per XCD when the padded grid fits on the GPU at once (one wave per block, at least two per
     // CU at this kernel's register use), else the launch order, in which no block waits on one launched after it.
     const int padded = (nb + 8 * RUN1 - 1) / (8 * RUN1) * (8 * RUN1);
-    const int run = padded <= 2 * device_cus() ? RUN1 : 1;
+    // Runs of RUN1 blocks per XCD need the probed placement (launch index w on XCD w mod 8); a caller that brings
+    // its own workspace (no allocation) never triggers the probe and takes runs of one until one has run.
+    const int run = padded <= 2 * device_cus() && exchange_local(buf == nullptr) ? RUN1 : 1;
     const int grid = run > 1 ? padded : (nb + 7) / 8 * 8;
     const size_t words = (size_t)n * (sizeof(T) == 8 ? 2 : 1);
     const bool pooled = buf == nullptr;
@@ -670,7 +672,7 @@ int trsv1_columns(bool lower, bool unit, int64_t n, const T* f, int64_t ld, T* x
     if (n == 0) return OK;
     const int nb = (int)((n + B1 - 1) / B1);
     const int padded = (nb + 8 * RUN1 - 1) / (8 * RUN1) * (8 * RUN1);
-    const int run = padded <= 2 * device_cus() ? RUN1 : 1;
+    const int run = padded <= 2 * device_cus() && exchange_local(true) ? RUN1 : 1;
     const int grid = run > 1 ? padded : (nb + 7) / 8 * 8;
     const size_t words = (size_t)n * (sizeof(T) == 8 ? 2 : 1);
     char* buf = (char*)pool_get(trsv1_bytes<T>(n));

@@ -45,6 +45,7 @@ def test_bad_arguments_are_rejected_before_any_device_work():
     assert lib.hs_dev_dpotrf(-1, None, 1, None, None) == -1
     assert lib.hs_dev_spotrf(4, None, 3, None, None) == -3
     assert lib.hs_dev_dgetri(-1, None, 1, None, None, 1, None) == -1
+    assert lib.hs_set_exchange(2, None) == -1 and lib.hs_set_exchange(-1, None) == -1
     assert lib.hs_dev_dgetri(4, None, 3, None, None, 4, None) == -3
     assert lib.hs_dev_sgetri(4, None, 4, None, None, 2, None) == -6
 

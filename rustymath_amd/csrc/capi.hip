@@ -130,7 +130,8 @@ int dev_getrf_perm(int64_t n, T* a, int64_t lda, int64_t* d_perm, int64_t* h_per
     if (info) *info = 0;
     int64_t* d_piv = (int64_t*)pool_get(sizeof(int64_t) * (size_t)((n ? n : 1) + 1));
     if (!d_piv) return hip_status(hipErrorOutOfMemory, "pool_get");
-    int rc = getrf<T>(n, a, lda, d_piv, s);
+    // The device perm comes from the panels' row moves, composed on the GPU; the host's from the pivots.
+    int rc = getrf<T>(n, a, lda, d_piv, s, 0, d_perm);
     if (rc) { pool_put(d_piv); return rc; }
     std::vector<int64_t> piv(n + 1), perm(n);
     unsigned long long* d_zero = (unsigned long long*)(d_piv + n);
@@ -143,7 +144,6 @@ int dev_getrf_perm(int64_t n, T* a, int64_t lda, int64_t* d_perm, int64_t* h_per
     if (info) *info = piv[n] > n ? 0 : piv[n];
     for (int64_t i = 0; i < n; ++i) perm[i] = i;
     for (int64_t c = 0; c < n; ++c) std::swap(perm[c], perm[piv[c]]);
-    if (d_perm) HS_HIP(hipMemcpyAsync(d_perm, perm.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
     if (h_perm_out) std::memcpy(h_perm_out, perm.data(), sizeof(int64_t) * n);
     if (h_piv_out) std::memcpy(h_piv_out, piv.data(), sizeof(int64_t) * n);
     HS_HIP(hipStreamSynchronize(s));

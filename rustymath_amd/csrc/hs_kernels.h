@@ -36,8 +36,10 @@ int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t ld
 // bf16_terms (FP32 only, 0 = none): the trailing products after each 256-column block take their operands as bfloat16
 // (gemm_bf16_update with that many terms); panels, row moves and the rows of U stay FP32. Not the reference's
 // arithmetic: the low-precision factors of the BF16 mixed solve.
+// perm_out (device, n items, may be null): rustymath's `perm` (position -> original row), composed on the GPU from the
+// panels' row moves.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms = 0);
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms = 0, int64_t* perm_out = nullptr);
 
 // Solves with factors from getrf: x = U⁻¹ L⁻¹ P b in place on b (n×k, ld ldb, row-major), with the
 // reference's substitution orders (factor.rs:633-641, :668-748). `trans` solves Aᵀ x = b instead

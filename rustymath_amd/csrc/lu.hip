@@ -787,6 +787,43 @@ __global__ void tall_moves(const int64_t* __restrict__ pivots, int64_t c0, int w
     moves[0] = k;
 }
 
+// perm (position -> original row, rustymath's `perm`, factor.rs:132-134) from the panels' move lists, in order: the
+// row at position dst after a panel is the row that was at src before it. One workgroup; perm lives in LDS up to
+// PERM_LDS rows, above that in memory (agent-scope accesses, so that no wave reads a line its L1 holds from before
+// another wave's write).
+constexpr int PERM_LDS = 16384;
+template <bool IN_LDS>
+__global__ __launch_bounds__(256) void moves_to_perm(int64_t n, const long long* __restrict__ moves, int64_t stride,
+                                                     int lists, int64_t* __restrict__ perm) {
+    __shared__ int sp[IN_LDS ? PERM_LDS : 1];
+    const int tid = threadIdx.x;
+    auto get = [&](int64_t i) -> int64_t {
+        if constexpr (IN_LDS) return sp[i];
+        else return __hip_atomic_load(&perm[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto put = [&](int64_t i, int64_t v) {
+        if constexpr (IN_LDS) sp[i] = (int)v;
+        else __hip_atomic_store(&perm[i], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    for (int64_t i = tid; i < n; i += 256) put(i, i);
+    __syncthreads();
+    for (int L = 0; L < lists; ++L) {
+        const long long* mv = moves + L * stride;
+        const int cnt = (int)mv[0];
+        long long dst = 0;
+        int64_t v = 0;
+        for (int k0 = 0; k0 < cnt; k0 += 256) {  // at most 2 PW moves: one pass
+            const int k = k0 + tid;
+            if (k < cnt) { dst = mv[1 + 2 * k]; v = get(mv[2 + 2 * k]); }
+            __syncthreads();
+            if (k < cnt) put(dst, v);
+            __syncthreads();
+        }
+    }
+    if constexpr (IN_LDS)
+        for (int64_t i = tid; i < n; i += 256) perm[i] = sp[i];
+}
+
 // The reference's leaves: lu_panel halves [c0, c1) at (c0 + c1) / 2 until 16 columns or fewer remain.
 void leaves(int64_t c0, int64_t c1, std::vector<int64_t>& ends) {
     if (c1 - c0 > 16) { int64_t mid = (c0 + c1) / 2; leaves(c0, mid, ends); leaves(mid, c1, ends); return; }
@@ -947,7 +984,7 @@ int lu_profile(int on, double* out) {
 // the trailing matrix takes one product with k up to 256, which reads and writes it a quarter as often as per-panel
 // updates would. Every item still takes its terms in order of the pivot column, one fused multiply-add each.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms) {
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms, int64_t* perm_out) {
     if (n == 0) return OK;
     if (n >= (int64_t)1 << 31) { set_error("LU of order 2^31 or more is not supported"); return ERR_UNSUPPORTED; }
     // Above 32 x 512 rows the panels are factored in memory, column by column (tall_*), without look-ahead.
@@ -1141,6 +1178,11 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
         dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);
         transpose_square_inplace<T><<<grid, block, 0, s>>>(a, lda, n);
         HS_HIP(hipGetLastError());
+        if (perm_out) {
+            if (n <= PERM_LDS) moves_to_perm<true><<<1, 256, 0, s>>>(n, moves, (int64_t)move_words, (int)np, perm_out);
+            else moves_to_perm<false><<<1, 256, 0, s>>>(n, moves, (int64_t)move_words, (int)np, perm_out);
+            HS_HIP(hipGetLastError());
+        }
         int stv = 0;
         HS_HIP(hipMemcpyAsync(&stv, status, sizeof(int), hipMemcpyDeviceToHost, s));
         HS_HIP(hipStreamSynchronize(s));
@@ -1157,8 +1199,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     return rc;
 }
 
-template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t, int);
-template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t, int);
+template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t, int, int64_t*);
+template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t, int, int64_t*);
 
 template <typename T>
 int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s) {

@@ -633,6 +633,102 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(T* __restrict__ a, int6
     }
 }
 
+// U12 = L11⁻¹ A12 for a whole block of rows [r0, r0 + m), m <= 256, columns [cs, n): the rows of U right of a block,
+// in one launch rather than a product and a triangular solve per panel. A workgroup keeps TCOL columns of all m rows in
+// LDS and walks the rows in 16-row steps: the step's rows substitute among themselves (one lane per column, in order
+// of the pivot column), then every row below takes the step's 16 terms on MFMA (16x16x4 steps, each rounding its four
+// terms as fused multiply-adds in order). L11 streams through LDS one 16-column strip per step, the next strip's loads
+// in flight during the step. Every item takes its terms in order of the pivot column, one fused multiply-add each,
+// so the result is the per-panel product and solve's, bit for bit, however the rows group into steps.
+template <typename T> struct RowsCols;
+template <> struct RowsCols<double> { static constexpr int v = 32; };
+template <> struct RowsCols<float> { static constexpr int v = 64; };
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ a, int64_t lda, int64_t r0, int m, int64_t cs,
+                                                        int64_t n) {
+    constexpr int TCOL = RowsCols<T>::v, MR = 4 * PW;
+    __shared__ T Xs[MR][TCOL + 1];
+    __shared__ T Lst[2][MR][17];  // Lst[b][r][q] = L(r, 16 kb + q) below the diagonal, 0 elsewhere
+    typedef typename Mfma<T>::acc Acc;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int64_t col0 = cs + (int64_t)blockIdx.x * TCOL;
+    const int nkb = (m + 15) / 16;
+    T sv[16];
+    // Strip kb: thread t loads row t's 16 items (addresses clamped into the block).
+    auto load_strip = [&](int kb) {
+        const int64_t base = (r0 + (t < m ? t : m - 1)) * lda + r0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int p = 16 * kb + q;
+            sv[q] = a[base + (p < m ? p : m - 1)];
+        }
+    };
+    auto store_strip = [&](int kb, int buf) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int p = 16 * kb + q;
+            Lst[buf][t][q] = (t < m && p < t) ? sv[q] : T(0);
+        }
+    };
+    load_strip(0);
+    {
+        T xv[MR * TCOL / 256];
+#pragma unroll
+        for (int q = 0; q < MR * TCOL / 256; ++q) {
+            const int idx = t + 256 * q, r = idx / TCOL, c = idx % TCOL;
+            const int64_t col = col0 + c < n ? col0 + c : n - 1;
+            xv[q] = a[(r0 + (r < m ? r : m - 1)) * lda + col];
+        }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < MR * TCOL / 256; ++q) {
+            const int idx = t + 256 * q, r = idx / TCOL, c = idx % TCOL;
+            Xs[r][c] = xv[q];
+        }
+    }
+    store_strip(0, 0);
+    __syncthreads();
+    for (int kb = 0; kb < nkb; ++kb) {
+        const int buf = kb & 1, k0 = 16 * kb;
+        if (kb + 1 < nkb) load_strip(kb + 1);
+        if (wave == 0 && lane < TCOL) {  // the step's rows among themselves; lane = column
+            T x[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) x[i] = Xs[k0 + i][lane];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+#pragma unroll
+                for (int r = q + 1; r < 16; ++r) x[r] = __builtin_elementwise_fma(-Lst[buf][k0 + r][q], x[q], x[r]);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) Xs[k0 + i][lane] = x[i];
+        }
+        __syncthreads();
+        for (int rb = kb + 1 + wave; rb < nkb; rb += 4) {  // the rows below, 16 at a time, on MFMA
+            T av[4];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) av[st] = -Lst[buf][16 * rb + (lane & 15)][4 * st + (lane >> 4)];
+#pragma unroll
+            for (int cb = 0; cb < TCOL / 16; ++cb) {
+                Acc acc;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = Xs[16 * rb + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)];
+#pragma unroll
+                for (int st = 0; st < 4; ++st)
+                    acc = Mfma<T>::step(av[st], Xs[k0 + 4 * st + (lane >> 4)][16 * cb + (lane & 15)], acc);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) Xs[16 * rb + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)] = acc[e];
+            }
+        }
+        if (kb + 1 < nkb) store_strip(kb + 1, buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < MR * TCOL / 256; ++q) {
+        const int idx = t + 256 * q, r = idx / TCOL, c = idx % TCOL;
+        if (r < m && col0 + c < n) a[(r0 + r) * lda + col0 + c] = Xs[r][c];
+    }
+}
+
 template <typename T>
 __global__ void transpose_square_inplace(T* a, int64_t lda, int64_t n) {
     __shared__ T t1[32][33], t2[32][33];
@@ -1135,15 +1231,12 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
             if ((rc = swap(moves + p0 * move_words, 0, n, B0, B1, (int)(p1 - p0)))) break;
         }
         if (B1 >= n) continue;
-        // Rows of U right of the block, panel by panel, then the trailing product.
-        for (size_t i = p0; i < p1 && rc == OK; ++i) {
-            const auto [c0, c1] = panels[i];
-            if (c0 > B0)
-                rc = gemm<T>(false, false, c1 - c0, n - B1, c0 - B0, T(-1), a + c0 * lda + B0, lda, a + B0 * lda + B1, lda,
-                             T(1), a + c0 * lda + B1, lda, s);
-            if (rc == OK) rc = trsm(c0, (int)(c1 - c0), B1, n);
+        // Rows of U right of the block (one launch for all its panels), then the trailing product.
+        {
+            constexpr int TCOL = RowsCols<T>::v;
+            trsm_rows_kernel<T><<<(unsigned)((n - B1 + TCOL - 1) / TCOL), 256, 0, s>>>(a, lda, B0, (int)(B1 - B0), B1, n);
+            HS_HIP(hipGetLastError());
         }
-        if (rc) break;
         if constexpr (std::is_same<T, float>::value) {
             if (bf16_terms) {
                 rc = gemm_bf16_update(n - B1, n - B1, B1 - B0, a + B1 * lda + B0, lda, a + B0 * lda + B1, lda,

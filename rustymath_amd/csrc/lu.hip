@@ -30,6 +30,7 @@ constexpr int SG = 16;            // columns a panel workgroup stages through LD
 constexpr int STOP_AHEAD = 8;     // columns before a panel's end at which its workers stop taking tiles
 constexpr int MAX_WG = 32;        // panel workgroups: one per CU of one XCD, one flag per lane of a wave
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
+constexpr int ARRIVE_STRIPS = 8;  // column strips of a panel's in-block update (at most 192 columns, 32 a strip)
 
 template <typename T> struct Bits;
 template <> struct Bits<double> { typedef unsigned long long u; };
@@ -562,40 +563,82 @@ __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t ld
     }
 }
 
-// U12 = L11⁻¹ A12 for rows [c0, c0 + w), columns [cs, n), w <= 64: a workgroup takes 64 columns and keeps them
-// and L11 in LDS. In four blocks of 16 rows: the block's rows substitute among themselves (one lane per column, in
-// order of the pivot column), then the blocks below take the block's terms on MFMA (16x16x4 steps, each rounding
-// its four terms as fused multiply-adds in order). So every item takes its terms in order of the pivot column,
-// one fused multiply-add each, as rustymath's `forward` (core/src/linalg/factor.rs:668-689) gives them.
-constexpr int TC = 64;
+// Columns per workgroup of the kernels below that hold a strip of columns of up to 256 rows in LDS.
+template <typename T> struct RowsCols;
+template <> struct RowsCols<double> { static constexpr int v = 32; };
+template <> struct RowsCols<float> { static constexpr int v = 64; };
+
+// The update of a block's columns [c1, ce) after its panel [c0, c1 = c0 + w), in one launch: U12 = L11⁻¹ A12 for the
+// panel's rows, then A22 -= L21 U12 for the rows [c1, n). Workgroup (bi, bj) takes rows c1 + 64 bi .. + 63 and columns
+// c1 + TCOL bj .. + TCOL - 1. It solves its columns' U12 in LDS in four steps of 16 rows: the step's rows substitute
+// among themselves (one lane per column, in order of the pivot column), then the rows below take the step's terms on
+// MFMA (16x16x4 steps, each rounding its four terms as fused multiply-adds in order), its L21 and A22 tiles in flight
+// meanwhile. Then each wave takes 16 rows of A22 on MFMA with k over the panel's columns in order (terms past w are
+// (+0)·(−0) = −0, which changes no sum). Every workgroup of a column strip reads the panel's rows of that strip and
+// solves the same U12; the last of them to have read those rows (a count per strip in `arrived`, cleared when the
+// factorization starts) writes U12 back, so that no workgroup reads a row another has already overwritten. Every item
+// takes its terms in order of the pivot column, one fused multiply-add each: the triangular solve and product this
+// replaces, bit for bit.
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_block_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int w, int64_t cs,
-                                                         int64_t n) {
-    __shared__ T Ls[PW][PW + 1];   // strictly lower part of L11, zeros elsewhere
-    __shared__ T Xs[PW][TC + 1];
+__global__ __launch_bounds__(256) void panel_update_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int w, int64_t ce,
+                                                           int64_t n, int* __restrict__ arrived) {
+    constexpr int TCOL = RowsCols<T>::v;
+    __shared__ T Ls[PW][PW + 1];    // strictly lower part of L11, zeros elsewhere
+    __shared__ T Xs[PW][TCOL + 1];  // U12 of this workgroup's columns
     typedef typename Mfma<T>::acc Acc;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int64_t col0 = cs + (int64_t)blockIdx.x * TC;
-    // All loads in flight at once, from addresses clamped into the matrix; zeros then pad.
-    T lv[PW * PW / 256], xv[PW * TC / 256];
+    const int64_t c1 = c0 + w, col0 = c1 + (int64_t)blockIdx.y * TCOL, rt = c1 + (int64_t)blockIdx.x * 64;
+    const int64_t rw = rt + 16 * wave;  // this wave's 16 rows
+    // L21 (A operand: lane holds row rw + lane % 16, columns c0 + 4 ks + lane / 16) and A22 (the accumulators), from
+    // addresses clamped into the matrix.
+    T av[PW / 4];
+    Acc acc[TCOL / 16];
+    {
+        const int64_t ar = rw + (lane & 15) < n ? rw + (lane & 15) : n - 1;
 #pragma unroll
-    for (int q = 0; q < PW * PW / 256; ++q) {
-        const int idx = t + 256 * q, r = idx / PW, c = idx % PW;
-        lv[q] = a[(c0 + (r < w ? r : w - 1)) * lda + c0 + (c < w ? c : w - 1)];
-        const int64_t col = col0 + c < n ? col0 + c : n - 1;
-        xv[q] = a[(c0 + (r < w ? r : w - 1)) * lda + col];
-    }
-    asm volatile("" ::: "memory");
+        for (int ks = 0; ks < PW / 4; ++ks) {
+            const int k = 4 * ks + (lane >> 4);
+            av[ks] = a[ar * lda + c0 + (k < w ? k : w - 1)];
+        }
 #pragma unroll
-    for (int q = 0; q < PW * PW / 256; ++q) {
-        const int idx = t + 256 * q, r = idx / PW, c = idx % PW;
-        Ls[r][c] = (r < w && c < r) ? lv[q] : T(0);
-        Xs[r][c] = (r < w && col0 + c < n) ? xv[q] : T(0);
+        for (int cb = 0; cb < TCOL / 16; ++cb)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t r = rw + Mfma<T>::row(lane, e), c = col0 + 16 * cb + (lane & 15);
+                acc[cb][e] = a[(r < n ? r : n - 1) * lda + (c < ce ? c : ce - 1)];
+            }
     }
-    __syncthreads();
+    {
+        T lv[PW * PW / 256], xv[PW * TCOL / 256];
+#pragma unroll
+        for (int q = 0; q < PW * PW / 256; ++q) {
+            const int idx = t + 256 * q, r = idx / PW, c = idx % PW;
+            lv[q] = a[(c0 + (r < w ? r : w - 1)) * lda + c0 + (c < w ? c : w - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < PW * TCOL / 256; ++q) {
+            const int idx = t + 256 * q, r = idx / TCOL, c = idx % TCOL;
+            const int64_t col = col0 + c < ce ? col0 + c : ce - 1;
+            xv[q] = a[(c0 + (r < w ? r : w - 1)) * lda + col];
+        }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < PW * PW / 256; ++q) {
+            const int idx = t + 256 * q, r = idx / PW, c = idx % PW;
+            Ls[r][c] = (r < w && c < r) ? lv[q] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < PW * TCOL / 256; ++q) {
+            const int idx = t + 256 * q, r = idx / TCOL, c = idx % TCOL;
+            Xs[r][c] = r < w ? xv[q] : T(-0.0);  // the padding rows: −0, against +0 on the L side
+        }
+    }
+    __shared__ int s_last;
+    __syncthreads();  // every thread's loads of the panel's rows have returned (their values are in LDS)
+    if (t == 0) s_last = atomicAdd(&arrived[blockIdx.y], 1) == (int)gridDim.x - 1;
     for (int kb = 0; kb < PW / 16 && 16 * kb < w; ++kb) {
         const int k0 = 16 * kb;
-        if (wave == 0) {  // the block's rows among themselves; lane = column
+        if (wave == 0 && lane < TCOL) {  // the 16 rows among themselves; lane = column
             T x[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) x[i] = Xs[k0 + i][lane];
@@ -604,33 +647,53 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(T* __restrict__ a, int6
 #pragma unroll
                 for (int r = q + 1; r < 16; ++r) x[r] = __builtin_elementwise_fma(-Ls[k0 + r][k0 + q], x[q], x[r]);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) Xs[k0 + i][lane] = x[i];
+            for (int i = 0; i < 16; ++i) if (k0 + i < w) Xs[k0 + i][lane] = x[i];
         }
         __syncthreads();
-        const int rb = wave, r0 = 16 * rb;  // the blocks below, one per wave, on MFMA
-        if (rb > kb && r0 < w) {
+        const int r0 = 16 * wave;  // the rows below, one 16-row block per wave, on MFMA
+        if (wave > kb && r0 < w) {
 #pragma unroll
-            for (int cb = 0; cb < TC / 16; ++cb) {
-                Acc acc;
+            for (int cb = 0; cb < TCOL / 16; ++cb) {
+                Acc u;
 #pragma unroll
-                for (int e = 0; e < 4; ++e) acc[e] = Xs[r0 + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)];
+                for (int e = 0; e < 4; ++e) u[e] = Xs[r0 + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)];
 #pragma unroll
-                for (int st = 0; st < 4; ++st) {
-                    const T av = -Ls[r0 + (lane & 15)][k0 + 4 * st + (lane >> 4)];
-                    const T bv = Xs[k0 + 4 * st + (lane >> 4)][16 * cb + (lane & 15)];
-                    acc = Mfma<T>::step(av, bv, acc);
+                for (int st = 0; st < 4; ++st)
+                    u = Mfma<T>::step(-Ls[r0 + (lane & 15)][k0 + 4 * st + (lane >> 4)],
+                                      Xs[k0 + 4 * st + (lane >> 4)][16 * cb + (lane & 15)], u);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int r = r0 + Mfma<T>::row(lane, e);
+                    if (r < w) Xs[r][16 * cb + (lane & 15)] = u[e];
                 }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) Xs[r0 + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)] = acc[e];
             }
         }
         __syncthreads();
     }
+    if (s_last) {  // U12 back to the panel's rows, by the strip's last workgroup to have read them
 #pragma unroll
-    for (int q = 0; q < PW * TC / 256; ++q) {
-        const int idx = t + 256 * q, r = idx / TC, c = idx % TC;
-        if (r < w && col0 + c < n) a[(c0 + r) * lda + col0 + c] = Xs[r][c];
+        for (int q = 0; q < PW * TCOL / 256; ++q) {
+            const int idx = t + 256 * q, r = idx / TCOL, c = idx % TCOL;
+            if (r < w && col0 + c < ce) a[(c0 + r) * lda + col0 + c] = Xs[r][c];
+        }
     }
+    // A22 -= L21 U12: k over the panel's columns in order; past w the L side is +0 and the U side −0.
+#pragma unroll
+    for (int ks = 0; ks < PW / 4; ++ks) {
+        if (4 * ks < w) {
+            const int k = 4 * ks + (lane >> 4);
+            const T l = k < w ? -av[ks] : T(0);
+#pragma unroll
+            for (int cb = 0; cb < TCOL / 16; ++cb) acc[cb] = Mfma<T>::step(l, Xs[k][16 * cb + (lane & 15)], acc[cb]);
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < TCOL / 16; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t r = rw + Mfma<T>::row(lane, e), c = col0 + 16 * cb + (lane & 15);
+            if (r < n && c < ce) a[r * lda + c] = acc[cb][e];
+        }
 }
 
 // U12 = L11⁻¹ A12 for a whole block of rows [r0, r0 + m), m <= 256, columns [cs, n): the rows of U right of a block,
@@ -640,9 +703,6 @@ __global__ __launch_bounds__(256) void trsm_block_kernel(T* __restrict__ a, int6
 // terms as fused multiply-adds in order). L11 streams through LDS one 16-column strip per step, the next strip's loads
 // in flight during the step. Every item takes its terms in order of the pivot column, one fused multiply-add each,
 // so the result is the per-panel product and solve's, bit for bit, however the rows group into steps.
-template <typename T> struct RowsCols;
-template <> struct RowsCols<double> { static constexpr int v = 32; };
-template <> struct RowsCols<float> { static constexpr int v = 64; };
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ a, int64_t lda, int64_t r0, int m, int64_t cs,
                                                         int64_t n) {
@@ -1123,7 +1183,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     const size_t np = panels.size();
     const size_t move_words = 1 + 4 * PW;
     // + status and stop mark, then one tile counter per block (for the product run beside that block's panels)
-    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 2 + (blocks.size() + 1) / 2 + (TALL_G * 12 + 8) / 8;
+    const size_t clear_words = sizeof(Exchange) / 8 + np * move_words + 2 + (blocks.size() + 1) / 2 + (TALL_G * 12 + 8) / 8 +
+                               np * ARRIVE_STRIPS / 2;
     int* d_leaf = (int*)pool_get(n * sizeof(int));
     T* scratch = (T*)pool_get(sizeof(T) * (size_t)n * PW);
     char* xbuf = (char*)pool_get(clear_words * 8);
@@ -1140,17 +1201,12 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     unsigned long long* tall_kv = (unsigned long long*)xbuf + (clear_words - (TALL_G * 12 + 8) / 8);
     int* tall_kp = (int*)(tall_kv + TALL_G);
     int* tall_pick = tall_kp + TALL_G;
+    int* arrived = (int*)(tall_kv - np * ARRIVE_STRIPS / 2);  // panel_update_kernel's counts, ARRIVE_STRIPS a panel
     HS_HIP(hipMemcpyAsync(d_leaf, leaf_end.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
     fill_words<<<(unsigned)((clear_words + 255) / 256), 256, 0, s>>>((unsigned long long*)xbuf, (int64_t)clear_words, 0ull);
     HS_HIP(hipGetLastError());
 
     int rc = OK;
-    auto trsm = [&](int64_t c0, int w, int64_t cs, int64_t ce) {
-        if (ce <= cs) return OK;
-        trsm_block_kernel<T><<<(unsigned)((ce - cs + TC - 1) / TC), 256, 0, s>>>(a, lda, c0, w, cs, ce);
-        HS_HIP(hipGetLastError());
-        return OK;
-    };
     // The row moves of `count` consecutive panels, in order, on the columns [lo, hi) outside [x0, x1).
     auto swap = [&](const long long* mv, int64_t lo, int64_t hi, int64_t x0, int64_t x1, int count = 1) {
         const int64_t cols = (hi - lo) - (x1 - x0);
@@ -1205,10 +1261,11 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
             }
             HS_HIP(hipGetLastError());
             if ((rc = ahead ? swap(mv, B0, B1, c0, c1) : swap(mv, 0, n, c0, c1))) break;
-            if (c1 < B1) {
-                if ((rc = trsm(c0, w, c1, B1))) break;
-                rc = gemm<T>(false, false, n - c1, B1 - c1, w, T(-1), a + c1 * lda + c0, lda, a + c0 * lda + c1, lda, T(1),
-                             a + c1 * lda + c1, lda, s);
+            if (c1 < B1) {  // the block's columns right of the panel: its rows of U, then the rows below
+                constexpr int TCOL = RowsCols<T>::v;
+                const dim3 ug((unsigned)((n - c1 + 63) / 64), (unsigned)((B1 - c1 + TCOL - 1) / TCOL));
+                panel_update_kernel<T><<<ug, 256, 0, s>>>(a, lda, c0, w, B1, n, arrived + i * ARRIVE_STRIPS);
+                HS_HIP(hipGetLastError());
             }
         }
         if (rc) break;

@@ -37,9 +37,11 @@ int gemm_bf16_update(int64_t m, int64_t n, int64_t k, const float* a, int64_t ld
 // (gemm_bf16_update with that many terms); panels, row moves and the rows of U stay FP32. Not the reference's
 // arithmetic: the low-precision factors of the BF16 mixed solve.
 // perm_out (device, n items, may be null): rustymath's `perm` (position -> original row), composed on the GPU from the
-// panels' row moves.
+// panels' row moves. keep_rows leaves the factors row-major (row i holds L's row i left of the diagonal and U's right of
+// it) instead of transposing them into the reference's column-major layout: for the mixed solves' own substitutions.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms = 0, int64_t* perm_out = nullptr);
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms = 0, int64_t* perm_out = nullptr,
+          bool keep_rows = false);
 
 // Solves with factors from getrf: x = U⁻¹ L⁻¹ P b in place on b (n×k, ld ldb, row-major), with the
 // reference's substitution orders (factor.rs:633-641, :668-748). `trans` solves Aᵀ x = b instead
@@ -124,6 +126,9 @@ int lu_profile(int on, double* out);
 template <typename T>
 int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s);
 int cast_d2s(int64_t rows, int64_t cols, const double* src, int64_t lds, float* dst, int64_t ldd, hipStream_t s);
+// dst = float(src) over a compact array of `count` items and part[b] = the sum of squares of src that block b of the
+// `blocks`-block grid read (FP64): ||src||² is then the sum of the parts, the same on every run.
+int cast_d2s_sumsq(int64_t count, const double* src, float* dst, double* part, int blocks, hipStream_t s);
 int cast_s2d(int64_t rows, int64_t cols, const float* src, int64_t lds, double* dst, int64_t ldd, hipStream_t s);
 
 }  // namespace hs

@@ -128,7 +128,8 @@ struct IrState {
 
 int sumsq(int64_t count, const double* v, double* part, double* out, hipStream_t s, const int* done = nullptr);
 
-// The blocked single-column substitutions of the refinement's corrections (FP32 factors, FP32 vector): one workgroup
+// The blocked single-column substitutions of the refinement's corrections (FP32 factors left row-major by getrf, FP32
+// vector): one workgroup
 // of 256 threads per block of IRB rows, workgroups in solve order (a block waits only on blocks launched before it, so
 // no residency is assumed). A block adds each earlier block's IRB solved values as they are published (8-byte words
 // that carry the pass's epoch beside the value, stored and loaded at agent scope), its tile of the factors for the
@@ -175,13 +176,13 @@ __global__ __launch_bounds__(256) void ir_trsv_kernel(int64_t n, const float* __
         if (!LOWER) __syncthreads();
         float tmp[IRB * IRB / 256];
 #pragma unroll
-        for (int k = 0; k < IRB * IRB / 256; ++k) {  // column j of the block, rows i along the threads: coalesced
-            const int idx = tid + 256 * k, j = idx / IRB, i = idx % IRB;
-            tmp[k] = f[(r0 + (j < h ? j : h - 1)) * ld + r0 + (i < h ? i : h - 1)];
+        for (int k = 0; k < IRB * IRB / 256; ++k) {  // row i of the block, columns j along the threads: coalesced
+            const int idx = tid + 256 * k, i = idx / IRB, j = idx % IRB;
+            tmp[k] = f[(r0 + (i < h ? i : h - 1)) * ld + r0 + (j < h ? j : h - 1)];
         }
 #pragma unroll
         for (int k = 0; k < IRB * IRB / 256; ++k) {
-            const int idx = tid + 256 * k, j = idx / IRB, i = idx % IRB;
+            const int idx = tid + 256 * k, i = idx / IRB, j = idx % IRB;
             const bool in = j < h && i < h && (LOWER ? i > j : i < j);
             s_t[i][j] = in ? (LOWER ? tmp[k] : tmp[k] * s_rc[i]) : 0.f;
         }
@@ -199,25 +200,30 @@ __global__ __launch_bounds__(256) void ir_trsv_kernel(int64_t n, const float* __
         }
     }
     // Earlier blocks, in solve order t = 0 .. u - 1: block p(t). The tile F(r0 + 4 rg + e, p IRB + 16 cg + j) of
-    // producer t, as four rows' items of 16 columns (columns of F are contiguous: f[col ld + row]).
+    // producer t: four rows' 16 items each, four to a 16-byte load (the factors are row-major: f[row ld + col]).
     const int np = u;
     auto blk = [&](int t) { return LOWER ? t : nb - 1 - t; };
     const int64_t rowb = r0 + 4 * rg;
-    const bool vec = (ld & 3) == 0;  // four rows' items in one 16-byte load where columns start 16-byte aligned
+    const bool vec = (ld & 3) == 0;  // rows start 16-byte aligned
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
     typedef float f4 __attribute__((ext_vector_type(4)));
-    auto load_tile = [&](int t, f4* e) {
+    auto load_tile = [&](int t, f4* e) {  // e[4 r + q]: row rowb + r, columns c0 + 4 q .. + 3
         const int64_t c0 = (int64_t)blk(t) * IRB + 16 * cg;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int64_t col = c0 + j < n ? c0 + j : n - 1;
-            if (vec && rowb + 3 < n) {
-                e[j] = *reinterpret_cast<const f4*>(f + col * ld + rowb);
-            } else {  // items past n read row n - 1 (their rows are never used)
-                f4 v;
+        for (int r = 0; r < 4; ++r) {
+            const int64_t row = rowb + r < n ? rowb + r : n - 1;
+            const float* fr = f + row * ld;
+            if (vec && c0 + 16 <= n) {
 #pragma unroll
-                for (int e2 = 0; e2 < 4; ++e2) v[e2] = f[col * ld + (rowb + e2 < n ? rowb + e2 : n - 1)];
-                e[j] = v;
+                for (int q = 0; q < 4; ++q) e[4 * r + q] = *reinterpret_cast<const f4*>(fr + c0 + 4 * q);
+            } else {  // columns past n read column n - 1 (their values meet zeros)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    f4 v;
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) { const int64_t c = c0 + 4 * q + m; v[m] = fr[c < n ? c : n - 1]; }
+                    e[4 * r + q] = v;
+                }
             }
         }
     };
@@ -259,7 +265,7 @@ __global__ __launch_bounds__(256) void ir_trsv_kernel(int64_t n, const float* __
         for (int j = 0; j < 16; ++j) {
             const float yj = y4[j / 4][j % 4];
 #pragma unroll
-            for (int e2 = 0; e2 < 4; ++e2) acc[e2] = __builtin_elementwise_fma(-e[j][e2], yj, acc[e2]);
+            for (int r = 0; r < 4; ++r) acc[r] = __builtin_elementwise_fma(-e[4 * r + j / 4][j % 4], yj, acc[r]);
         }
     };
     {
@@ -447,14 +453,18 @@ int gesv_ir(int64_t n, const double* a, int64_t lda, const double* b, double* x,
     const unsigned blocks = (unsigned)((n + 255) / 256);
     int it = 0;
     double be = INFINITY;
-    if (!rc) rc = cast_d2s(n, n, a, lda, a32, n, s);
-    if (!rc) rc = getrf<float>(n, a32, n, d_piv, s, 0, d_perm);
+    // float(A), with ||A||_F² from the same pass (A is compact: lda == n); then the FP32 factors, left row-major.
+    if (!rc) rc = cast_d2s_sumsq(n * n, a, a32, part, 1024, s);
     if (!rc) {
         ir_init_kernel<<<1, 1, 0, s>>>(st, tol, max_iter);
+        sum_parts<<<1, 256, 0, s>>>(1024, part, &st->anorm2, nullptr);
+        HS_HIP(hipGetLastError());
+    }
+    if (!rc) rc = getrf<float>(n, a32, n, d_piv, s, 0, d_perm, true);
+    if (!rc) {
         fill_zero<<<1, 64, 0, s>>>((unsigned long long*)status, 1);
         fill_zero<<<(unsigned)std::min<int64_t>((n + 255) / 256, 1024), 256, 0, s>>>(pub, n);
         HS_HIP(hipGetLastError());
-        rc = sumsq(n * n, a, part, &st->anorm2, s);  // ||A||_F² (A is compact: lda == n)
     }
     // x = 0, r = b before the first step.
     if (!rc) fill_zero<<<(unsigned)std::min<int64_t>((n + 255) / 256, 1024), 256, 0, s>>>((unsigned long long*)x, n);
@@ -554,7 +564,7 @@ int gesv_gmres_ir(int64_t n, const double* a, int64_t lda, const double* b, doub
         return OK;
     };
     if (!rc) rc = cast_d2s(n, n, a, lda, a32, n, s);
-    if (!rc) rc = getrf<float>(n, a32, n, d_piv, s, terms, d_perm);
+    if (!rc) rc = getrf<float>(n, a32, n, d_piv, s, terms, d_perm, true);
     if (!rc) {
         ir_init_kernel<<<1, 1, 0, s>>>(st, 0.0, 1);
         fill_zero<<<1, 64, 0, s>>>((unsigned long long*)status, 1);

@@ -580,7 +580,7 @@ template <> struct RowsCols<float> { static constexpr int v = 64; };
 // takes its terms in order of the pivot column, one fused multiply-add each: the triangular solve and product this
 // replaces, bit for bit.
 template <typename T>
-__global__ __launch_bounds__(256) void panel_update_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int w, int64_t ce,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void panel_update_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int w, int64_t ce,
                                                            int64_t n, int* __restrict__ arrived) {
     constexpr int TCOL = RowsCols<T>::v;
     __shared__ T Ls[PW][PW + 1];    // strictly lower part of L11, zeros elsewhere
@@ -713,9 +713,9 @@ __global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ a, int64
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t col0 = cs + (int64_t)blockIdx.x * TCOL;
     const int nkb = (m + 15) / 16;
-    T sv[16];
+    T sva[16], svb[16];  // strips in flight: each step issues the loads of the strip two steps ahead
     // Strip kb: thread t loads row t's 16 items (addresses clamped into the block).
-    auto load_strip = [&](int kb) {
+    auto load_strip = [&](int kb, T* sv) {
         const int64_t base = (r0 + (t < m ? t : m - 1)) * lda + r0;
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
@@ -723,14 +723,15 @@ __global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ a, int64
             sv[q] = a[base + (p < m ? p : m - 1)];
         }
     };
-    auto store_strip = [&](int kb, int buf) {
+    auto store_strip = [&](int kb, const T* sv, int buf) {
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const int p = 16 * kb + q;
             Lst[buf][t][q] = (t < m && p < t) ? sv[q] : T(0);
         }
     };
-    load_strip(0);
+    load_strip(0, sva);
+    if (nkb > 1) load_strip(1, svb);
     {
         T xv[MR * TCOL / 256];
 #pragma unroll
@@ -746,11 +747,13 @@ __global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ a, int64
             Xs[r][c] = xv[q];
         }
     }
-    store_strip(0, 0);
+    store_strip(0, sva, 0);
     __syncthreads();
-    for (int kb = 0; kb < nkb; ++kb) {
+    // Step kb: strip kb is in Lst[kb % 2]; `mine` held strip kb (free now) and takes strip kb + 2, `next` holds strip
+    // kb + 1, which goes to LDS at the step's end.
+    auto step = [&](int kb, T* mine, const T* next) {
         const int buf = kb & 1, k0 = 16 * kb;
-        if (kb + 1 < nkb) load_strip(kb + 1);
+        if (kb + 2 < nkb) load_strip(kb + 2, mine);
         if (wave == 0 && lane < TCOL) {  // the step's rows among themselves; lane = column
             T x[16];
 #pragma unroll
@@ -779,8 +782,12 @@ __global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ a, int64
                 for (int e = 0; e < 4; ++e) Xs[16 * rb + Mfma<T>::row(lane, e)][16 * cb + (lane & 15)] = acc[e];
             }
         }
-        if (kb + 1 < nkb) store_strip(kb + 1, buf ^ 1);
+        if (kb + 1 < nkb) store_strip(kb + 1, next, buf ^ 1);
         __syncthreads();
+    };
+    for (int kb = 0; kb < nkb; kb += 2) {
+        step(kb, sva, svb);
+        if (kb + 1 < nkb) step(kb + 1, svb, sva);
     }
 #pragma unroll
     for (int q = 0; q < MR * TCOL / 256; ++q) {
@@ -1140,7 +1147,7 @@ int lu_profile(int on, double* out) {
 // the trailing matrix takes one product with k up to 256, which reads and writes it a quarter as often as per-panel
 // updates would. Every item still takes its terms in order of the pivot column, one fused multiply-add each.
 template <typename T>
-int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms, int64_t* perm_out) {
+int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16_terms, int64_t* perm_out, bool keep_rows) {
     if (n == 0) return OK;
     if (n >= (int64_t)1 << 31) { set_error("LU of order 2^31 or more is not supported"); return ERR_UNSUPPORTED; }
     // Above 32 x 512 rows the panels are factored in memory, column by column (tall_*), without look-ahead.
@@ -1325,9 +1332,11 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
         job.stop = trail_stop;
     }
     if (rc == OK) {
-        dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);
-        transpose_square_inplace<T><<<grid, block, 0, s>>>(a, lda, n);
-        HS_HIP(hipGetLastError());
+        if (!keep_rows) {
+            dim3 grid((unsigned)((n + 31) / 32), (unsigned)((n + 31) / 32)), block(32, 8);
+            transpose_square_inplace<T><<<grid, block, 0, s>>>(a, lda, n);
+            HS_HIP(hipGetLastError());
+        }
         if (perm_out) {
             if (n <= PERM_LDS) moves_to_perm<true><<<1, 256, 0, s>>>(n, moves, (int64_t)move_words, (int)np, perm_out);
             else moves_to_perm<false><<<1, 256, 0, s>>>(n, moves, (int64_t)move_words, (int)np, perm_out);
@@ -1349,8 +1358,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
     return rc;
 }
 
-template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t, int, int64_t*);
-template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t, int, int64_t*);
+template int getrf<double>(int64_t, double*, int64_t, int64_t*, hipStream_t, int, int64_t*, bool);
+template int getrf<float>(int64_t, float*, int64_t, int64_t*, hipStream_t, int, int64_t*, bool);
 
 template <typename T>
 int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int64_t ldd, hipStream_t s) {
@@ -1362,6 +1371,46 @@ int transpose(int64_t rows, int64_t cols, const T* src, int64_t lds, T* dst, int
 }
 template int transpose<double>(int64_t, int64_t, const double*, int64_t, double*, int64_t, hipStream_t);
 template int transpose<float>(int64_t, int64_t, const float*, int64_t, float*, int64_t, hipStream_t);
+
+// dst = float(src) for a compact n-item array, two items per thread and step, and part[b] = block b's sum of the
+// squares of src (FP64, a fixed grid and order, so the sum is the same on every run).
+__global__ __launch_bounds__(256) void cast_sumsq_kernel(int64_t count, const double* __restrict__ src,
+                                                         float* __restrict__ dst, double* __restrict__ part) {
+    __shared__ double sp[4];
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    double acc = 0;
+    const int64_t pairs = count / 2, stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < pairs; i += stride) {
+        const d2 v = reinterpret_cast<const d2*>(src)[i];
+        f2 o;
+        o.x = (float)v.x;
+        o.y = (float)v.y;
+        reinterpret_cast<f2*>(dst)[i] = o;
+        acc = __builtin_elementwise_fma(v.x, v.x, acc);
+        acc = __builtin_elementwise_fma(v.y, v.y, acc);
+    }
+    if ((count & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const double v = src[count - 1];
+        dst[count - 1] = (float)v;
+        acc = __builtin_elementwise_fma(v, v, acc);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+}
+
+int cast_d2s_sumsq(int64_t count, const double* src, float* dst, double* part, int blocks, hipStream_t s) {
+    if (!count) return OK;
+    if ((reinterpret_cast<uintptr_t>(src) & 15) || (reinterpret_cast<uintptr_t>(dst) & 7)) {
+        set_error("cast_d2s_sumsq needs 16-byte aligned source and 8-byte aligned destination");
+        return ERR_UNSUPPORTED;
+    }
+    cast_sumsq_kernel<<<blocks, 256, 0, s>>>(count, src, dst, part);
+    HS_HIP(hipGetLastError());
+    return OK;
+}
 
 int cast_d2s(int64_t rows, int64_t cols, const double* src, int64_t lds, float* dst, int64_t ldd, hipStream_t s) {
     int64_t total = rows * cols;

@@ -236,7 +236,7 @@ __device__ __forceinline__ long long ticks() { return __builtin_amdgcn_s_memreal
 //
 // With one workgroup, the winner's row goes through LDS. With several (launch indices 0, 8, 16, ..., one XCD
 // under round-robin dispatch, one workgroup per CU, so all are resident), each workgroup publishes its best key
-// and, from the thread holding it, its row as tagged words (Exchange); wave 0 of every workgroup reads all keys,
+// and, from the thread holding it, its row as tagged words (Exchange); the last wave of every workgroup reads all keys,
 // picks the winner and fetches its row. The launch indices on the other XCDs return at once, or, where `job`
 // holds a trailing product, work on its tiles (trail_work) until the columns are done.
 //
@@ -414,8 +414,10 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         HS_PHASE(1);
 
         if (nwg > 1) {
-            // Wave 0 compares the workgroups' keys, then fetches the winner's row.
-            if (wave == 0) {
+            // The last wave compares the workgroups' keys, then fetches the winner's row. Not wave 0: where pivots lie
+            // on the diagonal (no row moves), the pivot row of every column of a panel sits in wave 0 of workgroup 0,
+            // whose hand-off of that row would otherwise delay the poll behind it.
+            if (wave == WAVES - 1) {
                 unsigned long long gkv = 0ull;
                 int gkp = 0x7fffffff, gl = 0;
                 bool ok = true;

@@ -322,6 +322,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     // update (pending), with multiplier l_prev, the pivot row prev and the zero-pivot skip.
     T x0 = v[0], l_prev = T(0);
     const T* prev = s_piv[1];
+    T* sc = scratch + (has ? phys : 0);  // this row's multiplier of the current column: scratch[c][row]
     bool pending = false;
     int skip_prev = 0;
 
@@ -451,13 +452,13 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         HS_PHASE(2);
         __syncthreads();
         HS_PHASE(3);
-        if (s_abort) return;
         const int p = s_p[buf];
         const T* piv = s_piv[buf];
+        const T pivot = piv[0], piv1 = piv[1], rcp = s_rcp[buf];  // read with the abort word, not behind its test
+        if (s_abort) return;
 
         // The winner takes position c; its row of U is kept in LDS. The row at position c moves to p.
         if (tid < PW) s_u11[c][tid] = piv[tid];
-        const T pivot = piv[0];
         if (has && pos == p) pos = c;
         else if (has && pos == c) pos = p;
         if (g == 0 && tid == 0) pivots[c0 + c] = c0 + p;
@@ -468,14 +469,15 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         if (has && pos > c) {
             const bool zero = pivot == T(0);
             const int skip = zero ? s_skip[c] : 0;
-            const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * s_rcp[buf] : x0 / pivot);
-            x0 = 1 >= skip ? __builtin_elementwise_fma(l, -piv[1], v[1]) : v[1];
-            scratch[(int64_t)c * rows + phys] = l;
+            const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * rcp : x0 / pivot);
+            x0 = 1 >= skip ? __builtin_elementwise_fma(l, -piv1, v[1]) : v[1];
+            *sc = l;
             l_prev = l;
             skip_prev = skip;
             pending = true;
         }
         prev = piv;
+        sc += rows;
         // The workers take no new tile once STOP_AHEAD columns remain, so that the tiles in progress end about when
         // the columns do.
         if (job.C && g == 0 && tid == 0 && c == (w > STOP_AHEAD ? w - 1 - STOP_AHEAD : 0))

@@ -354,12 +354,15 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     }
 #pragma unroll
                     for (int r = 0; r < R; ++r) lm[r] = nz ? l[r] : T(0);
+                    // The pivot row as the update reads it: +0 throughout for a zero pivot (one select per column, so
+                    // that reading an item takes no branch).
+                    const T mz = nz ? mine : T(0);
 #pragma unroll
                     for (int j0 = 1; j0 < BW; j0 += 16) {
                         if (j0 < w - c) {  // items past the panel are never used
 #pragma unroll
                             for (int j = j0; j < j0 + 16 && j < BW; ++j) {
-                                const T uj = nz ? lane_value(mine, j) : T(0);
+                                const T uj = lane_value(mz, j);
 #pragma unroll
                                 for (int r = 0; r < R; ++r) v[r][j - 1] = __builtin_elementwise_fma(lm[r], -uj, v[r][j]);
                             }

@@ -48,34 +48,6 @@ __device__ __forceinline__ void gemm_tile(int64_t m, int64_t n, int64_t k, T alp
     const int wm = wave >> 1, wn = wave & 1;
     const bool scale_a = alpha != T(1);
 
-    // Accumulators start at 0, or at beta c (c itself where beta is 1), as `native_gemm` scales c first.
-    Acc acc[NI][4];
-    if (beta != T(0)) {
-#pragma unroll
-        for (int i = 0; i < NI; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int64_t r = m0 + wm * RW + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
-                    T v;
-                    if (EDGE) {
-                        const int64_t rr = r < m ? r : m - 1, cc = col < n ? col : n - 1;
-                        v = C[rr * ldc + cc];
-                    } else {
-                        v = C[r * ldc + col];
-                    }
-                    acc[i][j][e] = beta != T(1) ? beta * v : v;
-                }
-    } else {
-#pragma unroll
-        for (int i = 0; i < NI; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
-    }
-
     // Global loads: each thread brings NQ items of A's tile and NQ of B's per step of BK. Items outside the matrix are
     // +0 on A's side and, past k, -0 on B's side, so that padding terms add (+0)(-0) = -0, which changes no sum. alpha
     // scales only the items that exist (bits of amask), when they are stored to LDS: the loads of the next step stay in
@@ -179,9 +151,47 @@ __device__ __forceinline__ void gemm_tile(int64_t m, int64_t n, int64_t k, T alp
     };
 
     const int64_t steps = (k + BK - 1) / BK;
+    // The first step's A and B go out before C, so that the stash below waits for them only (the wait counter counts
+    // loads in order) and C's loads stay in flight until the first MFMA.
     load(0);
+    // Accumulators start at 0, or at beta c (c itself where beta is 1), as `native_gemm` scales c first.
+    Acc acc[NI][4];
+    if (beta != T(0)) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t r = m0 + wm * RW + i * 16 + Mfma<T>::row(lane, e), col = n0 + wn * 64 + j * 16 + (lane & 15);
+                    T v;
+                    if (EDGE) {
+                        const int64_t rr = r < m ? r : m - 1, cc = col < n ? col : n - 1;
+                        v = C[rr * ldc + cc];
+                    } else {
+                        v = C[r * ldc + col];
+                    }
+                    acc[i][j][e] = v;  // scaled by beta below, after the first step's stash, so nothing waits for C here
+                }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
+    }
+
     stash(0);
     __syncthreads();
+    if (beta != T(0) && beta != T(1)) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[i][j][e] = beta * acc[i][j][e];
+    }
     for (int64_t s = 0; s < steps; ++s) {
         const int buf = s & 1;
         if (s + 1 < steps) load((s + 1) * BK);

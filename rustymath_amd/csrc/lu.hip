@@ -571,6 +571,9 @@ __global__ __launch_bounds__(256) void swap_kernel(T* __restrict__ a, int64_t ld
 template <typename T> struct RowsCols;
 template <> struct RowsCols<double> { static constexpr int v = 32; };
 template <> struct RowsCols<float> { static constexpr int v = 64; };
+// Columns per workgroup of trsm_rows_kernel: 32, so that at n = 16384 about two workgroups share each CU and one's
+// substitution steps overlap the other's.
+constexpr int ROWS_TCOL = 32;
 
 // The update of a block's columns [c1, ce) after its panel [c0, c1 = c0 + w), in one launch: U12 = L11⁻¹ A12 for the
 // panel's rows, then A22 -= L21 U12 for the rows [c1, n). Workgroup (bi, bj) takes rows c1 + 64 bi .. + 63 and columns
@@ -708,9 +711,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void panel_update_kern
 // in flight during the step. Every item takes its terms in order of the pivot column, one fused multiply-add each,
 // so the result is the per-panel product and solve's, bit for bit, however the rows group into steps.
 template <typename T>
-__global__ __launch_bounds__(256) void trsm_rows_kernel(T* __restrict__ a, int64_t lda, int64_t r0, int m, int64_t cs,
+__global__ __launch_bounds__(256, 2) void trsm_rows_kernel(T* __restrict__ a, int64_t lda, int64_t r0, int m, int64_t cs,
                                                         int64_t n) {
-    constexpr int TCOL = RowsCols<T>::v, MR = 4 * PW;
+    constexpr int TCOL = ROWS_TCOL, MR = 4 * PW;
     __shared__ T Xs[MR][TCOL + 1];
     __shared__ T Lst[2][MR][17];  // Lst[b][r][q] = L(r, 16 kb + q) below the diagonal, 0 elsewhere
     typedef typename Mfma<T>::acc Acc;
@@ -1301,8 +1304,8 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
         if (B1 >= n) continue;
         // Rows of U right of the block (one launch for all its panels), then the trailing product.
         {
-            constexpr int TCOL = RowsCols<T>::v;
-            trsm_rows_kernel<T><<<(unsigned)((n - B1 + TCOL - 1) / TCOL), 256, 0, s>>>(a, lda, B0, (int)(B1 - B0), B1, n);
+            trsm_rows_kernel<T><<<(unsigned)((n - B1 + ROWS_TCOL - 1) / ROWS_TCOL), 256, 0, s>>>(a, lda, B0, (int)(B1 - B0), B1,
+                                                                                          n);
             HS_HIP(hipGetLastError());
         }
         if constexpr (std::is_same<T, float>::value) {

@@ -57,6 +57,10 @@ extern "C" {
     pub fn hs_slu_solve_stack(count: i64, n: i64, k: i64, factors: *const f32, perm: *const i64, b: *const f32,
                               x: *mut f32) -> c_int;
     pub fn hs_slu_stack(count: i64, n: i64, a: *const f32, factors: *mut f32, perm: *mut i64) -> c_int;
+    pub fn hs_dgesv_batched(count: i64, n: *const i64, k: *const i64, a: *const *const f64, b: *const *const f64,
+                            x: *const *mut f64, regular: *mut c_int) -> c_int;
+    pub fn hs_sgesv_batched(count: i64, n: *const i64, k: *const i64, a: *const *const f32, b: *const *const f32,
+                            x: *const *mut f32, regular: *mut c_int) -> c_int;
     pub fn hs_dev_dgemm(op_a: c_int, op_b: c_int, m: i64, n: i64, k: i64, alpha: f64, a: *const f64, lda: i64,
                         b: *const f64, ldb: i64, beta: f64, c: *mut f64, ldc: i64, stream: *mut c_void) -> c_int;
     pub fn hs_dev_sgemm(op_a: c_int, op_b: c_int, m: i64, n: i64, k: i64, alpha: f32, a: *const f32, lda: i64,

@@ -141,6 +141,18 @@ int hs_slu_solve_stack(int64_t count, int64_t n, int64_t k, const float* factors
                        float* x);
 int hs_slu_stack(int64_t count, int64_t n, const float* a, float* factors, int64_t* perm);
 
+/* Uneven batched solves from host memory: `count` systems, system i being a[i] (row-major n[i]×n[i]) with b[i]
+ * (row-major n[i]×k[i]), each solved as `LuWork::solve` (factor.rs:207-226) solves a cell of a stack over
+ * `parallel::stack` (core/src/parallel.rs:118-192); the reference has no single call for systems of different orders
+ * (a stack holds one order), so this is the per-cell loop a caller would write, as one persistent batched launch for
+ * the systems up to order 512 with at most 8 right-hand sides (others one by one). x[i] (n[i]×k[i]) receives X_i where
+ * regular[i] = 1 and b[i]'s copy where regular[i] = 0 (the reference would take least squares; an empty system counts
+ * as not regular, factor.rs:222). regular may be NULL. Each X_i is bit for bit the single-matrix hs_*solve result. */
+int hs_dgesv_batched(int64_t count, const int64_t* n, const int64_t* k, const double* const* a, const double* const* b,
+                     double* const* x, int* regular);
+int hs_sgesv_batched(int64_t count, const int64_t* n, const int64_t* k, const float* const* a, const float* const* b,
+                     float* const* x, int* regular);
+
 /* Device-resident forms (no counterpart in the reference, whose API takes host slices): pointers are
  * device memory, `stream` a hipStream_t (NULL = the null stream). */
 int hs_dev_dgemm(int op_a, int op_b, int64_t m, int64_t n, int64_t k, double alpha, const double* a, int64_t lda,

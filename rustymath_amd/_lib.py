@@ -50,6 +50,8 @@ SIGNATURES = {
     'hs_ssolve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     'hs_dlu_stack': (_i32, [_i64, _i64, _vp, _vp, _vp]),
     'hs_slu_stack': (_i32, [_i64, _i64, _vp, _vp, _vp]),
+    'hs_dgesv_batched': (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'hs_sgesv_batched': (_i32, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     'hs_dlu_solve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     'hs_slu_solve_stack': (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     'hs_dsolve': (_i32, [_vp, _i64, _vp, _i64, _vp, _vp]),

@@ -548,6 +548,82 @@ int host_stack(int64_t count, int64_t n, int64_t k, const T* a, const T* b, T* x
     return rc;
 }
 
+// Uneven batched solves from host memory: system i is a[i] (row-major n[i]×n[i]) with b[i] (row-major n[i]×k[i]), each
+// `LuWork::solve` (core/src/linalg/factor.rs:207-226) as rustymath runs the cells of a stack (core/src/parallel.rs:118-192).
+// Systems up to order 512 with at most 8 right-hand sides go to one persistent batched launch (packed back to back into
+// two device buffers), others one by one. x[i] receives X_i where regular[i] = 1, b[i]'s copy where the reference would
+// take least squares (regular[i] = 0; an empty system counts as such, factor.rs:222).
+template <typename T>
+int host_batched(int64_t count, const int64_t* n, const int64_t* k, const T* const* a, const T* const* b, T* const* x,
+                 int* regular) {
+    if (count < 0) return -1;
+    if (count == 0) return OK;
+    if (!n) return -2;
+    for (int64_t i = 0; i < count; ++i) if (n[i] < 0) return -2;
+    if (!k) return -3;
+    for (int64_t i = 0; i < count; ++i) if (k[i] < 0) return -3;
+    if (!a) return -4;
+    for (int64_t i = 0; i < count; ++i) if (n[i] > 0 && !a[i]) return -4;
+    if (!b) return -5;
+    for (int64_t i = 0; i < count; ++i) if (n[i] > 0 && k[i] > 0 && !b[i]) return -5;
+    if (!x) return -6;
+    for (int64_t i = 0; i < count; ++i) if (n[i] > 0 && k[i] > 0 && !x[i]) return -6;
+    std::vector<int64_t> idx, ns, ks, ao, bo, po;
+    int64_t ta = 0, tb = 0, tp = 0;
+    for (int64_t i = 0; i < count; ++i) {
+        if (n[i] == 0) { if (regular) regular[i] = 0; continue; }
+        if (n[i] > 512 || k[i] > 8) {  // beyond the batched kernel: this system on its own
+            int reg = 0;
+            const int rc = host_solve<T>(a[i], n[i], b[i], k[i], x[i], &reg);
+            if (rc) return rc;
+            if (!reg && k[i]) std::memcpy(x[i], b[i], sizeof(T) * (size_t)(n[i] * k[i]));
+            if (regular) regular[i] = reg;
+            continue;
+        }
+        idx.push_back(i); ns.push_back(n[i]); ks.push_back(k[i]);
+        ao.push_back(ta); bo.push_back(tb); po.push_back(tp);
+        ta += n[i] * n[i]; tb += n[i] * k[i]; tp += n[i];
+    }
+    const int64_t m = (int64_t)idx.size();
+    if (m == 0) return OK;
+    std::vector<T> ha((size_t)ta), hb((size_t)std::max<int64_t>(tb, 1));
+    for (int64_t j = 0; j < m; ++j) {
+        const int64_t i = idx[j];
+        std::memcpy(ha.data() + ao[j], a[i], sizeof(T) * (size_t)(n[i] * n[i]));
+        if (k[i]) std::memcpy(hb.data() + bo[j], b[i], sizeof(T) * (size_t)(n[i] * k[i]));
+    }
+    hipStream_t s = thread_stream();
+    T* da = (T*)pool_get(sizeof(T) * (size_t)ta);
+    T* db = (T*)pool_get(sizeof(T) * (size_t)std::max<int64_t>(tb, 1));
+    int64_t* dp = (int64_t*)pool_get(sizeof(int64_t) * (size_t)tp);
+    int* dr = (int*)pool_get(sizeof(int) * (size_t)m);
+    int rc = OK;
+    if (!da || !db || !dp || !dr) rc = hip_status(hipErrorOutOfMemory, "pool_get");
+    if (!rc && (hipMemcpyAsync(da, ha.data(), sizeof(T) * ta, hipMemcpyHostToDevice, s) != hipSuccess ||
+                (tb && hipMemcpyAsync(db, hb.data(), sizeof(T) * tb, hipMemcpyHostToDevice, s) != hipSuccess)))
+        rc = hip_status(hipErrorUnknown, "H2D");
+    if (!rc) {
+        long long h = 0;
+        rc = batched_plan(m, ns.data(), ks.data(), ao.data(), bo.data(), po.data(), sizeof(T) == 8, da, db, dp, dr, 0, 32, 2,
+                          &h);
+        if (!rc) rc = batched_run(h, false, s);
+        if (h) { const int rf = batched_free(h); if (!rc) rc = rf; }
+    }
+    std::vector<int> hr((size_t)m);
+    if (!rc && ((tb && hipMemcpyAsync(hb.data(), db, sizeof(T) * tb, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+                hipMemcpyAsync(hr.data(), dr, sizeof(int) * m, hipMemcpyDeviceToHost, s) != hipSuccess))
+        rc = hip_status(hipErrorUnknown, "D2H");
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = hip_status(hipErrorUnknown, "sync");
+    pool_put(da); pool_put(db); pool_put(dp); pool_put(dr);
+    if (rc) return rc;
+    for (int64_t j = 0; j < m; ++j) {  // b is left on the device as it was where the system is not regular
+        const int64_t i = idx[j];
+        if (k[i]) std::memcpy(x[i], hb.data() + bo[j], sizeof(T) * (size_t)(n[i] * k[i]));
+        if (regular) regular[i] = hr[j];
+    }
+    return OK;
+}
+
 // Row r of a cell's reference-layout factors (column-major, "row j holds column j") into the batched kernel's layout
 // (row-major, row perm[r] holding position r): the inverse of batched_to_reference.
 template <typename T>
@@ -793,6 +869,14 @@ int hs_dsolve_stack(int64_t count, int64_t n, int64_t k, const double* a, const 
 }
 int hs_ssolve_stack(int64_t count, int64_t n, int64_t k, const float* a, const float* b, float* x, int* regular) {
     return host_stack<float>(count, n, k, a, b, x, regular, nullptr, nullptr);
+}
+int hs_dgesv_batched(int64_t count, const int64_t* n, const int64_t* k, const double* const* a, const double* const* b,
+                     double* const* x, int* regular) {
+    return host_batched<double>(count, n, k, a, b, x, regular);
+}
+int hs_sgesv_batched(int64_t count, const int64_t* n, const int64_t* k, const float* const* a, const float* const* b,
+                     float* const* x, int* regular) {
+    return host_batched<float>(count, n, k, a, b, x, regular);
 }
 int hs_dlu_stack(int64_t count, int64_t n, const double* a, double* factors, int64_t* perm) {
     return host_stack<double>(count, n, 0, a, nullptr, nullptr, nullptr, factors, perm);

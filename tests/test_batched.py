@@ -298,3 +298,41 @@ def test_refinement_stops_at_its_cap_and_skips_singular_systems(dv):
     assert (berr[:3] <= 10 * np.array(ns[:3]) * np.finfo(np.float64).eps).all() and np.isnan(berr[3])
     assert not p.x(3).abs().sum().item()
     p.close()
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_host_pointer_uneven_batch_matches_each_cell(dtype):
+    """hs_*gesv_batched through the C ABI: systems of different orders from host pointers, each X_i the oracle's
+    `solve` of that cell bit for bit (b's copy where it is not regular): orders past 512 and more than 8 right-hand
+    sides run one by one, an empty system and a singular one count as not regular."""
+    import ctypes
+    import torch
+    torch.cuda.set_device(0)
+    from rustymath_amd import _lib
+    from support import random
+    lib = _lib.load()
+    ns = [0, 1, 17, 64, 129, 300, 512, 530, 5, 200]
+    ks = [1, 3, 0, 8, 2, 5, 1, 2, 9, 4]
+    mats = [random(n, n, 100 + 7 * i, dtype) for i, n in enumerate(ns)]
+    mats[5][40] = 0.0  # a zero row: singular, left to least squares
+    rhs = [random(n, k, 300 + 11 * i, dtype) for i, (n, k) in enumerate(zip(ns, ks))]
+    xs = [np.full((n, k), np.nan, dtype) for n, k in zip(ns, ks)]
+    cnt = len(ns)
+    n_arr, k_arr = np.array(ns, np.int64), np.array(ks, np.int64)
+    ptrs = lambda arrs: (ctypes.c_void_p * cnt)(*[x.ctypes.data for x in arrs])
+    regular = (ctypes.c_int * cnt)()
+    f = getattr(lib, 'hs_dgesv_batched' if dtype == np.float64 else 'hs_sgesv_batched')
+    print(f'host uneven batch {np.dtype(dtype).name}: {cnt} systems', flush=True)
+    rc = f(cnt, n_arr.ctypes.data, k_arr.ctypes.data, ptrs(mats), ptrs(rhs), ptrs(xs), regular)
+    assert rc == 0, _lib.load().hs_last_error()
+    for i in range(cnt):
+        want = oracle.solve(mats[i], rhs[i]) if ns[i] else None
+        assert bool(regular[i]) == (want is not None), f'system {i}: regular {regular[i]}'
+        if want is not None:
+            assert bits_equal(xs[i], want), f'system {i}: solution'
+        elif ns[i]:
+            assert bits_equal(xs[i], rhs[i]), f'system {i}: b expected'
+    assert not regular[0] and not regular[5]
+    assert f(-1, None, None, None, None, None, None) == -1
+    bad = np.array([3], np.int64)
+    assert f(1, bad.ctypes.data, np.array([-1], np.int64).ctypes.data, None, None, None, None) == -3

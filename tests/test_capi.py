@@ -155,3 +155,18 @@ def test_bindings_match_the_header():
     got = {m.group(1): (0 if not m.group(2).strip() else m.group(2).count(':'))
            for m in re.finditer(r'pub fn (hs_\w+)\(([^)]*)\)', rs)}
     assert got == want
+
+
+def test_uneven_batched_solve_rejects_bad_arguments_on_the_host():
+    "hs_*gesv_batched checks its arguments in order before any device work (LAPACK-style negative positions)."
+    import numpy as np
+    from rustymath_amd import _lib
+    lib = _lib.load()
+    n, k = np.array([3, 2], np.int64), np.array([1, 1], np.int64)
+    for f in (lib.hs_dgesv_batched, lib.hs_sgesv_batched):
+        assert f(-1, None, None, None, None, None, None) == -1
+        assert f(0, None, None, None, None, None, None) == 0
+        assert f(2, None, k.ctypes.data, None, None, None, None) == -2
+        assert f(2, np.array([3, -1], np.int64).ctypes.data, k.ctypes.data, None, None, None, None) == -2
+        assert f(2, n.ctypes.data, np.array([1, -2], np.int64).ctypes.data, None, None, None, None) == -3
+        assert f(2, n.ctypes.data, k.ctypes.data, None, None, None, None) == -4

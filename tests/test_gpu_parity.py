@@ -83,6 +83,40 @@ def test_vector_products_follow_by_vector(la, dtype):
         assert bits_equal(la.matmul(w, a), oracle.matmul(w, a))
 
 
+def test_vector_products_take_the_known_answer_orders(la):
+    """The oracle's known answers for `by_vector`'s orders (`dots`, its `dot` tail rows and tail items,
+    `narrow_dots`, `narrow_sums` with and without whole quads, `scaled_runs`): each differs from an in-order fold."""
+    B = 2.0 ** 53
+    r8 = [1.0, 0.0, -B, 0.0, B, 0.0, 0.0, 0.0]
+    stored = lambda n: np.tile(np.array([B, 1.0, -B, 1.0])[:, None], (1, n))
+    cases = [(la.matmul(np.tile(r8 + [0.0] * 8, (5, 1)), np.ones(16)), [0.0, 0.0, 0.0, 0.0, 1.0]),
+             (la.matmul(np.tile(r8 + [0.0] * 8 + [1.0], (4, 1)), np.ones(17)), [1.0] * 4),
+             (la.matmul(np.tile(r8, (3, 1)), np.ones(8)), [1.0] * 3),
+             (la.matmul(np.tile([B, 1.0, 1.0, -B], (3, 1)), np.ones(4)), [1.0] * 3),
+             (la.matmul(np.ones(4), stored(8)), [2.0] * 8),
+             (hs_gemm(stored(8), np.ones((4, 1)), opa='T')[:, 0], [2.0] * 8),
+             (hs_gemm(stored(5), np.ones((4, 1)), opa='T')[:, 0], [1.0] * 5),
+             (la.matmul(np.ones(4), stored(20)), [1.0] * 20)]
+    for i, (got, want) in enumerate(cases):
+        assert bits_equal(got, np.array(want)), (i, got)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
+def test_transposed_matrix_times_vector_follows_narrow_sums(dtype):
+    """A stored k x m matrix read transposed times a vector: `narrow_sums` for m <= 8, whose quads stop at the last
+    stored row with 8 items left in the data (core/src/linalg.rs:943), and `scaled_runs` past 8 results."""
+    rng = np.random.default_rng(11)
+    for m in [1, 3, 5, 8, 9, 20]:
+        for k in [1, 3, 4, 7, 9, 37]:
+            a = rng.standard_normal((k, m)).astype(dtype)
+            x = rng.standard_normal((k, 1)).astype(dtype)
+            c0 = rng.standard_normal((m, 1)).astype(dtype)
+            for alpha, beta in [(1.0, 0.0), (-2.0, 0.5)]:
+                got = hs_gemm(a, x, alpha, beta, c0.copy(), 'T', 'N')
+                want = oracle.gemm(a, x, alpha, beta, c0.copy(), 'T', 'N')
+                assert bits_equal(got, want), (m, k, alpha, beta)
+
+
 def test_notebook_known_answers(la):
     "nbs/linalg.ipynb cells 4, 6, 25, 27 and README.md:79-82 on the GPU, signed zeros included."
     a = np.array([[1., 2], [2, 5]])

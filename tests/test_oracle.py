@@ -81,6 +81,38 @@ def test_vector_products_read_vectors_as_rows_and_columns():
     assert oracle.matmul(v, v) == v @ v
 
 
+def test_vector_products_take_the_reference_summation_orders():
+    """Known answers for each of `by_vector`'s summation orders (core/src/linalg.rs:1020-1073), each chosen with
+    B = 2^53 so that an in-order fold or another pairing gives a different result (B + 1 rounds to B, B + 2 does not).
+    - `dots` (:903-917, k > 8): a row of 1, 0, -B, 0, B and zeros puts 1 and B in running sum 0, -B in sum 2; the
+      pairwise reduction gives ((1 + B) + -B) = 0. Rows past the last quad take `dot` (factor.rs:466-473), eight sums
+      folded in order from zero: 1, 1 - B, 1. Items past the last chunk of eight add after the reduction: one more 1
+      gives 0 + 1 = 1, where folding it into sum 0 first, or an in-order fold, gives 2.
+    - `narrow_dots` (:923-937, k <= 8): two running sums by item parity. The same 8-item row keeps 1, -B, B in sum 0:
+      1 - B, then 1 (where `dots` would give 0). B, 1, 1, -B gives (B + 1) + (1 - B) = B + (1 - B) = 1; in order, 0.
+    - `narrow_sums` (:941-956, a transposed matrix with at most 8 results): stored rows whose 8-item read stays in
+      the data go in quads, row p of a quad into set p mod 4, the sets summed (s0 + s2) + (s1 + s3). Stored rows B, 1,
+      -B, 1 of 8 items give (B - B) + (1 + 1) = 2; in order or paired by neighbours, 1. With 5 items to a row only
+      the first 3 rows are whole, fewer than a quad, so every row folds into set 0 in order: 1.
+    - `scaled_runs` (:882-900, more than 8 results): an in-order fold over the stored rows, running B, B, 0, 1: 1."""
+    B = 2.0 ** 53
+    r8 = [1.0, 0.0, -B, 0.0, B, 0.0, 0.0, 0.0]
+    assert bits_equal(oracle.matmul(np.tile(r8 + [0.0] * 8, (5, 1)), np.ones(16)), np.array([0.0, 0.0, 0.0, 0.0, 1.0]))
+    assert bits_equal(oracle.matmul(np.tile(r8 + [0.0] * 8 + [1.0], (4, 1)), np.ones(17)), np.ones(4))
+    assert bits_equal(oracle.matmul(np.tile(r8, (3, 1)), np.ones(8)), np.ones(3))
+    assert bits_equal(oracle.matmul(np.tile([B, 1.0, 1.0, -B], (3, 1)), np.ones(4)), np.ones(3))
+    stored = lambda n: np.tile(np.array([B, 1.0, -B, 1.0])[:, None], (1, n))
+    assert bits_equal(oracle.matmul(np.ones(4), stored(8)), np.full(8, 2.0))
+    assert bits_equal(oracle.gemm(stored(8), np.ones((4, 1)), opa='T')[:, 0], np.full(8, 2.0))
+    assert bits_equal(oracle.gemm(stored(5), np.ones((4, 1)), opa='T')[:, 0], np.ones(5))
+    assert bits_equal(oracle.matmul(np.ones(4), stored(20)), np.ones(20))
+    # FP32 takes the same orders with B = 2^24.
+    b32 = np.float32(2.0 ** 24)
+    row32 = np.array([1, 0, -b32, 0, b32] + [0] * 11, np.float32)
+    assert bits_equal(oracle.matmul(np.tile(row32, (5, 1)), np.ones(16, np.float32)),
+                      np.array([0, 0, 0, 0, 1], np.float32))
+
+
 def test_notebook_known_answers():
     """nbs/linalg.ipynb cells 4, 6, 25, 27 and README.md:79-82, including the signed zeros that the order of the
     substitutions fixes: U's solve divides by a reciprocal for several right-hand sides and by a quotient for one."""

@@ -77,22 +77,32 @@ __device__ __forceinline__ void put_key(unsigned long long* k, unsigned tag, uns
         xput(&k[2], t | pp, agent);
     }
 }
+// A key's words as loaded (FP32: w[0]; FP64: w[0..2]), and their decoding: false while any word is older than `tag`.
+template <typename T>
+__device__ __forceinline__ void key_words(const unsigned long long* k, unsigned long long* w) {
+    w[0] = xget(&k[0]);
+    if constexpr (sizeof(T) == 8) { w[1] = xget(&k[1]); w[2] = xget(&k[2]); }
+}
+template <typename T>
+__device__ __forceinline__ bool key_decode(const unsigned long long* w, unsigned tag, unsigned long long& kv, int& kp) {
+    if constexpr (sizeof(T) == 4) {
+        kv = (w[0] >> 15) & 0x7fffffffffull;
+        const int pp = (int)(w[0] & 0x7fff);
+        kp = pp == 0 ? 0x7fffffff : 32767 - pp;
+        return (unsigned)(w[0] >> 46) == tag;
+    } else {
+        kv = ((w[0] & 0xffffffffull) << 32) | (w[1] & 0xffffffffull);
+        const int pp = (int)(w[2] & 0x7fff);
+        kp = pp == 0 ? 0x7fffffff : 32767 - pp;
+        return (unsigned)(w[0] >> 32) == tag && (unsigned)(w[1] >> 32) == tag && (unsigned)(w[2] >> 32) == tag;
+    }
+}
 // Reads a key; false while any of its words is older than `tag`.
 template <typename T>
 __device__ __forceinline__ bool get_key(const unsigned long long* k, unsigned tag, unsigned long long& kv, int& kp) {
-    if constexpr (sizeof(T) == 4) {
-        const unsigned long long w = xget(&k[0]);
-        kv = (w >> 15) & 0x7fffffffffull;
-        const int pp = (int)(w & 0x7fff);
-        kp = pp == 0 ? 0x7fffffff : 32767 - pp;
-        return (unsigned)(w >> 46) == tag;
-    } else {
-        const unsigned long long w0 = xget(&k[0]), w1 = xget(&k[1]), w2 = xget(&k[2]);
-        kv = ((w0 & 0xffffffffull) << 32) | (w1 & 0xffffffffull);
-        const int pp = (int)(w2 & 0x7fff);
-        kp = pp == 0 ? 0x7fffffff : 32767 - pp;
-        return (unsigned)(w0 >> 32) == tag && (unsigned)(w1 >> 32) == tag && (unsigned)(w2 >> 32) == tag;
-    }
+    unsigned long long w[3];
+    key_words<T>(k, w);
+    return key_decode<T>(w, tag, kv, kp);
 }
 template <typename T>
 __device__ __forceinline__ void put_item(unsigned long long* r, int j, unsigned tag, T v, bool agent) {
@@ -145,7 +155,7 @@ struct PanelLds {
     int s_kp[2][PT / 64];
     T s_piv[2][PW];  // the pivot rows of the last two columns
     T s_rcp[2];      // their pivots' reciprocals
-    T s_hand[PT / 64][PW + 2];
+    alignas(16) T s_hand[PT / 64][PW + (sizeof(T) == 4 ? 4 : 2)];  // rows of 16-byte multiples: b128 writes
     T s_u11[PW][PW];
     int s_p[2], s_abort, s_skip[PW], s_dest[PT];
 };
@@ -357,8 +367,14 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
             const bool holder = has && pos == bkp;
             if (__ballot(holder)) {
                 if (holder) {
+                    // Only the items that lanes below rem read: v[j] for j <= rem, four at a time.
 #pragma unroll
-                    for (int j = 0; j < PW; ++j) s_hand[wave][j] = v[j];
+                    for (int j0 = 0; j0 < PW; j0 += 4) {
+                        if (j0 <= rem) {
+#pragma unroll
+                            for (int j = j0; j < j0 + 4; ++j) s_hand[wave][j] = v[j];
+                        }
+                    }
                     s_hand[wave][PW] = x0;
                     s_hand[wave][PW + 1] = l_prev;
                 }
@@ -389,6 +405,11 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         // The pivot row reaches every lane as scalars: one LDS read per wave (lane j takes item j, with every lane
         // active, so that each lane's register holds its item), then lane j's item is read out as a uniform operand,
         // rather than every lane reading every item from LDS.
+        // The last wave's first look at the workgroups' keys goes out before the update, from an address clamped into
+        // the buffer, and is decoded after it: a key published by then costs no round trip of its own.
+        const bool poller = nwg > 1 && wave == WAVES - 1;
+        unsigned long long kw[3] = {0ull, 0ull, 0ull};
+        if (poller) key_words<T>(ex->key[buf][lane < nwg ? lane : 0], kw);
         T mine = prev[lane];
         asm volatile("" : "+v"(mine));  // loaded here, by every lane, not sunk into the branch below
         if (pending) {
@@ -418,11 +439,11 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
             // The last wave compares the workgroups' keys, then fetches the winner's row. Not wave 0: where pivots lie
             // on the diagonal (no row moves), the pivot row of every column of a panel sits in wave 0 of workgroup 0,
             // whose hand-off of that row would otherwise delay the poll behind it.
-            if (wave == WAVES - 1) {
+            if (poller) {
                 unsigned long long gkv = 0ull;
                 int gkp = 0x7fffffff, gl = 0;
                 bool ok = true;
-                if (lane < nwg) {
+                if (lane < nwg && !key_decode<T>(kw, tag, gkv, gkp)) {
                     uint64_t spins = 0;
                     while (!get_key<T>(ex->key[buf][lane], tag, gkv, gkp)) {
                         if (++spins > SPIN_LIMIT) { ok = false; break; }

@@ -25,12 +25,18 @@ namespace hs {
 namespace {
 
 constexpr int PW = 64;            // widest panel
-constexpr int PT = 512;           // threads (rows) of a panel workgroup
+constexpr int PT = 512;           // rows of a panel workgroup, one a thread
 constexpr int SG = 16;            // columns a panel workgroup stages through LDS at a time, loading and storing its rows
 constexpr int STOP_AHEAD = 8;     // columns before a panel's end at which its workers stop taking tiles
 constexpr int MAX_WG = 32;        // panel workgroups: one per CU of one XCD, one flag per lane of a wave
 constexpr uint64_t SPIN_LIMIT = 1ull << 22;
 constexpr int ARRIVE_STRIPS = 8;  // column strips of a panel's in-block update (at most 192 columns, 32 a strip)
+
+// The FP32 panel workgroups have a ninth wave that holds no row and only polls the exchange, so that no wave waits for
+// the other workgroups' keys and pivot row behind its own share of the column's update. The FP64 panel's registers
+// leave room for two waves a SIMD only.
+template <typename T> constexpr bool own_poller() { return sizeof(T) == 4; }
+template <typename T> constexpr int panel_threads() { return PT + (own_poller<T>() ? 64 : 0); }
 
 template <typename T> struct Bits;
 template <> struct Bits<double> { typedef unsigned long long u; };
@@ -255,7 +261,7 @@ __device__ __forceinline__ long long ticks() { return __builtin_amdgcn_s_memreal
 // pivot leaves its column undivided and skips the rest of that leaf (factor.rs:606). Where `prof` is not
 // null, workgroup 0 adds the ticks of each phase of the column loop into prof[0..7].
 template <typename T>
-__global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int64_t rows, int w,
+__global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict__ a, int64_t lda, int64_t c0, int64_t rows, int w,
                                                    int64_t* __restrict__ pivots, long long* __restrict__ moves,
                                                    const int* __restrict__ leaf_end, T* __restrict__ scratch,
                                                    Exchange* __restrict__ ex, unsigned tag0, int nwg, int* status,
@@ -265,7 +271,8 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     // again. The panel's workgroups test the word once their rows are in (its load then adds no latency of its own).
     const int void_run = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (blockIdx.x & 7) {  // off the panel's XCD: a worker of the trailing product, if there is one
-        if (job.C && !void_run) trail_work<T>(job, tag0, *reinterpret_cast<WorkLds<T>*>(lds));
+        // Eight waves compute a tile; a ninth returns here, and a barrier waits only for the waves that have not ended.
+        if (job.C && !void_run && threadIdx.x < PT) trail_work<T>(job, tag0, *reinterpret_cast<WorkLds<T>*>(lds));
         return;
     }
     const int g = (int)(blockIdx.x >> 3);
@@ -285,14 +292,16 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     auto& s_skip = L.s_skip;
     auto& s_dest = L.s_dest;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr int WAVES = PT / 64;
+    constexpr int WAVES = PT / 64;  // the waves that hold rows
+    constexpr int POLL_WAVE = own_poller<T>() ? WAVES : WAVES - 1;
+    const bool rower = tid < PT;  // false for the polling wave of its own
     typedef typename std::conditional<sizeof(T) == 4, unsigned, unsigned long long>::type Key;  // a candidate's key
     const bool timed = prof != nullptr && g == 0 && threadIdx.x == 0;
     long long tp[8] = {0, 0, 0, 0, 0, 0, 0, 0}, tl = timed ? ticks() : 0;
 #define HS_PHASE(q) do { if (timed) { long long t_ = ticks(); tp[q] += t_ - tl; tl = t_; } } while (0)
 
-    const int64_t phys = (int64_t)g * PT + tid;  // this thread's row, relative to c0
-    const bool has = phys < rows;
+    const int64_t phys = (int64_t)g * PT + (rower ? tid : 0);  // this thread's row, relative to c0
+    const bool has = rower && phys < rows;
     int pos = (int)phys;
     // The row's items, every load issued before any is used, from addresses clamped into the panel (a load behind a
     // per-lane condition would be issued and waited for one at a time). Items past the panel's width only ever meet
@@ -304,22 +313,24 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     const int wg_rows = (int)(rows - (int64_t)g * PT < PT ? rows - (int64_t)g * PT : PT);
 #pragma unroll
     for (int ch = 0; ch < PW / SG; ++ch) {
-        T tmp[SG];
+        if (rower) {
+            T tmp[SG];
 #pragma unroll
-        for (int q = 0; q < SG; ++q) {
-            const int idx = tid + PT * q, row = idx / SG, col = ch * SG + idx % SG;
-            const int64_t r = (int64_t)g * PT + (row < wg_rows ? row : wg_rows - 1);
-            tmp[q] = a[(c0 + r) * lda + c0 + (col < w ? col : w - 1)];
-        }
-        asm volatile("" ::: "memory");
+            for (int q = 0; q < SG; ++q) {
+                const int idx = tid + PT * q, row = idx / SG, col = ch * SG + idx % SG;
+                const int64_t r = (int64_t)g * PT + (row < wg_rows ? row : wg_rows - 1);
+                tmp[q] = a[(c0 + r) * lda + c0 + (col < w ? col : w - 1)];
+            }
+            asm volatile("" ::: "memory");
 #pragma unroll
-        for (int q = 0; q < SG; ++q) {
-            const int idx = tid + PT * q;
-            s_stage[idx / SG][idx % SG] = tmp[q];
+            for (int q = 0; q < SG; ++q) {
+                const int idx = tid + PT * q;
+                s_stage[idx / SG][idx % SG] = tmp[q];
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int jj = 0; jj < SG; ++jj) v[ch * SG + jj] = s_stage[tid][jj];
+        for (int jj = 0; jj < SG; ++jj) v[ch * SG + jj] = rower ? s_stage[tid][jj] : T(0);
         __syncthreads();
     }
 
@@ -335,19 +346,21 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
     T* sc = scratch + (has ? phys : 0);  // this row's multiplier of the current column: scratch[c][row]
     bool pending = false;
     int skip_prev = 0;
+    int col_skip = 0;  // the previous column's zero-pivot skip, the same for every row
 
     HS_PHASE(5);  // the rows' loads and the set-up
 #pragma clang loop unroll(disable)
     for (int c = 0; c < w; ++c) {
         const int buf = c & 1;
         const int rem = w - c;  // items of the current row that lie in the panel
+        const int leaf_skip = s_skip[c];  // read now, used once the pivot is known
         const unsigned tag = tag0 + (unsigned)c;
         // This row's candidacy for column c and the wave's best.
         const bool cand = has && pos >= c;
         Key kv = cand ? (Key)mag_key(x0) : Key(0);
         int kp = cand ? pos : 0x7fffffff, hl = 0;
         wave_best(kv, kp, hl);
-        if (lane == 0) { s_kv[buf][wave] = kv; s_kp[buf][wave] = kp; }
+        if (lane == 0 && rower) { s_kv[buf][wave] = kv; s_kp[buf][wave] = kp; }
         __syncthreads();
         HS_PHASE(0);
         // The best of the waves' keys: lane q takes wave q's, and one more wave reduction picks it (larger key, then
@@ -362,7 +375,14 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
 
         // The workgroup's best row goes out before the rest of the update runs: the holding lane writes its items,
         // still owing the previous column's update, to its wave's LDS row, and lane j applies that update to item j
-        // (one fused multiply-add, the same one the holder would apply) and publishes it.
+        // (one fused multiply-add, the same one the holder would apply) and publishes it. That update skips the items
+        // that the previous column's zero pivot skips (the same for every row), whichever rows the lanes hold.
+        auto hand_item = [&](const T* h) {
+            if (c == 0) return h[lane];
+            const int j = lane + 1 < PW ? lane + 1 : PW - 1;
+            const T u = __builtin_elementwise_fma(h[PW + 1], -prev[j], h[j]);
+            return lane == 0 ? h[PW] : (lane == PW - 1 ? T(0) : (j >= col_skip ? u : h[j]));
+        };
         {
             const bool holder = has && pos == bkp;
             if (__ballot(holder)) {
@@ -381,15 +401,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const T* h = s_hand[wave];
-                T item;
-                if (c == 0) {
-                    item = h[lane];
-                } else {
-                    const int j = lane + 1 < PW ? lane + 1 : PW - 1;
-                    const T u = __builtin_elementwise_fma(h[PW + 1], -prev[j], h[j]);
-                    item = lane == 0 ? h[PW] : (lane == PW - 1 ? T(0) : (j >= skip_prev ? u : h[j]));
-                }
+                const T item = hand_item(s_hand[wave]);
                 if (nwg > 1) {
                     if (lane < rem) put_item<T>(ex->row[buf][g], lane, tag, item, agent);
                 } else {
@@ -407,7 +419,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         // rather than every lane reading every item from LDS.
         // The last wave's first look at the workgroups' keys goes out before the update, from an address clamped into
         // the buffer, and is decoded after it: a key published by then costs no round trip of its own.
-        const bool poller = nwg > 1 && wave == WAVES - 1;
+        const bool poller = nwg > 1 && wave == POLL_WAVE;
         unsigned long long kw[3] = {0ull, 0ull, 0ull};
         if (poller) key_words<T>(ex->key[buf][lane < nwg ? lane : 0], kw);
         T mine = prev[lane];
@@ -489,7 +501,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         // column's candidates to be out.
         if (has && pos > c) {
             const bool zero = pivot == T(0);
-            const int skip = zero ? s_skip[c] : 0;
+            const int skip = zero ? leaf_skip : 0;
             const T l = zero ? x0 : (fabs(pivot) >= tiny ? x0 * rcp : x0 / pivot);
             x0 = 1 >= skip ? __builtin_elementwise_fma(l, -piv1, v[1]) : v[1];
             *sc = l;
@@ -497,6 +509,7 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
             skip_prev = skip;
             pending = true;
         }
+        col_skip = pivot == T(0) ? leaf_skip : 0;
         prev = piv;
         sc += rows;
         // The workers take no new tile once STOP_AHEAD columns remain, so that the tiles in progress end about when
@@ -517,16 +530,20 @@ __global__ __launch_bounds__(PT) void panel_kernel(T* __restrict__ a, int64_t ld
         const int upos = pos < PW ? pos : PW - 1;
 #pragma unroll
         for (int j = 0; j < PW; ++j) v[j] = j < pos ? v[j] : s_u11[upos][(j - pos) & (PW - 1)];
-        s_dest[tid] = pos;
+        if (rower) s_dest[tid] = pos;
 #pragma unroll
         for (int ch = 0; ch < PW / SG; ++ch) {
+            if (rower) {
 #pragma unroll
-            for (int jj = 0; jj < SG; ++jj) s_stage[tid][jj] = v[ch * SG + jj];
+                for (int jj = 0; jj < SG; ++jj) s_stage[tid][jj] = v[ch * SG + jj];
+            }
             __syncthreads();
+            if (rower) {
 #pragma unroll
-            for (int q = 0; q < SG; ++q) {
-                const int idx = tid + PT * q, row = idx / SG, col = ch * SG + idx % SG;
-                if (row < wg_rows && col < w) a[(c0 + s_dest[row]) * lda + c0 + col] = s_stage[row][idx % SG];
+                for (int q = 0; q < SG; ++q) {
+                    const int idx = tid + PT * q, row = idx / SG, col = ch * SG + idx % SG;
+                    if (row < wg_rows && col < w) a[(c0 + s_dest[row]) * lda + c0 + col] = s_stage[row][idx % SG];
+                }
             }
             __syncthreads();
         }
@@ -1284,12 +1301,13 @@ int getrf(int64_t n, T* a, int64_t lda, int64_t* pivots, hipStream_t s, int bf16
                     TrailProfile::PanelRec P{nullptr, nullptr, rows, w};
                     HS_HIP(g_trail.event(&P.a));
                     HS_HIP(g_trail.event(&P.b));
-                    hipExtLaunchKernelGGL(panel_kernel<T>, dim3(grid), dim3(PT), 0, s, P.a, P.b, 0, a, lda, c0, rows, w,
+                    hipExtLaunchKernelGGL(panel_kernel<T>, dim3(grid), dim3(panel_threads<T>()), 0, s, P.a, P.b, 0, a, lda,
+                                          c0, rows, w,
                                           pivots, mv, d_leaf, scratch, ex, (unsigned)(i * PW + 1), nwg, status,
                                           st->on ? st->prof : nullptr, ahead ? job : none, local);
                     g_trail.panels.push_back(P);
                 } else {
-                    panel_kernel<T><<<grid, PT, 0, s>>>(a, lda, c0, rows, w, pivots, mv, d_leaf, scratch, ex,
+                    panel_kernel<T><<<grid, panel_threads<T>(), 0, s>>>(a, lda, c0, rows, w, pivots, mv, d_leaf, scratch, ex,
                                                          (unsigned)(i * PW + 1), nwg, status, st->on ? st->prof : nullptr,
                                                          ahead ? job : none, local);
                 }

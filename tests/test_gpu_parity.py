@@ -219,6 +219,27 @@ def test_lu_edge_matrices_match_the_oracle(la, dtype):
 
 
 @pytest.mark.parametrize('dtype', [np.float64, np.float32])
+@pytest.mark.parametrize('n', [40, 600])
+def test_pivot_row_after_a_zero_pivot_keeps_its_leaf(la, dtype, n):
+    """A zero pivot in column 3 skips the rest of its leaf (factor.rs:606), so the next pivot row (row 4) keeps its
+    items there, whatever row 3 holds: U[4, 5] stays 2 beside an infinity above it (an update would give 0 * inf = NaN)
+    and U[4, 6] stays -0 (an update would give +0). Rows 1 and 2, pivots before the zero one, share row 4's wave on the
+    GPU; the items that lanes holding such rows hand over must take the skip of the column, not of their own rows.
+    n = 600 runs the panel over two workgroups, whose rows go through the exchange."""
+    a = np.zeros((n, n))
+    a[np.arange(n), np.arange(n)] = 4.0
+    a[3, 3] = 0.0
+    a[3, 4], a[3, 5], a[3, 6] = 1.0, np.inf, -2.0
+    a[4, 5], a[4, 6] = 2.0, -0.0
+    a[5:, 4] = 0.5
+    a = a.astype(dtype)
+    f = la.lu(a)
+    want, perm, _ = oracle.lu(a)
+    assert want[5, 4] == 2.0 and np.signbit(want[6, 4])
+    assert (f.p == perm).all() and bits_equal(f.factors, want)
+
+
+@pytest.mark.parametrize('dtype', [np.float64, np.float32])
 def test_hard_matrices_scale_exactly(la, dtype):
     "tests/linalg.rs:466-483 on the GPU: power-of-two scaling scales U exactly and leaves perm, L and solves alone."
     n = 40

@@ -430,9 +430,24 @@ __global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict
 #pragma unroll
                 for (int j0 = 2; j0 < PW; j0 += 16) {
                     if (j0 <= rem) {
+                        if constexpr (sizeof(T) == 4) {
+                            // Four items are read out into four scalars before their FMAs, so that no FMA waits
+                            // for the read just before it.
 #pragma unroll
-                        for (int j = j0; j < j0 + 16 && j < PW; ++j)
-                            v[j - 1] = __builtin_elementwise_fma(l_prev, -lane_value(mine, j), v[j]);
+                            for (int j = j0; j < j0 + 16 && j < PW; j += 4) {
+                                T u[4];
+#pragma unroll
+                                for (int q = 0; q < 4; ++q) u[q] = j + q < PW ? lane_value(mine, j + q) : T(0);
+                                asm volatile("" ::"s"(u[0]), "s"(u[1]), "s"(u[2]), "s"(u[3]));
+#pragma unroll
+                                for (int q = 0; q < 4; ++q)
+                                    if (j + q < PW) v[j + q - 1] = __builtin_elementwise_fma(l_prev, -u[q], v[j + q]);
+                            }
+                        } else {
+#pragma unroll
+                            for (int j = j0; j < j0 + 16 && j < PW; ++j)
+                                v[j - 1] = __builtin_elementwise_fma(l_prev, -lane_value(mine, j), v[j]);
+                        }
                     }
                 }
             } else {  // a zero pivot: the rest of its leaf (at most 16 columns) keeps its items

@@ -159,6 +159,7 @@ struct PanelLds {
     T s_stage[PT][SG + 1];
     unsigned long long s_kv[2][PT / 64];
     int s_kp[2][PT / 64];
+    unsigned long long s_best[2];  // FP32: the workgroup's best key and position, packed, by column parity
     T s_piv[2][PW];  // the pivot rows of the last two columns
     T s_rcp[2];      // their pivots' reciprocals
     alignas(16) T s_hand[PT / 64][PW + (sizeof(T) == 4 ? 4 : 2)];  // rows of 16-byte multiples: b128 writes
@@ -280,6 +281,7 @@ __global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict
     // local = 0: words go out at agent scope, and where the workgroups run does not matter.
     const bool agent = !local;
     PanelLds<T>& L = *reinterpret_cast<PanelLds<T>*>(lds);
+    if (threadIdx.x == 0) { L.s_best[0] = 0ull; L.s_best[1] = 0ull; }  // before the barriers of the rows' loads
     auto& s_stage = L.s_stage;
     auto& s_kv = L.s_kv;
     auto& s_kp = L.s_kp;
@@ -360,16 +362,31 @@ __global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict
         Key kv = cand ? (Key)mag_key(x0) : Key(0);
         int kp = cand ? pos : 0x7fffffff, hl = 0;
         wave_best(kv, kp, hl);
-        if (lane == 0 && rower) { s_kv[buf][wave] = kv; s_kp[buf][wave] = kp; }
+        // The best of the waves' keys (larger key, then smaller position). FP32: a key and 2^31 - 1 less the position
+        // pack into 64 bits whose maximum is the best, which each wave adds to an LDS word with one atomic maximum.
+        // FP64 (64-bit keys): lane q takes wave q's key after the barrier, and one more wave reduction picks it.
+        if constexpr (sizeof(T) == 4) {
+            const unsigned long long pk = ((unsigned long long)kv << 32) | (unsigned)(0x7fffffff - kp);
+            if (lane == 0 && rower && pk != 0ull) atomicMax(&L.s_best[buf], pk);
+        } else {
+            if (lane == 0 && rower) { s_kv[buf][wave] = kv; s_kp[buf][wave] = kp; }
+        }
         __syncthreads();
         HS_PHASE(0);
-        // The best of the waves' keys: lane q takes wave q's, and one more wave reduction picks it (larger key, then
-        // smaller position), in every wave.
-        Key bk = (Key)s_kv[buf][lane & (WAVES - 1)];
-        int bkp = s_kp[buf][lane & (WAVES - 1)], bl = 0;
-        bk = lane < WAVES ? bk : Key(0);
-        bkp = lane < WAVES ? bkp : 0x7fffffff;
-        wave_best(bk, bkp, bl);
+        Key bk;
+        int bkp;
+        if constexpr (sizeof(T) == 4) {
+            const unsigned long long pk = L.s_best[buf];
+            bk = (Key)(pk >> 32);
+            bkp = 0x7fffffff - (int)(unsigned)pk;
+        } else {
+            int bl = 0;
+            bk = (Key)s_kv[buf][lane & (WAVES - 1)];
+            bkp = s_kp[buf][lane & (WAVES - 1)];
+            bk = lane < WAVES ? bk : Key(0);
+            bkp = lane < WAVES ? bkp : 0x7fffffff;
+            wave_best(bk, bkp, bl);
+        }
         const unsigned long long bkv = bk;
         if (nwg > 1 && tid == 0) put_key<T>(ex->key[buf][g], tag, bkv, bkp, agent);
 
@@ -507,6 +524,7 @@ __global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict
 
         // The winner takes position c; its row of U is kept in LDS. The row at position c moves to p.
         if (tid < PW) s_u11[c][tid] = piv[tid];
+        if (sizeof(T) == 4 && tid == 0) L.s_best[buf] = 0ull;  // every wave has read it; next used at column c + 2
         if (has && pos == p) pos = c;
         else if (has && pos == c) pos = p;
         if (g == 0 && tid == 0) pivots[c0 + c] = c0 + p;

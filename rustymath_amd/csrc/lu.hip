@@ -438,7 +438,9 @@ __global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict
         // the buffer, and is decoded after it: a key published by then costs no round trip of its own.
         const bool poller = nwg > 1 && wave == POLL_WAVE;
         unsigned long long kw[3] = {0ull, 0ull, 0ull};
-        if (poller) key_words<T>(ex->key[buf][lane < nwg ? lane : 0], kw);
+        // (A polling wave of its own has no update: its keys go out in its polling branch below, so that the load and
+        // its use share one branch.)
+        if (!own_poller<T>() && poller) key_words<T>(ex->key[buf][lane < nwg ? lane : 0], kw);
         T mine = prev[lane];
         asm volatile("" : "+v"(mine));  // loaded here, by every lane, not sunk into the branch below
         if (pending) {
@@ -487,6 +489,7 @@ __global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict
                 unsigned long long gkv = 0ull;
                 int gkp = 0x7fffffff, gl = 0;
                 bool ok = true;
+                if (own_poller<T>()) key_words<T>(ex->key[buf][lane < nwg ? lane : 0], kw);
                 if (lane < nwg && !key_decode<T>(kw, tag, gkv, gkp)) {
                     uint64_t spins = 0;
                     while (!get_key<T>(ex->key[buf][lane], tag, gkv, gkp)) {
@@ -519,8 +522,10 @@ __global__ __launch_bounds__(panel_threads<T>()) void panel_kernel(T* __restrict
         HS_PHASE(3);
         const int p = s_p[buf];
         const T* piv = s_piv[buf];
-        const T pivot = piv[0], piv1 = piv[1], rcp = s_rcp[buf];  // read with the abort word, not behind its test
-        if (s_abort) return;
+        const T pivot = piv[0], piv1 = piv[1], rcp = s_rcp[buf];
+        const int abort_now = s_abort;
+        asm volatile("" ::"v"(p), "v"(pivot), "v"(piv1), "v"(rcp));  // all five reads in one LDS round trip
+        if (abort_now) return;
 
         // The winner takes position c; its row of U is kept in LDS. The row at position c moves to p.
         if (tid < PW) s_u11[c][tid] = piv[tid];

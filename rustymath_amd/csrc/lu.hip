@@ -797,20 +797,22 @@ __global__ __launch_bounds__(256, 2) void trsm_rows_kernel(T* __restrict__ a, in
     const int64_t col0 = cs + (int64_t)blockIdx.x * TCOL;
     const int nkb = (m + 15) / 16;
     T sva[16], svb[16];  // strips in flight: each step issues the loads of the strip two steps ahead
-    // Strip kb: thread t loads row t's 16 items (addresses clamped into the block).
+    // Strip kb (rows 0 .. 255, columns 16 kb .. +15): item i of thread t is row t / 16 + 16 i, column t % 16, so that a
+    // wave's load reads four rows' 16 items (addresses clamped into the block).
     auto load_strip = [&](int kb, T* sv) {
-        const int64_t base = (r0 + (t < m ? t : m - 1)) * lda + r0;
+        const int p = 16 * kb + (t & 15), pc = p < m ? p : m - 1;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int p = 16 * kb + q;
-            sv[q] = a[base + (p < m ? p : m - 1)];
+        for (int i = 0; i < 16; ++i) {
+            const int r = (t >> 4) + 16 * i;
+            sv[i] = a[(r0 + (r < m ? r : m - 1)) * lda + r0 + pc];
         }
     };
     auto store_strip = [&](int kb, const T* sv, int buf) {
+        const int q = t & 15, p = 16 * kb + q;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int p = 16 * kb + q;
-            Lst[buf][t][q] = (t < m && p < t) ? sv[q] : T(0);
+        for (int i = 0; i < 16; ++i) {
+            const int r = (t >> 4) + 16 * i;
+            Lst[buf][r][q] = (r < m && p < r) ? sv[i] : T(0);
         }
     };
     load_strip(0, sva);

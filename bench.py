@@ -19,8 +19,8 @@ PEAK = {'fp64_mfma': 78.6e12, 'fp32_mfma': 157.3e12, 'hbm': 8.0e12}
 # scaled by what the same counter reads on a 1 GiB device copy (`bench.py --copy-probe 1024`: FETCH_SIZE counts half the
 # bytes read, WRITE_SIZE all the bytes written); and the MFMA units' busy fraction (SQ_VALU_MFMA_BUSY_CYCLES over the
 # dispatch's GRBM_GUI_ACTIVE cycles times the 1024 SIMDs, rocprofv3's MfmaUtil) from a third pass.
-TRAFFIC = {('panel_f32', 16384): 136278633, ('trail_f32', 16384): 486234923}
-MFMA_BUSY = {('panel_f32', 16384): 0.2443, ('trail_f32', 16384): 0.6202}
+TRAFFIC = {('panel_f32', 16384): 136345027, ('trail_f32', 16384): 489906952}
+MFMA_BUSY = {('panel_f32', 16384): 0.2441, ('trail_f32', 16384): 0.6246}
 
 
 def host_threads():

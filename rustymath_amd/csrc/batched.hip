@@ -199,17 +199,20 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
     typedef typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned>::type Key;
     __shared__ Key s_wkv[2][WAVES];   // each wave's best candidate, double-buffered by column
     __shared__ int s_wkp[2][WAVES];
-    // Work area: the panel's candidate rows [2][WAVES][BW], its rows of U [BW][BW] and unit-lower triangle
-    // [BW][BW + 1]; or the substitutions' x [NMAX][MAXRHS].
-    constexpr int PANEL_ITEMS = 2 * WAVES * BW + BW * BW + BW * (BW + 1);
-    constexpr int WORK = PANEL_ITEMS > NMAX * MAXRHS ? PANEL_ITEMS : NMAX * MAXRHS;
-    __shared__ T s_work[WORK];
-    T (*s_cand)[WAVES][BW] = reinterpret_cast<T (*)[WAVES][BW]>(s_work);
-    T (*s_u11)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * BW);
+    // The panel's candidate rows [2][WAVES][CW] (a row's panel items, then its right-hand sides), its rows of U
+    // [BW][BW] and unit-lower triangle [BW][BW + 1].
+    constexpr int CW = BW + MAXRHS;
+    constexpr int PANEL_ITEMS = 2 * WAVES * CW + BW * BW + BW * (BW + 1);
+    __shared__ alignas(16) T s_work[PANEL_ITEMS];
+    T (*s_cand)[WAVES][CW] = reinterpret_cast<T (*)[WAVES][CW]>(s_work);
+    T (*s_u11)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * CW);
     // s_l11n[q][j] = −L11[q + 1 + j][q]: column q of the unit-lower triangle below its diagonal, negated and shifted
     // to the row start, so that step q of the U12 solve reads one row from one scalar offset.
-    T (*s_l11n)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * BW + BW * BW);
-    T (*s_x)[MAXRHS] = reinterpret_cast<T (*)[MAXRHS]>(s_work);
+    T (*s_l11n)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * CW + BW * BW);
+    // The substitutions' x by position; while a system is factored and solved in one run, the forward
+    // substitution's partial sums, kept with their rows (see the column loop).
+    __shared__ alignas(16) T s_x[NMAX][MAXRHS];
+    static_assert(CW <= 64, "a candidate row is read one item per lane");
     static_assert(BW == 16 || BW == 32, "panels hold whole 16-column leaves");
     static_assert(BT * R == NMAX && (R == 1 || R == 2), "a workgroup of 512 or 256 threads");
     const T tiny = sizeof(T) == 8 ? T(2.2250738585072014e-308) : T(1.17549435e-38f);
@@ -233,6 +236,11 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
         const Mat<T> A(a, n);
         long long* gperm = perm_all + d.p_off;
         bool solve;
+        // Factored and solved in one run, the forward substitution over L (lu_solve, factor.rs:633-641) rides
+        // along with the factorization: every row carries its x, and at column c each row below the pivot adds
+        // the pivot row's finished x times −l, the term `forward` gives it for that column (factor.rs:668-690),
+        // in the same column order and with the same fused multiply-add.
+        const bool fuse = mode == FACTOR_SOLVE && k > 0;
 
         if (mode == FACTOR_SOLVE) {
             for (int i = tid; i < n; i += BT) {
@@ -242,6 +250,12 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                 while (l1 - l0 > 16) { const int mid = (l0 + l1) / 2; if (i < mid) l1 = mid; else l0 = mid; }
                 leaf_end[i] = (short)l1;
             }
+            // x by position, which is row i's before any pivot: B's rows.
+            if (fuse)
+                for (int i = tid; i < n * MAXRHS; i += BT) {
+                    const int r = i / MAXRHS, c = i % MAXRHS;
+                    s_x[r][c] = c < k ? b[r * k + c] : T(0);
+                }
             __syncthreads();
             if (tid == 0) {
                 // Panels: greedy unions of whole leaves, at most BW columns.
@@ -271,6 +285,12 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
 #pragma unroll
                     for (int j = 0; j < BW; ++j) v[r][j] = A.ld(A.at(has[r] && j < w, row[r] * n + c0 + j));
                 }
+                // The slots' rows' x, where the forward substitution rides along.
+                T x[R][MAXRHS];
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < MAXRHS; ++cc) x[r][cc] = fuse ? s_x[c0 + (has[r] ? pos[r] : 0)][cc] : T(0);
                 // `greatest` magnitude of the matrix, NaN ignored (factor.rs:488), for `regular`: the first panel's
                 // loads, its rows of U right of it and its trailing update read every item of A once, as it was.
                 if (c0 == 0) {
@@ -295,13 +315,29 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     }
                     int hl = 0;
                     wave_best(kv, kp, hl);
-                    if (lane == hl) {
-                        if (kr == 0) {
+                    // The candidate's live items (w − c of them; the rest are never read) and x, from the slot that
+                    // holds it, named by a wave-uniform index so that one store sequence runs without selects.
+                    auto put_cand = [&](const T (&vr)[BW], const T (&xr)[MAXRHS]) {
+                        constexpr int G = 32 / sizeof(T);  // items per pair of 128-bit stores
+                        T* dst = s_cand[buf][wave];
 #pragma unroll
-                            for (int j = 0; j < BW; ++j) s_cand[buf][wave][j] = v[0][j];
-                        } else {
+                        for (int j0 = 0; j0 < BW; j0 += G)
+                            if (j0 < w - c)
 #pragma unroll
-                            for (int j = 0; j < BW; ++j) s_cand[buf][wave][j] = v[R - 1][j];
+                                for (int j = j0; j < j0 + G; ++j) dst[j] = vr[j];
+                        if (fuse) {
+#pragma unroll
+                            for (int j0 = 0; j0 < MAXRHS; j0 += G)
+                                if (j0 < k)
+#pragma unroll
+                                    for (int j = j0; j < j0 + G; ++j) dst[BW + j] = xr[j];
+                        }
+                    };
+                    if (hl >= 0) {
+                        const int ks = R == 1 ? 0 : __builtin_amdgcn_readlane(kr, hl);
+                        if (lane == hl) {
+                            if (ks == 0) put_cand(v[0], x[0]);
+                            else put_cand(v[R - 1], x[R - 1]);
                         }
                     }
                     if (lane == 0) { s_wkv[buf][wave] = kv; s_wkp[buf][wave] = kp; }
@@ -322,7 +358,7 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     const T* piv = s_cand[buf][win];
                     // The pivot row as scalar operands: lane j reads item j (every lane active), then reads go lane
                     // to scalar.
-                    T mine = piv[lane < BW ? lane : 0];
+                    T mine = piv[lane < CW ? lane : 0];
                     asm volatile("" : "+v"(mine));
                     if (tid < BW) s_u11[c][tid] = piv[tid];
                     const T pivot = lane_value(mine, 0);
@@ -354,6 +390,21 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     }
 #pragma unroll
                     for (int r = 0; r < R; ++r) lm[r] = nz ? l[r] : T(0);
+                    // The forward substitution's terms of this column. A zero pivot makes the system irregular, and
+                    // then no x is written.
+                    if (fuse && nz) {
+                        T y[MAXRHS];
+#pragma unroll
+                        for (int cc = 0; cc < MAXRHS; ++cc) y[cc] = cc < k ? lane_value(mine, BW + cc) : T(0);
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            if (upd[r]) {
+#pragma unroll
+                                for (int cc = 0; cc < MAXRHS; ++cc)
+                                    if (cc < k) x[r][cc] = __builtin_elementwise_fma(y[cc], -l[r], x[r][cc]);
+                            }
+                        }
+                    }
                     // The pivot row as the update reads it: +0 throughout for a zero pivot (one select per column, so
                     // that reading an item takes no branch).
                     const T mz = nz ? mine : T(0);
@@ -387,7 +438,13 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                 }
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    if (has[r]) perm[c0 + pos[r]] = (short)row[r];
+                    if (has[r]) {
+                        perm[c0 + pos[r]] = (short)row[r];
+                        if (fuse) {
+#pragma unroll
+                            for (int cc = 0; cc < MAXRHS; ++cc) s_x[c0 + pos[r]][cc] = x[r][cc];
+                        }
+                    }
                 __syncthreads();
                 HS_BPHASE(1);
                 // The panel's rows of U, from LDS to their rows.
@@ -478,14 +535,17 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
             // Item (i, j) of the triangle being solved: F(i, j), or F(j, i) for the transposed solve.
             auto ent_at = [&](bool ok, int i, int j) -> unsigned { return A.at(ok, tr ? perm[j] * n + i : perm[i] * n + j); };
             auto ent = [&](int i, int j) -> T { return A.ld(ent_at(true, i, j)); };
-            for (int i = tid; i < n * MAXRHS; i += BT) {
-                const int r = i / MAXRHS, c = i % MAXRHS;
-                s_x[r][c] = c < k ? b[(tr ? r : perm[r]) * k + c] : T(0);
+            if (!fuse) {
+                for (int i = tid; i < n * MAXRHS; i += BT) {
+                    const int r = i / MAXRHS, c = i % MAXRHS;
+                    s_x[r][c] = c < k ? b[(tr ? r : perm[r]) * k + c] : T(0);
+                }
+                __syncthreads();
             }
-            __syncthreads();
             // Forward: L (unit) or Uᵀ (a reciprocal multiply at each row's turn), increasing order. A diagonal block
-            // per right-hand side, one wave each (in turn where there are more columns than waves).
-            for (int p0 = 0; p0 < n; p0 += SB) {
+            // per right-hand side, one wave each (in turn where there are more columns than waves). Done already
+            // where the factorization carried it.
+            for (int p0 = 0; p0 < (fuse ? 0 : n); p0 += SB) {
                 const int h = n - p0 < SB ? n - p0 : SB;
                 // The block's terms for this thread's first row below it, in flight across the diagonal block.
                 const int i0 = p0 + h + tid, i0c = i0 < n ? i0 : n - 1;

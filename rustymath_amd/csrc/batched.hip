@@ -72,21 +72,22 @@ struct Mat {
     __device__ __forceinline__ unsigned at(bool ok, int i) const { return ok ? (unsigned)i : items; }
 };
 
-// The trailing update of one panel on MFMA: A22 (rows and columns [c1, n)) += (−L21) U12, with L21 the panel's
-// multipliers (columns [c0, c0 + w)) and U12 its rows of U, in 32×32 blocks (2×2 MFMA tiles per wave). With at least
-// as many 32-row blocks as waves, each wave takes whole rows of blocks and keeps its rows' L21 in registers across
-// them (the update reads memory for c and U12 only); otherwise the blocks are dealt out one by one.
-// Where `mx` is not null, it also takes the largest |c| of the items read (NaN ignored): at the first panel, these are
-// the matrix's own items.
+// The trailing update on MFMA: the items at positions [r0, r1) × [q0, q1) take the terms of pivot columns
+// [c0, c0 + w) in order, A22 += (−L21) U12, with L21 the rows' multipliers in those columns and U12 those pivot rows'
+// items of U, in 32×32 blocks (2×2 MFMA tiles per wave). With at least as many 32-row blocks as waves, each wave takes
+// whole rows of blocks and keeps its rows' L21 in registers across them (the update reads memory for c and U12
+// only); otherwise the blocks are dealt out one by one. BW: the most terms one update takes (w <= BW).
+// Where `mx` is not null, it also takes the largest |c| of the items read (NaN ignored): items the first panel's
+// terms reach are the matrix's own items.
 template <typename T, int BW, int WAVES>
-__device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, int w, int c1, const short* perm,
-                                                int wave, int lane, T* mx) {
+__device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, int w, int r0, int r1, int q0, int q1,
+                                                const short* perm, int wave, int lane, T* mx) {
     typedef typename Mfma<T>::acc V;
-    const int m = n - c1, nb = (m + 31) / 32, last = n - 1;
+    const int nbr = (r1 - r0 + 31) / 32, nbc = (q1 - q0 + 31) / 32;
     const int lr = lane & 15, lk = lane >> 4;
-    // The MFMA takes l · (−u): terms outside the matrix or past the panel load as +0 on the L side and −0 on the
-    // U side, so padding adds (+0)·(−0) = −0, and (−u)·l is the reference's term (its α = −1 folds into U12).
-    // All BW / 4 steps run, also past a narrower panel's end (padding then).
+    // The MFMA takes l · (−u): terms outside the block or past w load as +0 on the L side and −0 on the U side, so
+    // padding adds (+0)·(−0) = −0, and (−u)·l is the reference's term (its α = −1 folds into U12). All BW / 4 steps
+    // run, also past a narrower w (padding then).
     T av[BW / 4][2];
     auto load_l = [&](int rb) {  // the rows' L21, every load issued before any is used
         int arow[2];
@@ -94,8 +95,8 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
 #pragma unroll
         for (int ti = 0; ti < 2; ++ti) {
             const int r = rb + 16 * ti + lr;
-            aok[ti] = r < n;
-            arow[ti] = perm[r < n ? r : last] * n + c0;
+            aok[ti] = r < r1;
+            arow[ti] = perm[r < r1 ? r : r0] * n + c0;
         }
 #pragma unroll
         for (int s = 0; s < BW / 4; ++s) {
@@ -113,11 +114,11 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int r = rb + 16 * ti + Mfma<T>::row(lane, e);
-                const int prow = perm[r < n ? r : last] * n;
+                const int prow = perm[r < r1 ? r : r0] * n;
 #pragma unroll
                 for (int tj = 0; tj < 2; ++tj) {
                     const int col = cb + 16 * tj + lr;
-                    cidx[ti][tj][e] = A.at(r < n && col < n, prow + col);
+                    cidx[ti][tj][e] = A.at(r < r1 && col < q1, prow + col);
                 }
             }
 #pragma unroll
@@ -143,7 +144,7 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
 #pragma unroll
             for (int tj = 0; tj < 2; ++tj) {
                 const int col = cb + 16 * tj + lr;
-                bv[s][tj] = -A.ld(A.at(kok && col < n, brow + col));
+                bv[s][tj] = -A.ld(A.at(kok && col < q1, brow + col));
             }
         }
 #pragma unroll
@@ -159,15 +160,15 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
 #pragma unroll
                 for (int e = 0; e < 4; ++e) A.st(cidx[ti][tj][e], acc[ti][tj][e]);
     };
-    if (nb >= WAVES) {
-        for (int rbk = wave; rbk < nb; rbk += WAVES) {
-            load_l(c1 + 32 * rbk);
-            for (int cbk = 0; cbk < nb; ++cbk) block(c1 + 32 * rbk, c1 + 32 * cbk);
+    if (nbr >= WAVES) {
+        for (int rbk = wave; rbk < nbr; rbk += WAVES) {
+            load_l(r0 + 32 * rbk);
+            for (int cbk = 0; cbk < nbc; ++cbk) block(r0 + 32 * rbk, q0 + 32 * cbk);
         }
     } else {
-        for (int blk = wave; blk < nb * nb; blk += WAVES) {
-            load_l(c1 + 32 * (blk / nb));
-            block(c1 + 32 * (blk / nb), c1 + 32 * (blk % nb));
+        for (int blk = wave; blk < nbr * nbc; blk += WAVES) {
+            load_l(r0 + 32 * (blk / nbc));
+            block(r0 + 32 * (blk / nbc), q0 + 32 * (blk % nbc));
         }
     }
 }
@@ -270,6 +271,11 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
 
             // ---- lu_factor ----
             T big = T(0);  // this thread's share of the matrix's greatest magnitude
+            // Panels go in pairs: the first's terms reach only the second's columns before the second is factored;
+            // the second's pivot rows then take them right of it, and the rest of the trailing matrix takes both
+            // panels' terms in one product, read and written once per pair. Every item still takes its terms in
+            // pivot-column order. [dk0, dk1): the pair's first panel while its terms are pending, else dk0 < 0.
+            int dk0 = -1, dk1 = -1;
             for (int c0 = 0; c0 < n;) {
                 const int c1 = panel_end[c0], w = c1 - c0, rows = n - c0;
                 // Slot r of a thread holds the row at panel position tid + r·BT.
@@ -447,6 +453,12 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     }
                 __syncthreads();
                 HS_BPHASE(1);
+                if (dk0 >= 0 && c1 < n) {
+                    // The pair's first panel's terms for this panel's pivot rows, right of it.
+                    trailing_update<T, BW, WAVES>(A, n, dk0, dk1 - dk0, c0, c1, c1, n, perm, wave, lane,
+                                                  dk0 == 0 ? &big : nullptr);
+                    __syncthreads();
+                }
                 // The panel's rows of U, from LDS to their rows.
                 for (int i = tid; i < BW * BW; i += BT) {
                     const int cc = i / BW, j = i % BW;
@@ -502,7 +514,20 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     }
                     __syncthreads();
                     HS_BPHASE(5);
-                    trailing_update<T, BW, WAVES>(A, n, c0, w, c1, perm, wave, lane, c0 == 0 ? &big : nullptr);
+                    const int c2 = panel_end[c1];
+                    if (dk0 < 0 && c2 < n) {
+                        trailing_update<T, BW, WAVES>(A, n, c0, w, c1, n, c1, c2, perm, wave, lane,
+                                                      c0 == 0 ? &big : nullptr);
+                        dk0 = c0;
+                        dk1 = c1;
+                    } else if (dk0 >= 0) {
+                        trailing_update<T, 2 * BW, WAVES>(A, n, dk0, c1 - dk0, c1, n, c1, n, perm, wave, lane,
+                                                          dk0 == 0 ? &big : nullptr);
+                        dk0 = -1;
+                    } else {
+                        trailing_update<T, BW, WAVES>(A, n, c0, w, c1, n, c1, n, perm, wave, lane,
+                                                      c0 == 0 ? &big : nullptr);
+                    }
                 }
                 __syncthreads();
                 HS_BPHASE(2);

@@ -415,13 +415,21 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     // that reading an item takes no branch).
                     const T mz = nz ? mine : T(0);
 #pragma unroll
-                    for (int j0 = 1; j0 < BW; j0 += 16) {
+                    for (int j0 = 1; j0 < BW; j0 += 8) {
                         if (j0 < w - c) {  // items past the panel are never used
 #pragma unroll
-                            for (int j = j0; j < j0 + 16 && j < BW; ++j) {
-                                const T uj = lane_value(mz, j);
+                            for (int j = j0; j < j0 + 8 && j < BW; j += 2) {
+                                // Two items read out into scalars before their FMAs, so that no FMA waits for the
+                                // read just before it.
+                                const T u0 = lane_value(mz, j);
+                                const T u1 = j + 1 < BW ? lane_value(mz, j + 1) : T(0);
+                                asm volatile("" ::"s"(u0), "s"(u1));
 #pragma unroll
-                                for (int r = 0; r < R; ++r) v[r][j - 1] = __builtin_elementwise_fma(lm[r], -uj, v[r][j]);
+                                for (int r = 0; r < R; ++r) v[r][j - 1] = __builtin_elementwise_fma(lm[r], -u0, v[r][j]);
+                                if (j + 1 < BW) {
+#pragma unroll
+                                    for (int r = 0; r < R; ++r) v[r][j] = __builtin_elementwise_fma(lm[r], -u1, v[r][j + 1]);
+                                }
                             }
                         }
                     }

@@ -681,31 +681,36 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     int r0 = 0, r1 = n;
                     while (r1 - r0 > NS) { const int mid = (r0 + r1) / 2; if (end <= mid) r1 = mid; else r0 = mid; }
                     const int h = r1 - r0, off = SB - h;
-                    for (int cc = wave; cc < k; cc += WAVES) {
-                        if (lane < SB) {
-                            const bool act = lane >= off;
-                            const int r = act ? r0 + lane - off : r0;
-                            T x = act ? s_x[r][cc] : T(0);
-                            T mj[SB];
+                    // Each half of a wave solves one column (the leaf's rows right-aligned in its 32 lanes): the
+                    // wave's columns cc and cc + WAVES at once, with the leaf's items loaded once for both.
+                    static_assert(SB == 32, "a leaf's rows fill half a wave");
+                    const int half = lane >> 5, ln = lane & 31;
+                    const bool act = ln >= off;
+                    const int r = act ? r0 + ln - off : r0;
+                    T mj[SB];
 #pragma unroll
-                            for (int j = 0; j < SB; ++j) mj[j] = A.ld(A.at(act && j > lane, perm[r] * n + (j >= off ? r0 + j - off : r0)));
-                            const T dgl = a[perm[r] * n + r];
-                            const T dg = act ? dgl : T(1);
-                            T xs[SB];
+                    for (int j = 0; j < SB; ++j) mj[j] = A.ld(A.at(act && j > ln, perm[r] * n + (j >= off ? r0 + j - off : r0)));
+                    const T dgl = a[perm[r] * n + r];
+                    const T dg = act ? dgl : T(1);
+                    for (int cc0 = wave; cc0 < k; cc0 += 2 * WAVES) {
+                        const int cc = cc0 + half * WAVES;
+                        const bool live = act && cc < k;
+                        T x = live ? s_x[r][cc] : T(0);
+                        T xs[SB];
 #pragma unroll
-                            for (int s = 0; s < SB; ++s) {
-                                const int L = SB - 1 - s;
-                                if (s < h) {
-                                    if (lane == L) {
+                        for (int s = 0; s < SB; ++s) {
+                            const int L = SB - 1 - s;
+                            if (s < h) {
+                                if (ln == L) {
 #pragma unroll
-                                        for (int j = L + 1; j < SB; ++j) x = __builtin_elementwise_fma(xs[j], -mj[j], x);
-                                        x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
-                                    }
-                                    xs[L] = lane_value(x, L);
+                                    for (int j = L + 1; j < SB; ++j) x = __builtin_elementwise_fma(xs[j], -mj[j], x);
+                                    x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
                                 }
+                                const T lo = lane_value(x, L), hi = lane_value(x, 32 + L);
+                                xs[L] = half ? hi : lo;
                             }
-                            if (act) s_x[r][cc] = x;
                         }
+                        if (live) s_x[r][cc] = x;
                     }
                     __syncthreads();
                     int a0 = 0, a1 = n;

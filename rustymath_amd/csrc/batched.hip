@@ -585,25 +585,31 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                 T e0[SB];
 #pragma unroll
                 for (int j = 0; j < SB; ++j) e0[j] = A.ld(ent_at(j < h && i0 < n, i0c, p0 + (j < h ? j : 0)));
-                for (int cc = wave; cc < k; cc += WAVES) {
-                    if (lane < SB) {
-                        const bool act = lane < h;
-                        const int r = p0 + (act ? lane : 0);
-                        T x = act ? s_x[r][cc] : T(0);
-                        T mj[SB];
+                if (wave < k) {
+                    // Each half of a wave solves one column: the wave's columns cc and cc + WAVES at once, with the
+                    // block's items loaded once for both.
+                    const int half = lane >> 5, ln = lane & 31;
+                    const bool act = ln < h;
+                    const int r = p0 + (act ? ln : 0);
+                    T mj[SB];
 #pragma unroll
-                        for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j < lane, r, p0 + (j < h ? j : 0)));
-                        const T dgl = ent(r, r);
-                        const T dg = (tr && act) ? dgl : T(1);
+                    for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j < ln, r, p0 + (j < h ? j : 0)));
+                    const T dgl = ent(r, r);
+                    const T dg = (tr && act) ? dgl : T(1);
+                    for (int cc0 = wave; cc0 < k; cc0 += 2 * WAVES) {
+                        const int cc = cc0 + half * WAVES;
+                        const bool live = act && cc < k;
+                        T x = live ? s_x[r][cc] : T(0);
 #pragma unroll
                         for (int j = 0; j < SB; ++j) {
                             if (j < h) {
-                                if (tr && lane == j) x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
-                                const T xj = lane_value(x, j);
-                                if (act && lane > j) x = __builtin_elementwise_fma(xj, -mj[j], x);
+                                if (tr && ln == j) x = fabs(dg) >= tiny ? x * (T(1) / dg) : x / dg;
+                                const T lo = lane_value(x, j), hi = lane_value(x, 32 + j);
+                                const T xj = half ? hi : lo;
+                                if (act && ln > j) x = __builtin_elementwise_fma(xj, -mj[j], x);
                             }
                         }
-                        if (act) s_x[r][cc] = x;
+                        if (live) s_x[r][cc] = x;
                     }
                 }
                 __syncthreads();
@@ -634,26 +640,30 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     T e0[SB];
 #pragma unroll
                     for (int j = 0; j < SB; ++j) e0[j] = A.ld(ent_at(j < h && tid < q0, i0c, q0 + (j < h ? j : 0)));
-                    for (int cc = wave; cc < k; cc += WAVES) {
-                        if (lane < SB) {
-                            const bool act = lane >= off;
-                            const int r = act ? q0 + lane - off : q0;
-                            T x = act ? s_x[r][cc] : T(0);
-                            T mj[SB];
+                    if (wave < k) {  // two columns per wave, one per half, as in the forward substitution
+                        const int half = lane >> 5, ln = lane & 31;
+                        const bool act = ln >= off;
+                        const int r = act ? q0 + ln - off : q0;
+                        T mj[SB];
 #pragma unroll
-                            for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j > lane, r, j >= off ? q0 + j - off : q0));
-                            const T dgl = ent(r, r);
-                            const T dg = (!tr && act) ? dgl : T(1);
+                        for (int j = 0; j < SB; ++j) mj[j] = A.ld(ent_at(act && j > ln, r, j >= off ? q0 + j - off : q0));
+                        const T dgl = ent(r, r);
+                        const T dg = (!tr && act) ? dgl : T(1);
+                        for (int cc0 = wave; cc0 < k; cc0 += 2 * WAVES) {
+                            const int cc = cc0 + half * WAVES;
+                            const bool live = act && cc < k;
+                            T x = live ? s_x[r][cc] : T(0);
 #pragma unroll
                             for (int s = 0; s < SB; ++s) {
                                 const int L = SB - 1 - s;
                                 if (s < h) {
-                                    if (!tr && lane == L) x = x / dg;
-                                    const T xl = lane_value(x, L);
-                                    if (act && lane < L) x = __builtin_elementwise_fma(xl, -mj[L], x);
+                                    if (!tr && ln == L) x = x / dg;
+                                    const T lo = lane_value(x, L), hi = lane_value(x, 32 + L);
+                                    const T xl = half ? hi : lo;
+                                    if (act && ln < L) x = __builtin_elementwise_fma(xl, -mj[L], x);
                                 }
                             }
-                            if (act) s_x[r][cc] = x;
+                            if (live) s_x[r][cc] = x;
                         }
                     }
                     __syncthreads();
@@ -689,7 +699,7 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
                     const int r = act ? r0 + ln - off : r0;
                     T mj[SB];
 #pragma unroll
-                    for (int j = 0; j < SB; ++j) mj[j] = A.ld(A.at(act && j > ln, perm[r] * n + (j >= off ? r0 + j - off : r0)));
+                    for (int j = 0; j < SB; ++j) mj[j] = A.ld(A.at(wave < k && act && j > ln, perm[r] * n + (j >= off ? r0 + j - off : r0)));
                     const T dgl = a[perm[r] * n + r];
                     const T dg = act ? dgl : T(1);
                     for (int cc0 = wave; cc0 < k; cc0 += 2 * WAVES) {

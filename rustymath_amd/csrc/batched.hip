@@ -782,6 +782,26 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// v + (v of the lane CTRL names) over the lanes ROWS enables, 0 from lanes with no source (a DPP step of a wave sum).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, ROWS, 0xf, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, ROWS, 0xf, true);
+    return v + __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The sum of v over a wave's 64 lanes, in lane 63: shifts by 1, 2, 4 and 8 within each 16-lane row, then lane 15 of
+// each row into the next row and lane 31 into the upper half (DPP, no LDS).
+__device__ __forceinline__ double wave_sum63(double v) {
+    v = dpp_add<0x111, 0xf>(v);
+    v = dpp_add<0x112, 0xf>(v);
+    v = dpp_add<0x114, 0xf>(v);
+    v = dpp_add<0x118, 0xf>(v);
+    v = dpp_add<0x142, 0xa>(v);
+    return dpp_add<0x143, 0xc>(v);
+}
+
 // The power of two that brings a positive finite v to [1, 2), or 1.
 __device__ __forceinline__ double pow2_scale(double v) {
     if (!(v > 0) || !(v <= 1.7976931348623157e308)) return 1.0;
@@ -843,30 +863,54 @@ __global__ __launch_bounds__(IRT) void ir_step_kernel(int count, const Desc* __r
         xs[i / k][i % k] = v;
     }
     __syncthreads();
-    // R = B - A X: one wave per row, lanes along the row. The residual goes to r32 as FP32 at once (its scaling by a
-    // power of two afterwards is exact), its squares into rr.
+    // R = B - A X: one wave per pair of rows, lanes along the rows, every item of both rows loaded before the sums
+    // (n <= 512: eight per lane and row), then wave sums by DPP. The residual goes to r32 as FP32 at once (its scaling
+    // by a power of two afterwards is exact), its squares into rr.
     const double* A = a + d.a_off;
     double rr = 0, xx = 0;
-    for (int row = wave; row < n; row += IRT / 64) {
-        double acc[MAXRHS];
+    constexpr int PER = NMAX / 64;
+    for (int row0 = 2 * wave; row0 < n; row0 += 2 * (IRT / 64)) {
+        const int row1 = row0 + 1 < n ? row0 + 1 : row0;
+        const double* A0 = A + (long long)row0 * n;
+        const double* A1 = A + (long long)row1 * n;
+        double a0[PER], a1[PER];
 #pragma unroll
-        for (int c = 0; c < MAXRHS; ++c) acc[c] = 0;
-        for (int j = lane; j < n; j += 64) {
-            const double aij = A[(long long)row * n + j];
+        for (int t = 0; t < PER; ++t) {
+            const int j = lane + 64 * t;
+            a0[t] = j < n ? A0[j] : 0.0;
+            a1[t] = j < n ? A1[j] : 0.0;
+        }
+        double acc[2][MAXRHS];
 #pragma unroll
-            for (int c = 0; c < MAXRHS; ++c)
-                if (c < k) acc[c] = __builtin_elementwise_fma(aij, xs[j][c], acc[c]);
+        for (int c = 0; c < MAXRHS; ++c) acc[0][c] = acc[1][c] = 0;
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            const int j = lane + 64 * t;
+            if (j < n) {
+#pragma unroll
+                for (int c = 0; c < MAXRHS; ++c) {
+                    if (c < k) {
+                        const double xj = xs[j][c];
+                        acc[0][c] = __builtin_elementwise_fma(a0[t], xj, acc[0][c]);
+                        acc[1][c] = __builtin_elementwise_fma(a1[t], xj, acc[1][c]);
+                    }
+                }
+            }
         }
 #pragma unroll
-        for (int c = 0; c < MAXRHS; ++c) {
-            if (c < k) {
-                double v = acc[c];
+        for (int q = 0; q < 2; ++q) {
+            const int row = q ? row1 : row0;
+            if (q == 0 || row1 != row0) {
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (lane == 0) {
-                    const double r = b[d.b_off + (long long)row * k + c] - v;
-                    r32[d.b_off + (long long)row * k + c] = (float)r;
-                    rr = __builtin_elementwise_fma(r, r, rr);
+                for (int c = 0; c < MAXRHS; ++c) {
+                    if (c < k) {
+                        const double v = wave_sum63(acc[q][c]);
+                        if (lane == 63) {
+                            const double r = b[d.b_off + (long long)row * k + c] - v;
+                            r32[d.b_off + (long long)row * k + c] = (float)r;
+                            rr = __builtin_elementwise_fma(r, r, rr);
+                        }
+                    }
                 }
             }
         }

@@ -14,12 +14,15 @@
 //     own leaf;
 //   - right of a panel, each thread solves one column's U rows with the panel's unit-lower triangle, in
 //     order, and the trailing rows take the panel's product on MFMA: each wave owns 32×32 blocks and adds
-//     the panel's terms four at a time. gfx950's FP64 and FP32 16x16x4 MFMA rounds every step of its
-//     four-term sum as a fused multiply-add in order of k (measured), so the product is the in-order fold;
-//     padding terms are (+0)·(−0) = −0, which leaves every sum unchanged;
-//   - the solve substitutes in blocks of 32 rows: one wave per right-hand side solves a diagonal block
-//     with register broadcasts, then every row outside it adds the block's terms, in the reference's three
-//     orders (csrc/solve.hip has the same three) — and the transposed solve for adjoints.
+//     the terms four at a time. Panels go in pairs, so that the rest of the trailing matrix takes both
+//     panels' terms in one product. gfx950's FP64 and FP32 16x16x4 MFMA rounds every step of its four-term
+//     sum as a fused multiply-add in order of k (measured), so the product is the in-order fold; padding
+//     terms are (+0)·(−0) = −0, which leaves every sum unchanged;
+//   - factored and solved in one run, the forward substitution happens in the column loop (each row carries
+//     its x); the backward substitution, and both where earlier factors are reused, go in blocks of 32 rows:
+//     each half of a wave solves a diagonal block for one right-hand side with register broadcasts, then
+//     every row outside it adds the block's terms, in the reference's three orders (csrc/solve.hip has the
+//     same three) — and the transposed solve for adjoints.
 //
 // A is overwritten by its factors without moving rows: row perm[i] of the matrix holds position i of the
 // factors, where perm is rustymath's `perm` (factor.rs:132-134). B is overwritten by X where the system is
@@ -187,7 +190,8 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
     constexpr int WAVES = BT / 64, R = NMAX / BT;  // R: a panel's rows per thread
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Where `prof` is not null, thread 0 of workgroup 0 adds each phase's 100 MHz ticks into prof[0..5] and counts
-    // its systems in prof[7]: setup, panel columns, trailing update, regularity + forward, backward, U rows.
+    // its systems in prof[7]: setup, panel columns, trailing update, regularity + forward (the forward substitution
+    // only where it did not ride along with the factorization), backward, U rows.
     const bool timed = prof != nullptr && blockIdx.x == 0 && tid == 0;
     long long tl = timed ? (long long)__builtin_amdgcn_s_memrealtime() : 0;
 #define HS_BPHASE(q) do { if (timed) { long long t_ = __builtin_amdgcn_s_memrealtime(); prof[q] += t_ - tl; tl = t_; } } while (0)

@@ -181,8 +181,12 @@ __device__ __forceinline__ void trailing_update(const Mat<T>& A, int n, int c0, 
 // latency-bound pivot steps beside the other's products). Both are tuning choices; the results depend on neither.
 // `mode`: factor and solve (perm and regular written), or solve A X = B / Aᵀ X = B with the factors and perm of an
 // earlier run. Where `active` is not null, a system whose flag is 0 is skipped (the refinement's converged systems).
-template <typename T, int BW, int BT>
-__global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const Desc* __restrict__ desc, const int* __restrict__ order,
+// SOLVE_ONLY: the form for the solve modes, compiled without the factorization, so that its registers leave room for
+// more waves per SIMD in the latency-bound substitutions: SOLVE_WGS 256-thread workgroups (systems) per CU.
+template <typename T>
+constexpr int solve_wgs() { return sizeof(T) == 8 ? 3 : 4; }
+template <typename T, int BW, int BT, bool SOLVE_ONLY = false>
+__global__ __launch_bounds__(BT, SOLVE_ONLY ? solve_wgs<T>() : 2) void gesv_batched_kernel(int count, const Desc* __restrict__ desc, const int* __restrict__ order,
                                                           T* __restrict__ a_all, T* __restrict__ b_all,
                                                           long long* __restrict__ perm_all, int* __restrict__ regular_all,
                                                           const int* __restrict__ active, int* next, int mode,
@@ -208,7 +212,7 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
     // [BW][BW] and unit-lower triangle [BW][BW + 1].
     constexpr int CW = BW + MAXRHS;
     constexpr int PANEL_ITEMS = 2 * WAVES * CW + BW * BW + BW * (BW + 1);
-    __shared__ alignas(16) T s_work[PANEL_ITEMS];
+    __shared__ alignas(16) T s_work[SOLVE_ONLY ? 16 : PANEL_ITEMS];
     T (*s_cand)[WAVES][CW] = reinterpret_cast<T (*)[WAVES][CW]>(s_work);
     T (*s_u11)[BW] = reinterpret_cast<T (*)[BW]>(s_work + 2 * WAVES * CW);
     // s_l11n[q][j] = −L11[q + 1 + j][q]: column q of the unit-lower triangle below its diagonal, negated and shifted
@@ -245,9 +249,13 @@ __global__ __launch_bounds__(BT, 2) void gesv_batched_kernel(int count, const De
         // along with the factorization: every row carries its x, and at column c each row below the pivot adds
         // the pivot row's finished x times −l, the term `forward` gives it for that column (factor.rs:668-690),
         // in the same column order and with the same fused multiply-add.
-        const bool fuse = mode == FACTOR_SOLVE && k > 0;
+        const bool fuse = !SOLVE_ONLY && mode == FACTOR_SOLVE && k > 0;
 
-        if (mode == FACTOR_SOLVE) {
+        if constexpr (SOLVE_ONLY) {
+            for (int i = tid; i < n; i += BT) perm[i] = (short)gperm[i];
+            __syncthreads();
+            solve = n > 0 && k > 0;
+        } else if (mode == FACTOR_SOLVE) {
             for (int i = tid; i < n; i += BT) {
                 perm[i] = (short)i;
                 // The reference leaf holding column i: lu_panel's halving while wider than 16 (factor.rs:592-616).
@@ -979,6 +987,13 @@ long long g_next_plan = 1;
 template <typename T>
 int enqueue_mode(Plan* p, int mode, const int* active, hipStream_t s, long long* prof) {
     zero_int<<<1, 1, 0, s>>>(p->d_next);
+    if (mode != FACTOR_SOLVE) {
+        // The solve modes: three (FP64) or four (FP32) 256-thread workgroups per CU (the result depends on neither).
+        gesv_batched_kernel<T, 32, 256, true><<<dim3(device_cus() * solve_wgs<T>()), 256, 0, s>>>(
+            p->count, p->d_desc, p->d_order, (T*)p->a, (T*)p->b, p->perm, p->regular, active, p->d_next, mode, prof);
+        HS_HIP(hipGetLastError());
+        return OK;
+    }
     // One workgroup of 512 threads per CU, or two of 256.
     const dim3 grid(p->grid);
 #define HS_BLAUNCH(bw, bt) gesv_batched_kernel<T, bw, bt><<<grid, bt, 0, s>>>(p->count, p->d_desc, p->d_order, (T*)p->a, \
